@@ -1,0 +1,32 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C-ABI)")
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def small_mesh():
+    from lionbolt_amd import mesh
+    return mesh.jittered_cube(3, a=2.0, seed=1001)
+
+
+@pytest.fixture(scope="session")
+def cfg1_mesh():
+    from lionbolt_amd import mesh
+    return mesh.jittered_cube(20, a=10.0, seed=1001)

@@ -1,0 +1,149 @@
+"""Pins for the oracle's scattering product and Krylov solvers.
+
+Scattering (ScatteringAlgebra.f90:13-55, SNPrep.f90:106-133):
+* addition theorem: YY_l = w (2l+1)/(4 pi) P_l(k.k') equals M_l D_l built from
+  real spherical harmonics (associated Legendre + trigonometric azimuth, an
+  independent construction);
+* P0: K psi = sigma_0 phi / (4 pi) with phi = sum_i w_i psi_i;
+* ScatterMMS (validation/operators/ScatterMMS.f90:33-128): psi = exp(r.k) with
+  sigma_l = 2 pi int exp(mu) P_l(mu) dmu (zero for l >= 14) gives
+  K psi = 2 pi (exp(|r+k|) - exp(-|r+k|)) / |r+k|.
+Krylov (iters.f:1901-2206, SourceIteration.f90:3-82):
+* the converged GMRES and SI solutions equal a dense direct solve;
+* the Arnoldi residual estimates of the first GMRES cycle equal the minimal
+  residual over the Krylov space computed by dense least squares;
+* the Givens rotation zeroes the second component (Golub & van Loan).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from lionbolt_amd import angular, mesh as M
+from oracle.oracle import OracleOp
+
+
+def test_addition_theorem_yy_equals_moments(small_mesh):
+    for ang in (angular.snprep(3), angular.product(4, 8, 3), angular.level_symmetric_s8(3)):
+        op = OracleOp(small_mesh, ang)
+        yy = op.yy()
+        Y = angular.real_sph_harm(ang.L, ang.k)
+        deg = angular.moment_degree(ang.L)
+        for l in range(ang.L + 1):
+            rows = deg == l
+            via_moments = (ang.w[:, None] * Y[rows].T) @ Y[rows]   # [ip, i]
+            assert np.abs(yy[l] - via_moments).max() <= 1e-14
+
+
+def test_p0_scattering_closed_form(small_mesh):
+    ang = angular.level_symmetric_s8()
+    op = OracleOp(small_mesh, ang)
+    op.set_physics([1.0], [[0.9]])
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal(op.n)
+    k = op.scatter_matvec(psi).reshape(ang.NI, -1)
+    phi = (ang.w[:, None] * psi.reshape(ang.NI, -1)).sum(axis=0)
+    assert np.abs(k - 0.9 * phi / (4 * math.pi)).max() <= 1e-14 * np.abs(k).max()
+
+
+def test_two_material_scattering(small_mesh):
+    m = M.TetMesh(small_mesh.rg, small_mesh.node, np.where(small_mesh.centroids()[:, 2] > 1.0, 2, 1))
+    ang = angular.snprep(2)
+    op = OracleOp(m, ang)
+    sigs = np.array([[0.5, 0.2, 0.1], [2.0, 0.7, 0.3]])
+    op.set_physics([1.0, 3.0], sigs)
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal(op.n).reshape(ang.NI, m.NE, 4)
+    out = op.scatter_matvec(psi).reshape(ang.NI, m.NE, 4)
+    D, Mm = angular.moment_matrices(ang)
+    deg = angular.moment_degree(ang.L)
+    phi = np.einsum("ai,iek->aek", D, psi)
+    sig = sigs[m.mat - 1][:, deg].T                      # (moments, NE)
+    expect = np.einsum("ia,aek->iek", Mm, phi * sig[:, :, None])
+    assert np.abs(out - expect).max() <= 1e-13 * np.abs(expect).max()
+
+
+def _scatter_moment(l):
+    if l >= 14:
+        return 0.0
+    x, w = np.polynomial.legendre.leggauss(60)
+    return 2 * math.pi * float(np.sum(w * np.exp(x) * angular.legendre(l, x)[l]))
+
+
+def test_scatter_mms():
+    m = M.jittered_cube(2, a=2.0, seed=1001)
+    m.rg -= 1.0
+    L = 15
+    ang = angular.snprep(L)
+    op = OracleOp(m, ang)
+    op.set_physics([1.0], [[_scatter_moment(l) for l in range(L + 1)]])
+    r = m.dof_coords()
+    psi = np.exp(ang.k @ r.T)                      # (NI, NENK)
+    out = op.scatter_matvec(psi).reshape(psi.shape)
+    rho = np.linalg.norm(r[None, :, :] + ang.k[:, None, :], axis=2)
+    exact = np.where(rho == 0, 4 * math.pi, 2 * math.pi * (np.exp(rho) - np.exp(-rho)) / np.where(rho == 0, 1, rho))
+    assert np.abs(out - exact).max() <= 1e-9 * np.abs(exact).max()
+
+
+@pytest.fixture(scope="module")
+def tiny_problem():
+    m = M.jittered_cube(1, a=1.0, seed=1001)
+    ang = angular.snprep(1)
+    op = OracleOp(m, ang)
+    op.set_physics([1.0], [[0.8, 0.3]])
+    n = op.n
+    A = np.empty((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = op.boltzmann_matvec(e)
+    rng = np.random.default_rng(4)
+    q = rng.random(n)
+    b = op.transport_matinv(q)
+    return op, A, q, b
+
+
+def test_gmres_and_si_match_dense_solve(tiny_problem):
+    op, A, q, b = tiny_problem
+    xd = np.linalg.solve(A, b)
+    xg, st = op.boltzmann_matinv(q, kind="gmres", m=10)
+    assert st["status"] == 0
+    assert np.abs(xg - xd).max() <= 1e-7 * np.abs(xd).max()
+    xs, st2 = op.boltzmann_matinv(q, kind="si")
+    assert st2["status"] == 0
+    assert np.abs(xs - xd).max() <= 1e-7 * np.abs(xd).max()
+    # the reference's convergence test holds for the true residual too
+    assert np.linalg.norm(b - A @ xg) <= 1e-8 * np.linalg.norm(b) + 1e-12 + 1e-14
+
+
+def test_gmres_estimates_equal_minimal_residual(tiny_problem):
+    op, A, q, b = tiny_problem
+    x0 = b / op.sigt[0]
+    m = 6
+    _, st = op.gmres(b, x0, m=m, maxit=m + 1)
+    r0 = b - A @ x0
+    K = [r0 / np.linalg.norm(r0)]
+    for _ in range(m):
+        K.append(A @ K[-1])
+    hist = st["hist"]
+    # hist[0] = 0 (bisinit), hist[1] = |r0|, hist[k+1] = estimate after k Arnoldi steps
+    assert abs(hist[1] - np.linalg.norm(r0)) <= 1e-14 * np.linalg.norm(r0)
+    for k in range(1, m):
+        Q, _ = np.linalg.qr(np.stack(K[:k], axis=1))
+        W = A @ Q
+        y, *_ = np.linalg.lstsq(W, r0, rcond=None)
+        rmin = np.linalg.norm(r0 - W @ y)
+        assert abs(hist[k + 1] - rmin) <= 1e-9 * np.linalg.norm(r0)
+
+
+def test_gmres_iteration_limit_and_breakdown(small_mesh):
+    ang = angular.snprep(1)
+    op = OracleOp(small_mesh, ang)
+    op.set_physics([1.0], [[0.95, 0.0]])
+    q = np.ones(op.n)
+    _, st = op.boltzmann_matinv(q, kind="gmres", m=5, maxit=3)
+    assert st["status"] == -1
+    # no scattering: A = I, the first Arnoldi vector is already in the span -> breakdown (-3)
+    op.set_physics([1.0], [[0.0, 0.0]])
+    _, st = op.boltzmann_matinv(q, kind="gmres", m=5)
+    assert st["status"] in (0, -3)
