@@ -1,0 +1,164 @@
+/*
+ * lionbolt.h - C ABI of the MI355X transport-solve library (liblbhip.so).
+ *
+ * The entry points replace the type-bound procedures of the reference's
+ * abstract LinearOperator stack (src/operators/LinearOperatorClass.f90:43-112,
+ * re-exported by src/LionboltAPI.f90:1-11).  A Fortran caller binds them with
+ * iso_c_binding (lionbolt_amd/fortran/LionboltHIP.f90); other callers use the
+ * same symbols through ctypes or plain C.
+ *
+ * Conventions
+ *  - Host pointers, FP64, synchronous on return, unless the name ends in _dev.
+ *  - A space-angle vector uses the reference layout v[sdof + i*NENK], 0-based,
+ *    sdof = 4*e + k (LinearOperatorClass.f90:30-41,140,166-172).
+ *  - _dev entry points take device pointers in the group-batched layout
+ *    v[((i*NE + e)*4 + k)*G + g] (for G = 1 this is the reference layout) and
+ *    run on the operator's HIP stream; they return before the work completes
+ *    unless stated otherwise.
+ *  - Every entry point returns 0 on success or a negative LB_E* code, and
+ *    lb_last_error() describes the last failure of the calling thread.  The
+ *    reference stops the process instead (stophere); the Fortran shim keeps
+ *    that behaviour by calling stophere(lb_last_error()) on a non-zero code.
+ *  - One handle per operator; a handle is not thread-safe.
+ */
+#ifndef LIONBOLT_H
+#define LIONBOLT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LB_OK = 0,
+  LB_E_ARG = -1,        /* invalid argument or size mismatch */
+  LB_E_STAGNANT = -2,   /* SweepOrder.f90:178-184 "Sweep is stagnant" */
+  LB_E_NOT_READY = -3,  /* operator used before Build (BoltzmannOperator.f90:73-74) */
+  LB_E_HIP = -4,        /* HIP runtime failure */
+  LB_E_NOMEM = -5,      /* device allocation failed */
+  LB_E_NAN = -6,        /* "RESIDUAL NORM is NaN." (SPARSKIT.f90:123-125) */
+  LB_E_MAXIT = -7,      /* GMRES ipar(1) = -1 (SPARSKIT.f90:159-162) */
+  LB_E_BREAKDOWN = -8   /* GMRES ipar(1) = -3 (SPARSKIT.f90:166-169) */
+};
+
+enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
+enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
+enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
+
+typedef struct lb_op lb_op;
+
+/* MeshClass fields the operators read (Geometry.f90:25-69).  Arrays are
+ * borrowed for the duration of lb_op_create only. */
+typedef struct {
+  int64_t NE;            /* 4-node tetrahedra */
+  int64_t NK;            /* global nodes */
+  const double* rg;      /* rg(3,NK): x,y,z of each node, node-major */
+  const int64_t* node;   /* node(4,NE): global nodes of each element */
+  const int64_t* nbr;    /* neighbors(4,NE) across local faces [1,2,3],[1,2,4],[1,3,4],[2,3,4] */
+  const int64_t* mat;    /* el%mat: material of each element */
+  int index_base;        /* 1: Fortran indices, boundary neighbour 0; 0: C indices, boundary -1 */
+} lb_mesh_desc;
+
+/* AngularClass fields the operators read (AngularSpace.f90:5-20). */
+typedef struct {
+  int64_t NI;            /* directions */
+  const double* k;       /* k(3,NI) unit propagation directions */
+  const double* w;       /* w(NI) weights (sum 4 pi) */
+  int L;                 /* scattering order: YY(:,:,0:L) */
+} lb_quad_desc;
+
+typedef struct {
+  int kind;              /* LB_SOLVER_GMRES (reference default), LB_SOLVER_SI */
+  int space;             /* LB_SPACE_PSI: Krylov on psi as the reference; LB_SPACE_MOMENT: on flux moments */
+  int restart;           /* GMRES(m); the reference hard-codes 10 (BoltzmannOperator.f90:135) */
+  int maxit;             /* <= 0: no limit (SPARSKIT.f90:89) */
+  double rtol, atol;     /* 1e-8, 1e-12 (SPARSKIT.f90:100-101, SourceIteration.f90:14-15) */
+  int verbose;           /* 1: print the reference's residual table to stdout */
+} lb_solver_opts;
+
+typedef struct {
+  int status;            /* 0 converged, else a negative LB_E* code */
+  int iterates;          /* the reference's "Total number of iterates" */
+  int matvecs;           /* operator applications inside the solver */
+  int sweeps;            /* transport sweeps including the T^-1 q of MatInv */
+  double final_norm;     /* last printed residual norm (Arnoldi estimate / SI increment) */
+  double true_residual;  /* ||T^-1 q - (I - T^-1 K) psi|| / ||T^-1 q||, recomputed in FP64 */
+  double seconds;        /* wall time of the solve */
+} lb_solve_stats;
+
+/* Op%Build(mesh) + Op%Build(mesh, angular): normals, geometry and the upwind
+ * schedule (TransportOperator.f90:172-388, ScatteringOperator.f90:100-213,
+ * BoltzmannOperator.f90:171-296). */
+int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out);
+
+/* Op%Build(angular, XS): sigma_t(Nm) and Legendre moments sigma_s(0:L, Nm)
+ * stored [m][l] (TransportOperator.f90:390-429, ScatteringOperator.f90:215-277). */
+int lb_op_set_physics(lb_op* op, int Nm, const double* sigma_t, int L, const double* sigma_s);
+
+/* Multigroup physics for the batched device entry points: sigma_t[g][m],
+ * within-group sigma_s[g][m][l]. */
+int lb_op_set_physics_groups(lb_op* op, int G, int Nm, const double* sigma_t, int L, const double* sigma_s);
+
+/* Scattering kernel choice: LB_SCATTER_DENSE applies YYS(NI,NI) as the
+ * reference does (ScatteringAlgebra.f90:30-53); LB_SCATTER_MOMENT applies the
+ * identical K = M diag(sigma_l) D (default). */
+int lb_op_set_scatter_mode(lb_op* op, int mode);
+
+/* TransportOp%MatInv (TransportOperator.f90:72-151): s <- T^-1 s in place. */
+int lb_transport_matinv(lb_op* op, double* s, int64_t n);
+
+/* ScatteringOp%MatVec (ScatteringOperator.f90:39-70): x <- K x in place. */
+int lb_scatter_matvec(lb_op* op, double* x, int64_t n);
+
+/* BoltzmannOp%MatVec (BoltzmannOperator.f90:59-104): x <- x - T^-1 K x. */
+int lb_boltzmann_matvec(lb_op* op, double* x, int64_t n);
+
+/* BoltzmannOp%MatInv (BoltzmannOperator.f90:106-151): s (source q) <- psi. */
+int lb_boltzmann_matinv(lb_op* op, double* s, int64_t n, const lb_solver_opts* opts, lb_solve_stats* stats);
+
+/* SpaceAngleVector%Fluence (LinearOperatorClass.f90:154-174): phi(NENK) = sum_i w_i v_i. */
+int lb_fluence(lb_op* op, const double* v, double* phi);
+
+/* Schedule of SweepOrder.f90:105-207 for parity checks: sl[i][es] (0-based
+ * element visited at position es for direction i) and nlev[i]. */
+int lb_op_get_schedule(lb_op* op, int32_t* sl, int32_t* nlev);
+
+/* The host half of lb_op_create without a device: the schedule sl[i][es],
+ * nlev[i] and normals n[e][f][3] (any output may be NULL).  Lets the ordering
+ * be checked where no GPU is present. */
+int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t* sl, int32_t* nlev, double* normals);
+
+/* Outward unit normals n[e][f][3] as PostProcess.f90:48-102 computes them. */
+int lb_op_get_normals(lb_op* op, double* n);
+
+/* Sizes: NE, NI, number of moments (L+1)^2, max level count. */
+int lb_op_info(lb_op* op, int64_t* NE, int64_t* NI, int64_t* nmom, int64_t* maxlev);
+
+/* ---- device-resident entry points (no reference counterpart: they expose
+ * the same operators on HBM-resident data so callers skip the PCIe copies) -- */
+
+/* T^-1 over G groups starting at group g0: psi = T^-1 q (in place allowed). */
+int lb_transport_matinv_dev(lb_op* op, const double* d_q, double* d_psi, int g0, int G);
+
+/* phi[lm][sdof][g] = sum_i D[lm][i] psi[i][sdof][g] (discrete to moment). */
+int lb_moments_dev(lb_op* op, const double* d_psi, double* d_phi, int G);
+
+/* psi[i][sdof][g] = sum_lm M[i][lm] sigma_l(mat,g) phi[lm][sdof][g] (+ psi if accumulate). */
+int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, int g0, int G, int accumulate);
+
+/* Block until the operator's stream is idle. */
+int lb_op_synchronize(lb_op* op);
+
+/* Timing of the most recent device sweep, measured with HIP events on the
+ * operator's stream around the level launches: milliseconds. */
+int lb_last_sweep_ms(lb_op* op, double* ms);
+
+void lb_op_destroy(lb_op* op);
+const char* lb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIONBOLT_H */

@@ -1,0 +1,112 @@
+"""ctypes binding of liblbhip.so (the C ABI declared in include/lionbolt.h).
+
+The library is loaded from this package directory only; a missing library is a
+hard error, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbhip.so")
+
+LB_OK = 0
+ERRORS = {-1: "LB_E_ARG", -2: "LB_E_STAGNANT", -3: "LB_E_NOT_READY", -4: "LB_E_HIP", -5: "LB_E_NOMEM",
+          -6: "LB_E_NAN", -7: "LB_E_MAXIT", -8: "LB_E_BREAKDOWN"}
+SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
+SPACE_PSI, SPACE_MOMENT = 0, 1
+SCATTER_MOMENT, SCATTER_DENSE = 0, 1
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("NE", ctypes.c_int64), ("NK", ctypes.c_int64), ("rg", _d), ("node", _i64), ("nbr", _i64),
+                ("mat", _i64), ("index_base", ctypes.c_int)]
+
+
+class QuadDesc(ctypes.Structure):
+    _fields_ = [("NI", ctypes.c_int64), ("k", _d), ("w", _d), ("L", ctypes.c_int)]
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("space", ctypes.c_int), ("restart", ctypes.c_int), ("maxit", ctypes.c_int),
+                ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterates", ctypes.c_int), ("matvecs", ctypes.c_int),
+                ("sweeps", ctypes.c_int), ("final_norm", ctypes.c_double), ("true_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol the header declares
+SIGNATURES = {
+    "lb_op_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), ctypes.POINTER(_vp)]),
+    "lb_op_set_physics": (ctypes.c_int, [_vp, ctypes.c_int, _d, ctypes.c_int, _d]),
+    "lb_op_set_physics_groups": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d]),
+    "lb_op_set_scatter_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lb_transport_matinv": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),
+    "lb_scatter_matvec": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),
+    "lb_boltzmann_matvec": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),
+    "lb_boltzmann_matinv": (ctypes.c_int, [_vp, _d, ctypes.c_int64, ctypes.POINTER(SolverOpts),
+                                           ctypes.POINTER(SolveStats)]),
+    "lb_fluence": (ctypes.c_int, [_vp, _d, _d]),
+    "lb_op_get_schedule": (ctypes.c_int, [_vp, _i32, _i32]),
+    "lb_schedule_host": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), _i32, _i32, _d]),
+    "lb_op_get_normals": (ctypes.c_int, [_vp, _d]),
+    "lb_op_info": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64]),
+    "lb_transport_matinv_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "lb_moments_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_moments_to_discrete_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "lb_op_synchronize": (ctypes.c_int, [_vp]),
+    "lb_last_sweep_ms": (ctypes.c_int, [_vp, _d]),
+    "lb_op_destroy": (None, [_vp]),
+    "lb_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+class LionboltError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(make -C lionbolt_amd/csrc). There is no CPU fallback.")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != LB_OK:
+        raise LionboltError(rc, lib().lb_last_error().decode())
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_d)
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(_i64)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(_i32)

@@ -1,0 +1,350 @@
+// C ABI of liblbhip: operator lifecycle and the host/device entry points that
+// stand in for the reference's LinearOperator procedures.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "lb_internal.hpp"
+
+namespace lb {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int DevBuf::reserve(size_t n) {
+  if (n <= bytes && p) return LB_OK;
+  if (p) {
+    hipError_t e = hipFree(p);
+    (void)e;
+    p = nullptr;
+    bytes = 0;
+  }
+  size_t want = std::max<size_t>(n, 256);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    set_error("device allocation of " + std::to_string(want) + " bytes failed: " + hipGetErrorString(e));
+    return LB_E_NOMEM;
+  }
+  bytes = want;
+  return LB_OK;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+}  // namespace lb
+
+using namespace lb;
+
+namespace {
+
+int check_ready(lb_op* op, const char* who) {
+  if (!op) {
+    set_error(std::string(who) + ": null operator");
+    return LB_E_ARG;
+  }
+  if (!op->phys) {
+    set_error(std::string(who) + " was called, but the operator was not ready.");
+    return LB_E_NOT_READY;
+  }
+  return LB_OK;
+}
+
+int check_n(lb_op* op, int64_t n, const char* who) {
+  if (n != op->NENK * op->NI) {
+    set_error(std::string(who) + ": vector length " + std::to_string(n) + " != NENK*NI = " +
+              std::to_string(op->NENK * op->NI));
+    return LB_E_ARG;
+  }
+  return LB_OK;
+}
+
+int to_device(lb_op* op, const double* h, int64_t n) {
+  LB_TRY(op->w_a.reserve(n * sizeof(double)));
+  LB_HIP(hipMemcpyAsync(op->w_a.p, h, n * sizeof(double), hipMemcpyHostToDevice, op->stream));
+  return LB_OK;
+}
+
+int to_host(lb_op* op, double* h, int64_t n) {
+  LB_HIP(hipMemcpyAsync(h, op->w_a.p, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return LB_OK;
+}
+
+int upload_physics(lb_op* op) {
+  LB_TRY(op->d_sigt.reserve(op->h_sigt.size() * sizeof(double)));
+  LB_TRY(op->d_sigs.reserve(op->h_sigs.size() * sizeof(double)));
+  LB_HIP(hipMemcpy(op->d_sigt.p, op->h_sigt.data(), op->h_sigt.size() * sizeof(double), hipMemcpyHostToDevice));
+  LB_HIP(hipMemcpy(op->d_sigs.p, op->h_sigs.data(), op->h_sigs.size() * sizeof(double), hipMemcpyHostToDevice));
+  // dense kernel YYS(ip, i, m) = sum_l YY(ip, i, l) XS%s(l, m) for group 0
+  // (ScatteringOperator.f90:263-269)
+  const int64_t NI = op->NI;
+  std::vector<double> yys((size_t)op->Nm * NI * NI, 0.0);
+  for (int m = 0; m < op->Nm; ++m)
+    for (int l = 0; l <= op->L; ++l) {
+      const double sl = op->h_sigs[((size_t)m * op->G + 0) * (op->L + 1) + l];
+      const double* yy = &op->h_yy[(size_t)l * NI * NI];
+      double* dst = &yys[(size_t)m * NI * NI];
+      for (int64_t j = 0; j < NI * NI; ++j) dst[j] += yy[j] * sl;
+    }
+  LB_TRY(op->d_yys.reserve(yys.size() * sizeof(double)));
+  LB_HIP(hipMemcpy(op->d_yys.p, yys.data(), yys.size() * sizeof(double), hipMemcpyHostToDevice));
+  op->phys = true;
+  return LB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lb_last_error(void) { return g_error.c_str(); }
+
+int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out) {
+  if (!mesh || !quad || !out || !mesh->rg || !mesh->node || !mesh->nbr || !mesh->mat || !quad->k || !quad->w) {
+    set_error("lb_op_create: null argument");
+    return LB_E_ARG;
+  }
+  if (mesh->NE <= 0 || mesh->NK <= 0 || quad->NI <= 0 || quad->L < 0 || quad->L > 7 ||
+      (mesh->index_base != 0 && mesh->index_base != 1)) {
+    set_error("lb_op_create: invalid sizes (need NE, NK, NI > 0, 0 <= L <= 7, index_base 0 or 1)");
+    return LB_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    set_error("lb_op_create: no HIP device is visible");
+    return LB_E_HIP;
+  }
+  lb_op* op = new lb_op;
+  op->NE = mesh->NE;
+  op->NK = mesh->NK;
+  op->NI = quad->NI;
+  op->NENK = 4 * mesh->NE;
+  op->L = quad->L;
+  op->h_k.assign(quad->k, quad->k + 3 * quad->NI);
+  op->h_w.assign(quad->w, quad->w + quad->NI);
+  int rc = LB_OK;
+  if (hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&op->ev0) != hipSuccess || hipEventCreate(&op->ev1) != hipSuccess) {
+    set_error("lb_op_create: stream or event creation failed");
+    rc = LB_E_HIP;
+  }
+  if (rc == LB_OK) rc = compute_geometry(op, mesh->rg, mesh->node, mesh->nbr, mesh->mat, mesh->index_base);
+  if (rc == LB_OK) rc = build_angular(op);
+  if (rc != LB_OK) {
+    lb_op_destroy(op);
+    return rc;
+  }
+  *out = op;
+  return LB_OK;
+}
+
+int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t* sl, int32_t* nlev, double* normals) {
+  if (!mesh || !quad || !mesh->rg || !mesh->node || !mesh->nbr || !mesh->mat || !quad->k || !quad->w ||
+      mesh->NE <= 0 || quad->NI <= 0 || (mesh->index_base != 0 && mesh->index_base != 1)) {
+    set_error("lb_schedule_host: invalid argument");
+    return LB_E_ARG;
+  }
+  lb_op op;
+  op.host_only = true;
+  op.NE = mesh->NE;
+  op.NK = mesh->NK;
+  op.NI = quad->NI;
+  op.NENK = 4 * mesh->NE;
+  op.L = quad->L;
+  op.h_k.assign(quad->k, quad->k + 3 * quad->NI);
+  op.h_w.assign(quad->w, quad->w + quad->NI);
+  LB_TRY(compute_geometry(&op, mesh->rg, mesh->node, mesh->nbr, mesh->mat, mesh->index_base));
+  LB_TRY(lb_op_get_schedule(&op, sl, nlev));
+  if (normals) std::memcpy(normals, op.h_nrm.data(), op.h_nrm.size() * sizeof(double));
+  return LB_OK;
+}
+
+int lb_op_set_physics(lb_op* op, int Nm, const double* sigma_t, int L, const double* sigma_s) {
+  return lb_op_set_physics_groups(op, 1, Nm, sigma_t, L, sigma_s);
+}
+
+int lb_op_set_physics_groups(lb_op* op, int G, int Nm, const double* sigma_t, int L, const double* sigma_s) {
+  if (!op || G < 1 || Nm < op->Nm || L < 0 || !sigma_t || !sigma_s) {
+    set_error("lb_op_set_physics: need G >= 1, one sigma_t per mesh material (Nm >= " + std::to_string(op ? op->Nm : 0) +
+              ") and L >= 0");
+    return LB_E_ARG;
+  }
+  op->G = G;
+  op->h_sigt.assign((size_t)op->Nm * G, 0.0);
+  op->h_sigs.assign((size_t)op->Nm * G * (op->L + 1), 0.0);
+  for (int g = 0; g < G; ++g)
+    for (int m = 0; m < op->Nm; ++m) {
+      op->h_sigt[(size_t)m * G + g] = sigma_t[(size_t)g * Nm + m];
+      for (int l = 0; l <= std::min(L, op->L); ++l)
+        op->h_sigs[((size_t)m * G + g) * (op->L + 1) + l] = sigma_s[((size_t)g * Nm + m) * (L + 1) + l];
+    }
+  return upload_physics(op);
+}
+
+int lb_op_set_scatter_mode(lb_op* op, int mode) {
+  if (!op || (mode != LB_SCATTER_MOMENT && mode != LB_SCATTER_DENSE)) {
+    set_error("lb_op_set_scatter_mode: invalid mode");
+    return LB_E_ARG;
+  }
+  op->scatter_mode = mode;
+  return LB_OK;
+}
+
+int lb_transport_matinv(lb_op* op, double* s, int64_t n) {
+  LB_TRY(check_ready(op, "TransportOp%MatInv"));
+  LB_TRY(check_n(op, n, "TransportOp%MatInv"));
+  LB_TRY(to_device(op, s, n));
+  LB_TRY(launch_sweep(op, op->w_a.as<double>(), op->w_a.as<double>(), 0, 1));
+  return to_host(op, s, n);
+}
+
+int lb_scatter_matvec(lb_op* op, double* x, int64_t n) {
+  LB_TRY(check_ready(op, "ScatteringOp%MatVec"));
+  LB_TRY(check_n(op, n, "ScatteringOp%MatVec"));
+  LB_TRY(to_device(op, x, n));
+  LB_TRY(op->w_c.reserve(n * sizeof(double)));
+  LB_TRY(scatter_dev(op, op->w_a.as<double>(), op->w_c.as<double>(), 0, 1));
+  LB_HIP(hipMemcpyAsync(x, op->w_c.p, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return LB_OK;
+}
+
+int lb_boltzmann_matvec(lb_op* op, double* x, int64_t n) {
+  LB_TRY(check_ready(op, "BoltzmannOp%MatVec"));
+  LB_TRY(check_n(op, n, "BoltzmannOp%MatVec"));
+  LB_TRY(to_device(op, x, n));
+  LB_TRY(boltzmann_matvec_dev(op, op->w_a.as<double>(), 0));
+  return to_host(op, x, n);
+}
+
+int lb_boltzmann_matinv(lb_op* op, double* s, int64_t n, const lb_solver_opts* opts, lb_solve_stats* stats) {
+  LB_TRY(check_ready(op, "BoltzmannOp%MatInv"));
+  LB_TRY(check_n(op, n, "BoltzmannOp%MatInv"));
+  LB_TRY(to_device(op, s, n));
+  int rc = boltzmann_matinv_dev(op, op->w_a.as<double>(), 0, opts, stats);
+  if (rc != LB_OK && rc != LB_E_MAXIT && rc != LB_E_BREAKDOWN && rc != LB_E_NAN) return rc;
+  std::string err = g_error;
+  LB_TRY(to_host(op, s, n));
+  g_error = err;
+  return rc;
+}
+
+int lb_fluence(lb_op* op, const double* v, double* phi) {
+  if (!op || !v || !phi) {
+    set_error("lb_fluence: null argument");
+    return LB_E_ARG;
+  }
+  const int64_t n = op->NENK * op->NI;
+  LB_TRY(to_device(op, v, n));
+  LB_TRY(op->w_c.reserve(op->NENK * sizeof(double)));
+  LB_TRY(launch_fluence(op, op->w_a.as<double>(), op->w_c.as<double>()));
+  LB_HIP(hipMemcpyAsync(phi, op->w_c.p, op->NENK * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return LB_OK;
+}
+
+int lb_op_get_schedule(lb_op* op, int32_t* sl, int32_t* nlev) {
+  if (!op) {
+    set_error("lb_op_get_schedule: null operator");
+    return LB_E_ARG;
+  }
+  if (sl)
+    for (size_t j = 0; j < op->h_tasks.size(); ++j) sl[j] = int32_t(op->h_tasks[j] & 0xFFFFFFu);
+  if (nlev) std::memcpy(nlev, op->h_nlev.data(), op->h_nlev.size() * sizeof(int32_t));
+  return LB_OK;
+}
+
+int lb_op_get_normals(lb_op* op, double* n) {
+  if (!op || !n) {
+    set_error("lb_op_get_normals: null argument");
+    return LB_E_ARG;
+  }
+  std::memcpy(n, op->h_nrm.data(), op->h_nrm.size() * sizeof(double));
+  return LB_OK;
+}
+
+int lb_op_info(lb_op* op, int64_t* NE, int64_t* NI, int64_t* nmom, int64_t* maxlev) {
+  if (!op) {
+    set_error("lb_op_info: null operator");
+    return LB_E_ARG;
+  }
+  if (NE) *NE = op->NE;
+  if (NI) *NI = op->NI;
+  if (nmom) *nmom = op->nmom;
+  if (maxlev) *maxlev = op->maxlev;
+  return LB_OK;
+}
+
+int lb_transport_matinv_dev(lb_op* op, const double* d_q, double* d_psi, int g0, int G) {
+  LB_TRY(check_ready(op, "lb_transport_matinv_dev"));
+  if (g0 < 0 || G < 1 || g0 + G > op->G) {
+    set_error("lb_transport_matinv_dev: group range outside the physics table");
+    return LB_E_ARG;
+  }
+  return launch_sweep(op, d_q, d_psi, g0, G);
+}
+
+int lb_moments_dev(lb_op* op, const double* d_psi, double* d_phi, int G) {
+  if (!op || G < 1) {
+    set_error("lb_moments_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  return launch_moments(op, d_psi, d_phi, G, op->NENK * G);
+}
+
+int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, int g0, int G, int accumulate) {
+  LB_TRY(check_ready(op, "lb_moments_to_discrete_dev"));
+  if (g0 < 0 || G < 1 || g0 + G > op->G) {
+    set_error("lb_moments_to_discrete_dev: group range outside the physics table");
+    return LB_E_ARG;
+  }
+  return launch_moments_to_discrete(op, d_phi, d_psi, g0, G, accumulate);
+}
+
+int lb_op_synchronize(lb_op* op) {
+  if (!op) {
+    set_error("lb_op_synchronize: null operator");
+    return LB_E_ARG;
+  }
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return LB_OK;
+}
+
+int lb_last_sweep_ms(lb_op* op, double* ms) {
+  if (!op || !ms) {
+    set_error("lb_last_sweep_ms: null argument");
+    return LB_E_ARG;
+  }
+  float f = 0.f;
+  LB_HIP(hipEventSynchronize(op->ev1));
+  LB_HIP(hipEventElapsedTime(&f, op->ev0, op->ev1));
+  *ms = f;
+  return LB_OK;
+}
+
+void lb_op_destroy(lb_op* op) {
+  if (!op) return;
+  if (op->stream) (void)hipStreamSynchronize(op->stream);
+  for (DevBuf* b : {&op->d_geo, &op->d_nbr, &op->d_perm, &op->d_mat, &op->d_tasks, &op->d_k, &op->d_w, &op->d_D,
+                    &op->d_M, &op->d_deg, &op->d_yys, &op->d_sigt, &op->d_sigs, &op->w_a, &op->w_a2, &op->w_b,
+                    &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov})
+    b->release();
+  for (auto& kv : op->tables) kv.second.d_entries.release();
+  if (op->w_host.p) (void)hipHostFree(op->w_host.p);
+  if (op->ev0) (void)hipEventDestroy(op->ev0);
+  if (op->ev1) (void)hipEventDestroy(op->ev1);
+  if (op->stream) (void)hipStreamDestroy(op->stream);
+  delete op;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal declarations shared by the liblbhip sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lionbolt.h"
+
+namespace lb {
+
+void set_error(const std::string& msg);
+
+#define LB_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (call);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      lb::set_error(std::string("HIP error ") + hipGetErrorString(_e) + " at " + __FILE__ + \
+                    ":" + std::to_string(__LINE__) + " in " #call);                          \
+      return LB_E_HIP;                                                                       \
+    }                                                                                        \
+  } while (0)
+
+#define LB_TRY(call)              \
+  do {                            \
+    int _rc = (call);             \
+    if (_rc != LB_OK) return _rc; \
+  } while (0)
+
+// A grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t n);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Work list of one sweep configuration: for every global level, the workgroups
+// (direction, first sweep position, cell count) that sweep that level.
+struct LevelTable {
+  int cpw = 0;                        // cells per workgroup
+  std::vector<int32_t> level_off;     // [maxlev + 1] offsets into entries
+  std::vector<int32_t> entries;       // 4 ints per workgroup
+  DevBuf d_entries;
+};
+
+// Local faces of a tetrahedron (FiniteElementAnalysis.f90:129-132), 0-based.
+constexpr int kFaceNodes[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+
+}  // namespace lb
+
+struct lb_op {
+  int64_t NE = 0, NK = 0, NI = 0, NENK = 0;
+  int L = 0, nmom = 1, Nm = 1;
+  int maxlev = 0;
+  bool host_only = false;           // precompute only (no device): used by lb_schedule_host
+  // host-side copies kept for queries and table builds
+  std::vector<double> h_nrm;        // [NE][4][3]
+  std::vector<double> h_k, h_w;     // [NI][3], [NI]
+  std::vector<int32_t> h_mat;       // [NE] 0-based
+  std::vector<uint32_t> h_tasks;    // [NI][NE] packed cell | up << 24 | out << 28
+  std::vector<int32_t> h_nlev;      // [NI]
+  std::vector<int32_t> h_lvl_off;   // [NI][maxlev + 1]
+  std::vector<double> h_D, h_M;     // [nmom][NI], [NI][nmom]
+  std::vector<double> h_yy;         // [L+1][NI][NI] = YY(ip, i, l) stored [l][ip][i]
+  std::vector<int32_t> h_deg;       // [nmom] Legendre order of each moment
+  // device geometry and schedule
+  lb::DevBuf d_geo;     // double [NE][4][4]: n_x, n_y, n_z, c_f = 10 A_f / detJ
+  lb::DevBuf d_nbr;     // int32 [NE][4], -1 on the boundary
+  lb::DevBuf d_perm;    // uint32 [NE]: neighbour local node of each face node, 2 bits each
+  lb::DevBuf d_mat;     // int32 [NE]
+  lb::DevBuf d_tasks;   // uint32 [NI][NE]
+  lb::DevBuf d_k;       // double [NI][3]
+  lb::DevBuf d_w;       // double [NI]
+  lb::DevBuf d_D, d_M;  // moment matrices
+  lb::DevBuf d_deg;     // int32 [nmom] Legendre order of each moment
+  lb::DevBuf d_yys;     // double [Nm][NI][NI] dense kernel (YYS(ip,i,m) stored [m][ip][i])
+  std::map<int, lb::LevelTable> tables;
+  // physics
+  int G = 0;                        // groups in the physics table
+  bool phys = false;
+  std::vector<double> h_sigt;       // [Nm][G]
+  std::vector<double> h_sigs;       // [Nm][G][L+1]
+  lb::DevBuf d_sigt, d_sigs;
+  int scatter_mode = LB_SCATTER_MOMENT;
+  // runtime
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_sweep_ms = 0.f;
+  // workspaces
+  lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host;
+};
+
+namespace lb {
+
+// lb_setup.cpp
+int upload(lb_op* op, DevBuf& dst, const void* src, size_t bytes);
+int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int64_t* nbr, const int64_t* mat,
+                     int base);
+int compute_schedule(lb_op* op, const int64_t* nbr_c);
+int build_angular(lb_op* op);
+LevelTable* level_table(lb_op* op, int G, int* rc);
+
+// lb_kernels.hip (launchers; all enqueue on op->stream)
+int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G);
+int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols);
+int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate);
+int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G);
+int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* z);
+int launch_scale(hipStream_t s, int64_t n, double a, double* x);
+int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
+int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
+int launch_fluence(lb_op* op, const double* v, double* phi);
+
+// lb_krylov.cpp
+int boltzmann_matvec_dev(lb_op* op, double* x, int g0);
+int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* opts, lb_solve_stats* st);
+int scatter_dev(lb_op* op, const double* x, double* y, int g0, int G);
+
+}  // namespace lb

@@ -1,0 +1,398 @@
+// gfx950 kernels of the transport-solve path.
+//
+// Sweep: level-synchronous upwind sweep of TransportAlgebra.f90:29-246.  One
+// launch per global level sweeps that level of every direction at once; a
+// 256-lane workgroup takes consecutive cells of one level of one direction, one
+// lane per (cell, group).  Each lane gathers the three face-node values of its
+// upstream neighbours, builds the 4x4 local matrix in registers from the compact
+// face data (unit normal + c_f), and solves it with the reference's partial-pivot
+// LU.  Vector layout: v[((i*NE + e)*4 + k)*G + g] (groups fastest) so the lanes of
+// one cell read consecutive doubles.
+#include <hip/hip_runtime.h>
+
+#include "lb_internal.hpp"
+
+namespace lb {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool in_face(int f, int a) {
+  // node a lies on local face f unless it is the opposite node 3 - f
+  return a != 3 - f;
+}
+
+// One lane = one (cell, direction, group) local solve.
+__global__ __launch_bounds__(kBlock) void sweep_level_kernel(
+    const int4* __restrict__ work, const uint32_t* __restrict__ tasks, const double4* __restrict__ geo,
+    const int4* __restrict__ nbr, const uint32_t* __restrict__ perm, const int32_t* __restrict__ mat,
+    const double* __restrict__ kdir, const double* __restrict__ sigt, int Gtot, int g0, int G, int64_t NE,
+    const double* q, double* psi) {
+  const int4 wd = work[blockIdx.x];
+  const int i = wd.x;
+  const int t = threadIdx.x;
+  const int c = t / G;
+  const int g = t - c * G;
+  if (c >= wd.z) return;
+  const uint32_t w = tasks[(int64_t)i * NE + wd.y + c];
+  const int e = int(w & 0xFFFFFFu);
+  const unsigned up = (w >> 24) & 0xFu;
+  const unsigned out = w >> 28;
+
+  const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
+  double kn[4], cf[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const double4 gf = geo[(int64_t)e * 4 + f];
+    kn[f] = kx * gf.x + ky * gf.y + kz * gf.z;
+    cf[f] = gf.w;
+  }
+  const double sig = sigt[(int64_t)mat[e] * Gtot + g0 + g];
+  const int64_t base = ((int64_t)i * NE + e) * 4 * G + g;
+  double b[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) b[a] = q[base + a * G];
+
+  // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
+  // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
+  // node, u = neighbour values at the shared nodes, U their sum.
+  if (up) {
+    const int4 nb = nbr[e];
+    const uint32_t pm = perm[e];
+    const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (up >> f & 1u) {
+        const int64_t ub = ((int64_t)i * NE + nbv[f]) * 4 * G + g;
+        const double u0 = psi[ub + ((pm >> (6 * f)) & 3u) * G];
+        const double u1 = psi[ub + ((pm >> (6 * f + 2)) & 3u) * G];
+        const double u2 = psi[ub + ((pm >> (6 * f + 4)) & 3u) * G];
+        const double U = u0 + u1 + u2;
+        const double coef = -kn[f] * cf[f];
+        b[kFaceNodes[f][0]] += coef * (u0 + 0.2 * U);
+        b[kFaceNodes[f][1]] += coef * (u1 + 0.2 * U);
+        b[kFaceNodes[f][2]] += coef * (u2 + 0.2 * U);
+        b[3 - f] += coef * (-0.8 * U);
+      }
+    }
+  }
+
+  // Local matrix (ApplyInvertedDiagonal, TransportAlgebra.f90:126-152):
+  // sigma I - sum_d k_d IC_d + sum_{f: k.n >= 0} (k.n) IFM_f.
+  double T[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double r = cf[3 - a] * kn[3 - a];
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb) T[a][bb] = r;
+    T[a][a] += sig;
+  }
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (out >> f & 1u) {
+      const double v = cf[f] * kn[f];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int bb = kFaceNodes[f][j];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const double pat = (a == bb) ? 1.2 : (in_face(f, a) ? 0.2 : -0.8);
+          T[a][bb] += v * pat;
+        }
+      }
+    }
+  }
+
+  // LocalLU (TransportAlgebra.f90:173-246): partial pivoting, row swap when a
+  // strictly larger |A(i,k)| is found, then back substitution.
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double mx = fabs(T[k][k]);
+    int p = k;
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r) {
+      const double v = fabs(T[r][k]);
+      if (v > mx) {
+        mx = v;
+        p = r;
+      }
+    }
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r) {
+      if (p == r) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double tmp = T[k][j];
+          T[k][j] = T[r][j];
+          T[r][j] = tmp;
+        }
+        const double tb = b[k];
+        b[k] = b[r];
+        b[r] = tb;
+      }
+    }
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r) {
+      const double factor = T[r][k] / T[k][k];
+#pragma unroll
+      for (int j = k + 1; j < 4; ++j) T[r][j] = T[r][j] - factor * T[k][j];
+      b[r] = b[r] - factor * b[k];
+    }
+  }
+  double x[4];
+  x[3] = b[3] / T[3][3];
+#pragma unroll
+  for (int r = 2; r >= 0; --r) {
+    double s = b[r];
+#pragma unroll
+    for (int j = r + 1; j < 4; ++j) s = s - T[r][j] * x[j];
+    x[r] = s / T[r][r];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) psi[base + a * G] = x[a];
+}
+
+// phi[lm][col] = sum_i D[lm][i] psi[i][col]; one lane per column (column =
+// spatial dof x group), D read through the scalar cache.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void moments_kernel(const double* __restrict__ D, int NI, int64_t ncols,
+                                                         const double* __restrict__ psi, double* __restrict__ phi) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= ncols) return;
+  double acc[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) acc[m] = 0.0;
+  for (int i = 0; i < NI; ++i) {
+    const double v = psi[(int64_t)i * ncols + col];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) acc[m] += D[m * NI + i] * v;
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) phi[(int64_t)m * ncols + col] = acc[m];
+}
+
+// psi[i][col] (+)= sum_lm M[i][lm] sigma_{l(lm)}(mat, g) phi[lm][col].
+template <int NM>
+__global__ __launch_bounds__(kBlock) void moments_to_discrete_kernel(
+    const double* __restrict__ M, const int32_t* __restrict__ deg, int NI, int64_t ncols, int G, int Gtot, int g0,
+    int L, const int32_t* __restrict__ mat, const double* __restrict__ sigs, const double* __restrict__ phi,
+    double* __restrict__ psi, int accumulate) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= ncols) return;
+  const int64_t sd = col / G;
+  const int g = int(col - sd * G);
+  const int m_el = mat[sd >> 2];
+  const double* sl = sigs + ((int64_t)m_el * Gtot + g0 + g) * (L + 1);
+  double sp[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) sp[m] = phi[(int64_t)m * ncols + col] * sl[deg[m]];
+  for (int i = 0; i < NI; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) acc += M[i * NM + m] * sp[m];
+    double* dst = psi + (int64_t)i * ncols + col;
+    *dst = accumulate ? *dst + acc : acc;
+  }
+}
+
+// Dense kernel of ScatteringAlgebra.f90:30-53: y[i][col] = sum_ip x[ip][col] YYS_m[ip][i],
+// m = material of the column; 16 outputs per pass over the column.
+__global__ __launch_bounds__(kBlock) void scatter_dense_kernel(const double* __restrict__ yys, int NI, int64_t ncols,
+                                                               int G, const int32_t* __restrict__ mat,
+                                                               const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= ncols) return;
+  const int m_el = mat[(col / G) >> 2];
+  const double* Y = yys + (int64_t)m_el * NI * NI;
+  for (int i0 = 0; i0 < NI; i0 += 16) {
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    const int nj = min(16, NI - i0);
+    for (int ip = 0; ip < NI; ++ip) {
+      const double v = x[(int64_t)ip * ncols + col];
+      const double* row = Y + (int64_t)ip * NI + i0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j < nj) acc[j] += v * row[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < nj) y[(int64_t)(i0 + j) * ncols + col] = acc[j];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, double a, const double* x, double b, const double* y,
+                                                       double* z) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock)
+    z[j] = a * x[j] + b * y[j];
+}
+
+__global__ __launch_bounds__(kBlock) void scale_kernel(int64_t n, double a, double* x) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) x[j] *= a;
+}
+
+// Deterministic two-stage dot product: fixed grid, fixed reduction tree.
+constexpr int kDotBlocks = 2048;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n, const double* __restrict__ x,
+                                                             const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock)
+    acc += x[j] * y[j];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void dot_final_kernel(int nparts, const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < nparts; j += kBlock) acc += part[j];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void fluence_kernel(const double* __restrict__ w, int NI, int64_t NENK,
+                                                         const double* __restrict__ v, double* __restrict__ phi) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= NENK) return;
+  double acc = 0.0;
+  for (int i = 0; i < NI; ++i) acc += w[i] * v[(int64_t)i * NENK + j];
+  phi[j] = acc;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned grid_stride(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  return (unsigned)(g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
+}  // namespace
+
+int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
+  if (G < 1 || G > kBlock) {
+    set_error("sweep: the group batch must be between 1 and 256");
+    return LB_E_ARG;
+  }
+  int rc = LB_OK;
+  LevelTable* t = level_table(op, G, &rc);
+  if (!t) return rc;
+  const int4* work = t->d_entries.as<const int4>();
+  LB_HIP(hipEventRecord(op->ev0, op->stream));
+  for (int l = 0; l < op->maxlev; ++l) {
+    const int nb = t->level_off[l + 1] - t->level_off[l];
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l],
+                       op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
+                       op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
+                       op->d_sigt.as<const double>(), op->G, g0, G, op->NE, q, psi);
+  }
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipEventRecord(op->ev1, op->stream));
+  return LB_OK;
+}
+
+#define LB_NM_DISPATCH(NMV, CALL) \
+  switch (NMV) {                  \
+    case 1: CALL(1); break;       \
+    case 4: CALL(4); break;       \
+    case 9: CALL(9); break;       \
+    case 16: CALL(16); break;     \
+    case 25: CALL(25); break;     \
+    case 36: CALL(36); break;     \
+    case 49: CALL(49); break;     \
+    case 64: CALL(64); break;     \
+    default:                      \
+      set_error("moments: scattering order above 7 is not supported"); \
+      return LB_E_ARG;            \
+  }
+
+int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols) {
+  (void)G;
+#define CALL(NMV)                                                                                                  \
+  hipLaunchKernelGGL(moments_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream, op->d_D.as<double>(), \
+                     (int)op->NI, ncols, psi, phi)
+  LB_NM_DISPATCH(op->nmom, CALL)
+#undef CALL
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate) {
+  const int64_t ncols = op->NENK * G;
+#define CALL(NMV)                                                                                                 \
+  hipLaunchKernelGGL(moments_to_discrete_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream,       \
+                     op->d_M.as<double>(), op->d_deg.as<const int32_t>(), (int)op->NI, ncols, G, op->G, g0, op->L,             \
+                     op->d_mat.as<const int32_t>(), op->d_sigs.as<const double>(), phi, psi, accumulate)
+  LB_NM_DISPATCH(op->nmom, CALL)
+#undef CALL
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G) {
+  (void)g0;
+  const int64_t ncols = op->NENK * G;
+  hipLaunchKernelGGL(scatter_dense_kernel, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream, op->d_yys.as<double>(),
+                     (int)op->NI, ncols, G, op->d_mat.as<const int32_t>(), x, y);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* z) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_stride(n)), dim3(kBlock), 0, s, n, a, x, b, y, z);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_scale(hipStream_t s, int64_t n, double a, double* x) {
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_stride(n)), dim3(kBlock), 0, s, n, a, x);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_copy(hipStream_t s, int64_t n, const double* x, double* y) {
+  if (x == y || n == 0) return LB_OK;
+  LB_HIP(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return LB_OK;
+}
+
+int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
+  LB_TRY(op->w_red.reserve((kDotBlocks + 8) * sizeof(double)));
+  if (!op->w_host.p) {
+    LB_HIP(hipHostMalloc(&op->w_host.p, 64, hipHostMallocDefault));
+    op->w_host.bytes = 64;
+  }
+  double* part = op->w_red.as<double>();
+  double* res = part + kDotBlocks;
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kBlock), 0, op->stream, n, x, y, part);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kBlock), 0, op->stream, kDotBlocks, part, res);
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipMemcpyAsync(op->w_host.p, res, sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  *out = *static_cast<double*>(op->w_host.p);
+  return LB_OK;
+}
+
+int launch_fluence(lb_op* op, const double* v, double* phi) {
+  hipLaunchKernelGGL(fluence_kernel, dim3(grid_for(op->NENK)), dim3(kBlock), 0, op->stream, op->d_w.as<double>(),
+                     (int)op->NI, op->NENK, v, phi);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+}  // namespace lb

@@ -1,0 +1,383 @@
+// Krylov drivers over device vectors.
+//
+// GMRES follows SPARSKIT's restarted gmres (third_party/SPARSKIT/iters.f:1901-2206)
+// as GMRESWrapper drives it (src/solvers/SPARSKIT.f90:20-181): convergence test
+// ipar(3) = 2, ||r|| <= rtol ||rhs|| + atol; modified Gram-Schmidt with selective
+// re-orthogonalisation when fct^2 > 0.98 ||v||^2 (mgsro, iters.f:3465-3586); Givens
+// rotations of Golub & van Loan (givens, iters.f:3225-3260); at every restart the
+// residual is recomputed with one operator application.  The vectors and all
+// O(n) work live on the GPU (FP64); the (m+1) x m Hessenberg recurrences run on
+// the host in FP64, which is where SPARSKIT keeps them too.
+//
+// Source iteration follows SourceIteration.f90:3-82.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <vector>
+
+#include "lb_internal.hpp"
+
+namespace lb {
+
+namespace {
+
+using Apply = std::function<int(double*)>;
+
+struct GmresResult {
+  int status = 0;       // 0, LB_E_MAXIT, LB_E_BREAKDOWN, LB_E_NAN
+  int iterates = 0;     // reverse-communication returns of the reference
+  int matvecs = 0;
+  double final_norm = 0.0;
+};
+
+void print_row(int verbose, int iter, double v) {
+  if (verbose) std::printf("             %3d       %24.16E\n", iter, v);
+}
+
+// Restarted GMRES(m) on device vectors of length n.  V holds m+1 vectors.
+int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* sol, double* V, int m, double rtol,
+          double atol, int maxit, int verbose, GmresResult* res) {
+  hipStream_t s = op->stream;
+  std::vector<double> H((size_t)(m + 1) * (m + 2) / 2 + 2, 0.0), c(m + 2), sn(m + 2), rs(m + 3);
+  auto hcol = [&](int k) { return &H[(size_t)k * (k - 1) / 2]; };
+  auto vec = [&](int j) { return V + (int64_t)j * n; };
+  int iter = 0, nmv = 0;
+  double fpar4 = 0.0, fpar5 = 0.0;
+  bool first = true;
+  auto ret = [&](double v) {
+    ++iter;
+    print_row(verbose, iter, v);
+    if (std::isnan(v)) return false;
+    return true;
+  };
+  if (verbose) {
+    std::printf("          -------------------------------------\n");
+    std::printf("           ITERATE           RESIDUAL NORM     \n");
+    std::printf("          ---------     -----------------------\n");
+  }
+  ret(0.0);
+  int status = 0;
+  while (true) {
+    double* v0 = vec(0);
+    LB_TRY(launch_copy(s, n, sol, v0));
+    LB_TRY(apply(v0));
+    ++nmv;
+    LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    double a2;
+    LB_TRY(dot(op, n, v0, v0, &a2));
+    double alpha = std::sqrt(a2);
+    if (first) {
+      double r2;
+      LB_TRY(dot(op, n, rhs, rhs, &r2));
+      fpar4 = rtol * std::sqrt(r2) + atol;
+      first = false;
+    }
+    fpar5 = alpha;
+    rs[1] = alpha;
+    if (alpha <= fpar4) {
+      if (!ret(fpar5)) status = LB_E_NAN;
+      break;
+    }
+    if (std::isnan(alpha)) {
+      ret(alpha);
+      status = LB_E_NAN;
+      break;
+    }
+    LB_TRY(launch_scale(s, n, 1.0 / alpha, v0));
+    int k = 0, ierr = 0;
+    while (true) {
+      ++k;
+      if (!ret(fpar5)) {
+        status = LB_E_NAN;
+        break;
+      }
+      double* vk = vec(k);
+      LB_TRY(launch_copy(s, n, vec(k - 1), vk));
+      LB_TRY(apply(vk));
+      ++nmv;
+      double* hh = hcol(k);
+      double nrm0;
+      LB_TRY(dot(op, n, vk, vk, &nrm0));
+      double thr = nrm0 * 0.98;
+      if (!(nrm0 > 0.0)) {
+        ierr = -1;
+        break;
+      }
+      for (int i = 0; i < k; ++i) {
+        const double* vi = vec(i);
+        double fct;
+        LB_TRY(dot(op, n, vk, vi, &fct));
+        hh[i] = fct;
+        LB_TRY(launch_axpby(s, n, -fct, vi, 1.0, vk, vk));
+        if (fct * fct > thr) {
+          LB_TRY(dot(op, n, vk, vi, &fct));
+          hh[i] = hh[i] + fct;
+          LB_TRY(launch_axpby(s, n, -fct, vi, 1.0, vk, vk));
+        }
+        nrm0 = nrm0 - hh[i] * hh[i];
+        if (nrm0 < 0.0) nrm0 = 0.0;
+        thr = nrm0 * 0.98;
+      }
+      double n1;
+      LB_TRY(dot(op, n, vk, vk, &n1));
+      double nrm1 = std::sqrt(n1);
+      hh[k] = nrm1;
+      if (!(nrm1 > 0.0)) {
+        ierr = -3;
+        break;
+      }
+      LB_TRY(launch_scale(s, n, 1.0 / nrm1, vk));
+      for (int i = 1; i <= k - 1; ++i) {
+        double a = hh[i - 1];
+        hh[i - 1] = c[i] * a + sn[i] * hh[i];
+        hh[i] = c[i] * hh[i] - sn[i] * a;
+      }
+      double& x = hh[k - 1];
+      double& y = hh[k];
+      double cc, ss;
+      if (x == 0.0 && y == 0.0) {
+        cc = 1.0;
+        ss = 0.0;
+      } else if (std::fabs(y) > std::fabs(x)) {
+        double t = x / y;
+        x = std::sqrt(1.0 + t * t);
+        ss = std::copysign(1.0 / x, y);
+        cc = t * ss;
+      } else if (std::fabs(y) <= std::fabs(x)) {
+        double t = y / x;
+        y = std::sqrt(1.0 + t * t);
+        cc = std::copysign(1.0 / y, x);
+        ss = t * cc;
+      } else {
+        x = 0.0;
+        y = 0.0;
+        cc = 1.0;
+        ss = 0.0;
+      }
+      x = std::fabs(x * y);
+      c[k] = cc;
+      sn[k] = ss;
+      double a2k = -ss * rs[k];
+      rs[k] = cc * rs[k];
+      rs[k + 1] = a2k;
+      fpar5 = std::fabs(a2k);
+      if (!(k < m && !(fpar5 <= fpar4) && (maxit <= 0 || nmv < maxit))) break;
+    }
+    if (status != 0) break;
+    // triangular solve and update (iters.f:2100-2155)
+    int kk = k;
+    while (kk > 0 && hcol(kk)[kk - 1] == 0.0) --kk;
+    if (kk > 0 && ierr == 0) {
+      std::vector<double> y(rs.begin(), rs.end());
+      y[kk] = y[kk] / hcol(kk)[kk - 1];
+      for (int i = kk - 1; i >= 1; --i) {
+        const double* col = hcol(i + 1);
+        for (int ii = 1; ii <= i; ++ii) y[ii] = y[ii] - y[i + 1] * col[ii - 1];
+        y[i] = y[i] / hcol(i)[i - 1];
+      }
+      double* u = vec(0);
+      LB_TRY(launch_scale(s, n, y[1], u));
+      for (int i = 1; i <= kk - 1; ++i) LB_TRY(launch_axpby(s, n, y[i + 1], vec(i), 1.0, u, u));
+      LB_TRY(launch_axpby(s, n, 1.0, u, 1.0, sol, sol));
+    }
+    if (ierr != 0 || kk == 0) {
+      ret(fpar5);
+      status = LB_E_BREAKDOWN;
+      break;
+    }
+    if ((maxit <= 0 || nmv < maxit) && fpar5 > fpar4) {
+      if (!ret(fpar5)) {
+        status = LB_E_NAN;
+        break;
+      }
+      continue;
+    }
+    status = (fpar5 <= fpar4) ? 0 : LB_E_MAXIT;
+    ret(fpar5);
+    break;
+  }
+  res->status = status;
+  res->iterates = iter;
+  res->matvecs = nmv;
+  res->final_norm = fpar5;
+  if (verbose && status == 0) {
+    std::printf("\n          CONVERGED.\n          Total number of iterates : %3d\n", iter);
+    std::printf("          Final Residual Norm      : %24.16E\n\n", fpar5);
+  }
+  return LB_OK;
+}
+
+int sweep_inplace(lb_op* op, double* v, int g0) { return launch_sweep(op, v, v, g0, 1); }
+
+}  // namespace
+
+int scatter_dev(lb_op* op, const double* x, double* y, int g0, int G) {
+  if (op->scatter_mode == LB_SCATTER_DENSE) {
+    if (G != 1 || op->G != 1) {
+      set_error("scatter: the dense kernel applies to single-group physics only");
+      return LB_E_ARG;
+    }
+    return launch_scatter_dense(op, x, y, g0, G);
+  }
+  const int64_t ncols = op->NENK * G;
+  LB_TRY(op->w_phi.reserve((size_t)op->nmom * ncols * sizeof(double)));
+  LB_TRY(launch_moments(op, x, op->w_phi.as<double>(), G, ncols));
+  return launch_moments_to_discrete(op, op->w_phi.as<double>(), y, g0, G, 0);
+}
+
+// x <- x - T^-1 K x (BoltzmannOperator.f90:59-104).  K x goes to a separate
+// buffer, so x itself serves as the reference's saved copy xw.
+int boltzmann_matvec_dev(lb_op* op, double* x, int g0) {
+  const int64_t n = op->NENK * op->NI;
+  LB_TRY(op->w_c.reserve(n * sizeof(double)));
+  double* y = op->w_c.as<double>();
+  LB_TRY(scatter_dev(op, x, y, g0, 1));
+  LB_TRY(sweep_inplace(op, y, g0));
+  return launch_axpby(op->stream, n, 1.0, x, -1.0, y, x);
+}
+
+// BoltzmannOp%MatInv (BoltzmannOperator.f90:106-151) for one group g0: s enters
+// as the source q and leaves as psi.
+int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, lb_solve_stats* st) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = op->NENK * op->NI;
+  const int64_t nphi = op->NENK * op->nmom;
+  lb_solver_opts opt = o ? *o : lb_solver_opts{LB_SOLVER_GMRES, LB_SPACE_PSI, 10, 0, 1e-8, 1e-12, 0};
+  if (opt.restart < 1) opt.restart = 10;
+  lb_solve_stats stats{};
+  int sweeps = 0;
+  hipStream_t str = op->stream;
+  // keep q for the moment-space reconstruction, then b = T^-1 q
+  LB_TRY(op->w_b.reserve(n * sizeof(double)));
+  double* q = op->w_b.as<double>();
+  LB_TRY(launch_copy(str, n, s, q));
+  LB_TRY(sweep_inplace(op, s, g0));
+  ++sweeps;
+  const double sig1 = op->h_sigt[(size_t)0 * op->G + g0];  // XS%t(1) of this group
+
+  if (opt.kind == LB_SOLVER_SI) {
+    // SourceIteration.f90:34-80 on s = T^-1 q
+    LB_TRY(op->w_krylov.reserve(2 * n * sizeof(double)));
+    double* w = op->w_krylov.as<double>();
+    double* t = w + n;
+    LB_TRY(launch_copy(str, n, s, w));
+    double ns;
+    LB_TRY(dot(op, n, s, s, &ns));
+    const double norms = std::sqrt(ns);
+    LB_TRY(scatter_dev(op, w, t, g0, 1));
+    LB_TRY(sweep_inplace(op, t, g0));
+    ++sweeps;
+    std::swap(w, t);
+    LB_TRY(launch_axpby(str, n, 1.0, w, 1.0, s, s));
+    int iter = 1;
+    stats.status = LB_E_MAXIT;
+    if (opt.verbose) std::printf("             %3d         ----------------------\n", 1);
+    while (opt.maxit <= 0 || iter < opt.maxit) {
+      ++iter;
+      double r2;
+      LB_TRY(dot(op, n, w, w, &r2));
+      const double res = std::sqrt(r2);
+      print_row(opt.verbose, iter, res);
+      stats.final_norm = res;
+      if (std::isnan(res)) {
+        stats.status = LB_E_NAN;
+        break;
+      }
+      if (res <= opt.atol + norms * opt.rtol) {
+        stats.status = 0;
+        break;
+      }
+      LB_TRY(scatter_dev(op, w, t, g0, 1));
+      LB_TRY(sweep_inplace(op, t, g0));
+      ++sweeps;
+      std::swap(w, t);
+      LB_TRY(launch_axpby(str, n, 1.0, w, 1.0, s, s));
+    }
+    stats.iterates = iter;
+    stats.matvecs = iter - 1;
+  } else if (opt.space == LB_SPACE_PSI) {
+    const int m = opt.restart;
+    LB_TRY(op->w_krylov.reserve((size_t)(m + 2) * n * sizeof(double)));
+    double* V = op->w_krylov.as<double>();
+    double* sol = V + (int64_t)(m + 1) * n;
+    LB_TRY(launch_copy(str, n, s, sol));
+    LB_TRY(launch_scale(str, n, 1.0 / sig1, sol));  // xg = s / sigma_t(1)
+    GmresResult r;
+    Apply A = [&](double* v) {
+      ++sweeps;
+      return boltzmann_matvec_dev(op, v, g0);
+    };
+    LB_TRY(gmres(op, n, A, s, sol, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r));
+    LB_TRY(launch_copy(str, n, sol, s));
+    stats.status = r.status;
+    stats.iterates = r.iterates;
+    stats.matvecs = r.matvecs;
+    stats.final_norm = r.final_norm;
+  } else {
+    // Flux-moment GMRES: with phi = D psi the within-group system
+    // (I - T^-1 M S D) psi = T^-1 q is equivalent to (I - D T^-1 M S) phi = D T^-1 q,
+    // and psi = T^-1 (M S phi + q).  Vectors shrink from NI to (L+1)^2 per dof.
+    const int m = opt.restart;
+    LB_TRY(op->w_krylov.reserve((size_t)(m + 3) * nphi * sizeof(double)));
+    LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+    double* V = op->w_krylov.as<double>();
+    double* phi = V + (int64_t)(m + 1) * nphi;
+    double* rhs = phi + nphi;
+    double* psi_t = op->w_a2.as<double>();
+    LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
+    LB_TRY(launch_copy(str, nphi, rhs, phi));
+    LB_TRY(launch_scale(str, nphi, 1.0 / sig1, phi));
+    GmresResult r;
+    Apply A = [&](double* v) -> int {
+      ++sweeps;
+      LB_TRY(launch_moments_to_discrete(op, v, psi_t, g0, 1, 0));
+      LB_TRY(sweep_inplace(op, psi_t, g0));
+      LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
+      double* tphi = op->w_phi.as<double>();
+      LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+      return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
+    };
+    LB_TRY(gmres(op, nphi, A, rhs, phi, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r));
+    // psi = T^-1 (M S phi + q)
+    LB_TRY(launch_copy(str, n, q, s));
+    LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
+    LB_TRY(sweep_inplace(op, s, g0));
+    ++sweeps;
+    stats.status = r.status;
+    stats.iterates = r.iterates;
+    stats.matvecs = r.matvecs;
+    stats.final_norm = r.final_norm;
+  }
+  // true transport residual ||T^-1 q - (I - T^-1 K) psi|| / ||T^-1 q||, with
+  // T^-1 q recomputed from the saved source
+  {
+    LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+    double* b = op->w_a2.as<double>();
+    LB_TRY(launch_copy(str, n, q, b));
+    LB_TRY(sweep_inplace(op, b, g0));
+    LB_TRY(launch_copy(str, n, s, q));  // q no longer needed: reuse as A psi
+    LB_TRY(boltzmann_matvec_dev(op, q, g0));
+    sweeps += 2;
+    double bb;
+    LB_TRY(dot(op, n, b, b, &bb));
+    LB_TRY(launch_axpby(str, n, 1.0, b, -1.0, q, q));
+    double rr;
+    LB_TRY(dot(op, n, q, q, &rr));
+    stats.true_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  }
+  LB_HIP(hipStreamSynchronize(str));
+  stats.sweeps = sweeps;
+  stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st) *st = stats;
+  if (stats.status != 0) {
+    const char* what = stats.status == LB_E_MAXIT ? "did not converge in the iteration limit"
+                       : stats.status == LB_E_BREAKDOWN ? "anticipated breakdown / division by zero"
+                       : "RESIDUAL NORM is NaN";
+    set_error(std::string(opt.kind == LB_SOLVER_SI ? "SI : " : "GMRES : ") + what);
+    return stats.status;
+  }
+  return LB_OK;
+}
+
+}  // namespace lb

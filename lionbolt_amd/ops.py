@@ -1,0 +1,284 @@
+"""Host-side mirror of the reference's operator API on top of liblbhip.
+
+The reference exposes (src/LionboltAPI.f90:1-11, LinearOperatorClass.f90:30-112):
+
+* ``SpaceAngleVector(mesh, angular)`` with ``%v`` and ``%Fluence(w)``;
+* ``TransportOp``, ``ScatteringOp``, ``BoltzmannOp`` with the generic
+  ``%Build(mesh, angular, XS)`` / ``%Build(mesh)`` / ``%Build(mesh, angular)`` /
+  ``%Build(angular, XS)``, ``%MatVec(x)``, ``%MatInv(s)`` and ``%Destroy()``.
+
+The classes below keep those names, argument meanings and in-place semantics;
+every failure raises ``LionboltError`` where the reference calls ``stophere``.
+``Operator`` is the thin handle wrapper the classes share, which also exposes
+the device-resident entry points.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .angular import AngularSet
+from .mesh import TetMesh
+
+MeshClass = TetMesh
+AngularClass = AngularSet
+
+
+class XSType:
+    """Cross sections of one group: ``t(Nm)`` and Legendre moments ``s(0:L, Nm)``."""
+
+    def __init__(self, t, s):
+        self.t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+        self.s = np.asarray(s, dtype=np.float64).reshape(-1, self.t.size)   # (L+1, Nm)
+
+
+class Operator:
+    """One liblbhip operator handle (geometry + schedule + physics)."""
+
+    def __init__(self, mesh, ang):
+        L = _lib.lib()
+        self.mesh, self.ang = mesh, ang
+        self.NE, self.NI = mesh.NE, ang.NI
+        self.NENK = 4 * mesh.NE
+        self.n = self.NENK * self.NI
+        self.nmom = (ang.L + 1) ** 2
+        self._keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat, ang.k, ang.w)]
+        rg, node, nbr, mat, k, w = self._keep
+        self._mesh = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr),
+                                   _lib.i64ptr(mat), 0)
+        self._quad = _lib.QuadDesc(ang.NI, _lib.dptr(k), _lib.dptr(w), ang.L)
+        h = ctypes.c_void_p()
+        _lib.check(L.lb_op_create(ctypes.byref(self._mesh), ctypes.byref(self._quad), ctypes.byref(h)))
+        self.h = h
+        ne, ni, nm, ml = (ctypes.c_int64() for _ in range(4))
+        _lib.check(L.lb_op_info(h, ctypes.byref(ne), ctypes.byref(ni), ctypes.byref(nm), ctypes.byref(ml)))
+        self.maxlev = ml.value
+        self.G = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().lb_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- physics ---------------------------------------------------------------
+    def set_physics(self, sigt, sigs):
+        """One group: ``sigt`` (Nm,), ``sigs`` (Nm, L+1)."""
+        sigt = np.ascontiguousarray(np.atleast_1d(sigt), dtype=np.float64)
+        sigs = np.ascontiguousarray(np.atleast_2d(sigs), dtype=np.float64)
+        _lib.check(_lib.lib().lb_op_set_physics(self.h, sigt.size, _lib.dptr(sigt), sigs.shape[1] - 1,
+                                                _lib.dptr(sigs)))
+        self.G = 1
+        self.sigt = sigt.reshape(1, -1)
+
+    def set_physics_groups(self, sigt, sigs):
+        """``sigt`` (G, Nm), ``sigs`` (G, Nm, L+1)."""
+        sigt = np.ascontiguousarray(sigt, dtype=np.float64)
+        sigs = np.ascontiguousarray(sigs, dtype=np.float64)
+        G, Nm = sigt.shape
+        _lib.check(_lib.lib().lb_op_set_physics_groups(self.h, G, Nm, _lib.dptr(sigt), sigs.shape[2] - 1,
+                                                       _lib.dptr(sigs)))
+        self.G = G
+        self.sigt = sigt
+
+    def set_scatter_mode(self, mode):
+        m = {"moment": _lib.SCATTER_MOMENT, "dense": _lib.SCATTER_DENSE}[mode]
+        _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
+
+    # -- host entry points (reference layout v[sdof + i*NENK]) -------------------
+    def _vec(self, x):
+        x = np.array(x, dtype=np.float64, copy=True).reshape(-1)
+        if x.size != self.n:
+            raise ValueError(f"vector length {x.size} != NENK*NI = {self.n}")
+        return x
+
+    def transport_matinv(self, s):
+        x = self._vec(s)
+        _lib.check(_lib.lib().lb_transport_matinv(self.h, _lib.dptr(x), x.size))
+        return x
+
+    def scatter_matvec(self, s):
+        x = self._vec(s)
+        _lib.check(_lib.lib().lb_scatter_matvec(self.h, _lib.dptr(x), x.size))
+        return x
+
+    def boltzmann_matvec(self, s):
+        x = self._vec(s)
+        _lib.check(_lib.lib().lb_boltzmann_matvec(self.h, _lib.dptr(x), x.size))
+        return x
+
+    def boltzmann_matinv(self, s, kind="gmres", space="psi", m=10, rtol=1e-8, atol=1e-12, maxit=0, verbose=False,
+                         raise_on_fail=True):
+        x = self._vec(s)
+        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI}[kind],
+                               {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol,
+                               int(verbose))
+        st = _lib.SolveStats()
+        rc = _lib.lib().lb_boltzmann_matinv(self.h, _lib.dptr(x), x.size, ctypes.byref(opts), ctypes.byref(st))
+        if raise_on_fail:
+            _lib.check(rc)
+        return x, st.as_dict()
+
+    def fluence(self, v):
+        v = self._vec(v)
+        out = np.empty(self.NENK)
+        _lib.check(_lib.lib().lb_fluence(self.h, _lib.dptr(v), _lib.dptr(out)))
+        return out
+
+    def schedule(self):
+        sl = np.empty((self.NI, self.NE), dtype=np.int32)
+        nlev = np.empty(self.NI, dtype=np.int32)
+        _lib.check(_lib.lib().lb_op_get_schedule(self.h, _lib.i32ptr(sl), _lib.i32ptr(nlev)))
+        return sl, nlev
+
+    def normals(self):
+        n = np.empty((self.NE, 4, 3))
+        _lib.check(_lib.lib().lb_op_get_normals(self.h, _lib.dptr(n)))
+        return n
+
+    # -- device entry points (torch tensors on cuda, group-batched layout) --------
+    def transport_matinv_dev(self, q, psi, g0=0, G=1):
+        _lib.check(_lib.lib().lb_transport_matinv_dev(self.h, q.data_ptr(), psi.data_ptr(), g0, G))
+
+    def moments_dev(self, psi, phi, G=1):
+        _lib.check(_lib.lib().lb_moments_dev(self.h, psi.data_ptr(), phi.data_ptr(), G))
+
+    def moments_to_discrete_dev(self, phi, psi, g0=0, G=1, accumulate=False):
+        _lib.check(_lib.lib().lb_moments_to_discrete_dev(self.h, phi.data_ptr(), psi.data_ptr(), g0, G,
+                                                         int(accumulate)))
+
+    def synchronize(self):
+        _lib.check(_lib.lib().lb_op_synchronize(self.h))
+
+    def last_sweep_ms(self):
+        ms = ctypes.c_double()
+        _lib.check(_lib.lib().lb_last_sweep_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+
+def schedule_host(mesh, ang):
+    """The upwind schedule and normals computed by the library's host half (no GPU needed)."""
+    keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat, ang.k, ang.w)]
+    rg, node, nbr, mat, k, w = keep
+    md = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr), _lib.i64ptr(mat), 0)
+    qd = _lib.QuadDesc(ang.NI, _lib.dptr(k), _lib.dptr(w), ang.L)
+    sl = np.empty((ang.NI, mesh.NE), dtype=np.int32)
+    nlev = np.empty(ang.NI, dtype=np.int32)
+    nrm = np.empty((mesh.NE, 4, 3))
+    _lib.check(_lib.lib().lb_schedule_host(ctypes.byref(md), ctypes.byref(qd), _lib.i32ptr(sl), _lib.i32ptr(nlev),
+                                           _lib.dptr(nrm)))
+    return sl, nlev, nrm
+
+
+class SpaceAngleVector:
+    """``SpaceAngleVector(mesh, angular)``: flat FP64 ``v`` of length NENK*NI."""
+
+    def __init__(self, mesh, angular):
+        self.NENK = 4 * mesh.NE
+        self.NI = angular.NI
+        self.v = np.zeros(self.NENK * self.NI)
+
+    def Fluence(self, w):
+        """``phi(sdof) = sum_i w_i v(sdof + i NENK)`` (LinearOperatorClass.f90:154-174)."""
+        return (np.asarray(w)[:, None] * self.v.reshape(self.NI, self.NENK)).sum(axis=0)
+
+    def Destroy(self):
+        self.v = np.zeros(0)
+
+
+class _LinearOperator:
+    name = "Linear Operator"
+
+    def __init__(self):
+        self.SPACEREADY = self.ANGLEREADY = self.PHYSREADY = False
+        self.mesh = self.angular = self.XS = None
+        self._op = None
+
+    def Build(self, *args):
+        """Generic Build: (mesh, angular, XS) | (mesh) | (mesh, angular) | (angular, XS)."""
+        if len(args) == 3:
+            self.Build(args[0])
+            self.Build(args[0], args[1])
+            self.Build(args[1], args[2])
+        elif len(args) == 1:
+            self.mesh = args[0]
+            self.SPACEREADY = True
+        elif len(args) == 2 and isinstance(args[0], TetMesh):
+            mesh, ang = args
+            self.mesh, self.angular = mesh, ang
+            self.SPACEREADY = True
+            if self._op is not None:
+                self._op.close()
+            self._op = Operator(mesh, ang)
+            self.ANGLEREADY = True
+            if self.XS is not None:
+                self._physics()
+        elif len(args) == 2:
+            self.XS = args[1]
+            if self._op is not None:
+                self._physics()
+        else:
+            raise TypeError("Build(mesh, angular, XS) | Build(mesh) | Build(mesh, angular) | Build(angular, XS)")
+
+    def _physics(self):
+        L = self.angular.L
+        s = self.XS.s if self.XS.s.size else np.zeros((L + 1, self.XS.t.size))
+        self._op.set_physics(self.XS.t, np.ascontiguousarray(s.T))
+        self.PHYSREADY = True
+
+    def _ready(self, who):
+        if not (self.SPACEREADY and self.ANGLEREADY and self.PHYSREADY):
+            raise _lib.LionboltError(-3, f"{self.name}%{who} was called, but the operator was not ready.")
+
+    def Destroy(self):
+        if self._op is not None:
+            self._op.close()
+        self._op = None
+        self.SPACEREADY = self.ANGLEREADY = self.PHYSREADY = False
+
+
+class TransportOp(_LinearOperator):
+    name = "Transport Operator"
+
+    def MatInv(self, s):
+        self._ready("MatInv")
+        s.v[:] = self._op.transport_matinv(s.v)
+
+    def MatVec(self, x):
+        raise _lib.LionboltError(-1, "TransportOp%MatVec was called, but the operator has yet to be implemented.")
+
+
+class ScatteringOp(_LinearOperator):
+    name = "Scattering Operator"
+
+    def MatVec(self, x):
+        self._ready("MatVec")
+        x.v[:] = self._op.scatter_matvec(x.v)
+
+    def MatInv(self, s):
+        raise _lib.LionboltError(-1, "ScatteringOp%MatInv was called, but this operation is undefined.")
+
+
+class BoltzmannOp(_LinearOperator):
+    """``BoltzmannOp`` with ``solver`` 'GMRES' or 'SI' (the angular set's solver in the reference)."""
+
+    name = "Boltzmann Operator"
+
+    def __init__(self, solver="GMRES", space="psi", restart=10):
+        super().__init__()
+        self.solver = solver
+        self.space = space
+        self.restart = restart
+        self.stats = None
+
+    def MatVec(self, x):
+        self._ready("MatVec")
+        x.v[:] = self._op.boltzmann_matvec(x.v)
+
+    def MatInv(self, s):
+        self._ready("MatInv")
+        out, st = self._op.boltzmann_matinv(s.v, kind=self.solver.lower(), space=self.space, m=self.restart)
+        s.v[:] = out
+        self.stats = st
