@@ -147,6 +147,15 @@ int lb_moments_dev(lb_op* op, const double* d_psi, double* d_phi, int G);
 /* psi[i][sdof][g] = sum_lm M[i][lm] sigma_l(mat,g) phi[lm][sdof][g] (+ psi if accumulate). */
 int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, int g0, int G, int accumulate);
 
+/* Multigroup downscatter solve (the group loop of Main.f90:326-476 with the
+ * energy-coupling source of PIEISource, Main.f90:584-629): groups in order,
+ * each a within-group BoltzmannOp%MatInv with source
+ * M (qmom_g + sum_{g'<g} sigma_l(g'->g) phi_g'), flux moments kept in HBM.
+ * d_qmom [G][nmom][NENK] or NULL; transfer (host) [G][G][Nm][L+1] indexed
+ * [g'][g]; d_phi [G][nmom][NENK] out; stats[G] out.  Synchronous. */
+int lb_multigroup_solve_dev(lb_op* op, const double* d_qmom, const double* transfer, const lb_solver_opts* opts,
+                            double* d_phi, lb_solve_stats* stats);
+
 /* Block until the operator's stream is idle. */
 int lb_op_synchronize(lb_op* op);
 

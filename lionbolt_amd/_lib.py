@@ -64,6 +64,8 @@ SIGNATURES = {
     "lb_transport_matinv_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "lb_moments_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_moments_to_discrete_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "lb_multigroup_solve_dev": (ctypes.c_int, [_vp, _vp, _d, ctypes.POINTER(SolverOpts), _vp,
+                                               ctypes.POINTER(SolveStats)]),
     "lb_op_synchronize": (ctypes.c_int, [_vp]),
     "lb_last_sweep_ms": (ctypes.c_int, [_vp, _d]),
     "lb_op_destroy": (None, [_vp]),

@@ -311,6 +311,16 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
   return launch_moments_to_discrete(op, d_phi, d_psi, g0, G, accumulate);
 }
 
+int lb_multigroup_solve_dev(lb_op* op, const double* d_qmom, const double* transfer, const lb_solver_opts* opts,
+                            double* d_phi, lb_solve_stats* stats) {
+  LB_TRY(check_ready(op, "lb_multigroup_solve_dev"));
+  if (!transfer || !d_phi) {
+    set_error("lb_multigroup_solve_dev: null argument");
+    return LB_E_ARG;
+  }
+  return multigroup_solve_dev(op, d_qmom, transfer, opts, d_phi, stats);
+}
+
 int lb_op_synchronize(lb_op* op) {
   if (!op) {
     set_error("lb_op_synchronize: null operator");

@@ -93,7 +93,7 @@ struct lb_op {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_sweep_ms = 0.f;
   // workspaces
-  lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host;
+  lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host, w_mg, w_acc, w_S;
 };
 
 namespace lb {
@@ -109,7 +109,10 @@ LevelTable* level_table(lb_op* op, int G, int* rc);
 // lb_kernels.hip (launchers; all enqueue on op->stream)
 int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G);
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols);
-int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate);
+int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate,
+                               bool scaled = true);
+int launch_downscatter(lb_op* op, const double* phi_all, const double* S, int g, int G, const double* qmom,
+                       double* acc);
 int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G);
 int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* z);
 int launch_scale(hipStream_t s, int64_t n, double a, double* x);
@@ -121,5 +124,7 @@ int launch_fluence(lb_op* op, const double* v, double* phi);
 int boltzmann_matvec_dev(lb_op* op, double* x, int g0);
 int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* opts, lb_solve_stats* st);
 int scatter_dev(lb_op* op, const double* x, double* y, int g0, int G);
+int multigroup_solve_dev(lb_op* op, const double* qmom, const double* transfer, const lb_solver_opts* opts,
+                        double* phi_out, lb_solve_stats* stats);
 
 }  // namespace lb

@@ -182,11 +182,16 @@ __global__ __launch_bounds__(kBlock) void moments_to_discrete_kernel(
   if (col >= ncols) return;
   const int64_t sd = col / G;
   const int g = int(col - sd * G);
-  const int m_el = mat[sd >> 2];
-  const double* sl = sigs + ((int64_t)m_el * Gtot + g0 + g) * (L + 1);
   double sp[NM];
+  if (sigs) {
+    const int m_el = mat[sd >> 2];
+    const double* sl = sigs + ((int64_t)m_el * Gtot + g0 + g) * (L + 1);
 #pragma unroll
-  for (int m = 0; m < NM; ++m) sp[m] = phi[(int64_t)m * ncols + col] * sl[deg[m]];
+    for (int m = 0; m < NM; ++m) sp[m] = phi[(int64_t)m * ncols + col] * sl[deg[m]];
+  } else {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) sp[m] = phi[(int64_t)m * ncols + col];
+  }
   for (int i = 0; i < NI; ++i) {
     double acc = 0.0;
 #pragma unroll
@@ -221,6 +226,27 @@ __global__ __launch_bounds__(kBlock) void scatter_dense_kernel(const double* __r
     for (int j = 0; j < 16; ++j)
       if (j < nj) y[(int64_t)(i0 + j) * ncols + col] = acc[j];
   }
+}
+
+// Energy-coupling source of group g in moment form (the EI loop of
+// Main.f90:584-629 with K_{g'->g} = M diag(sigma_l(g'->g)) D):
+// acc[lm][sd] = qmom[lm][sd] + sum_{g' < g} sigma_{l(lm)}(g' -> g, mat) phi_{g'}[lm][sd].
+__global__ __launch_bounds__(kBlock) void downscatter_kernel(const double* __restrict__ phi_all,
+                                                             const double* __restrict__ S, int g, int G, int Nm,
+                                                             int nmom, int L, const int32_t* __restrict__ mat,
+                                                             const int32_t* __restrict__ deg, int64_t NENK,
+                                                             const double* __restrict__ qmom,
+                                                             double* __restrict__ acc) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= (int64_t)nmom * NENK) return;
+  const int lm = int(j / NENK);
+  const int64_t sd = j - (int64_t)lm * NENK;
+  const int m_el = mat[sd >> 2];
+  const int l = deg[lm];
+  double a = qmom ? qmom[j] : 0.0;
+  for (int gp = 0; gp < g; ++gp)
+    a += S[(((int64_t)gp * G + g) * Nm + m_el) * (L + 1) + l] * phi_all[(int64_t)gp * nmom * NENK + j];
+  acc[j] = a;
 }
 
 __global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, double a, const double* x, double b, const double* y,
@@ -332,12 +358,13 @@ int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t nco
   return LB_OK;
 }
 
-int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate) {
+int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate, bool scaled) {
   const int64_t ncols = op->NENK * G;
+  const double* sigs = scaled ? op->d_sigs.as<const double>() : nullptr;
 #define CALL(NMV)                                                                                                 \
   hipLaunchKernelGGL(moments_to_discrete_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream,       \
                      op->d_M.as<double>(), op->d_deg.as<const int32_t>(), (int)op->NI, ncols, G, op->G, g0, op->L,             \
-                     op->d_mat.as<const int32_t>(), op->d_sigs.as<const double>(), phi, psi, accumulate)
+                     op->d_mat.as<const int32_t>(), sigs, phi, psi, accumulate)
   LB_NM_DISPATCH(op->nmom, CALL)
 #undef CALL
   LB_HIP(hipGetLastError());
@@ -349,6 +376,16 @@ int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G) {
   const int64_t ncols = op->NENK * G;
   hipLaunchKernelGGL(scatter_dense_kernel, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream, op->d_yys.as<double>(),
                      (int)op->NI, ncols, G, op->d_mat.as<const int32_t>(), x, y);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_downscatter(lb_op* op, const double* phi_all, const double* S, int g, int G, const double* qmom,
+                       double* acc) {
+  const int64_t tot = (int64_t)op->nmom * op->NENK;
+  hipLaunchKernelGGL(downscatter_kernel, dim3(grid_for(tot)), dim3(kBlock), 0, op->stream, phi_all, S, g, G, op->Nm,
+                     op->nmom, op->L, op->d_mat.as<const int32_t>(), op->d_deg.as<const int32_t>(), op->NENK, qmom,
+                     acc);
   LB_HIP(hipGetLastError());
   return LB_OK;
 }

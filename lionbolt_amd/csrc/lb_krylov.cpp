@@ -380,4 +380,43 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
   return LB_OK;
 }
 
+// Multigroup downscatter driver: the group loop of Main.f90:326-476 with the
+// energy-coupling source of PIEISource (Main.f90:584-629).  Groups are solved in
+// order; each group's flux moments phi_g = D psi_g stay in HBM and feed the
+// coupling source of later groups in moment form, replacing the reference's
+// per-group angular-flux tapes and its O(G^2) dense K applications.
+//   qmom     [G][nmom][NENK] external (first-collision) source moments, or null
+//   transfer host [G][G][Nm][L+1]: sigma_l(g' -> g) at [g'][g]; the diagonal
+//            g' = g must equal the within-group moments set with the physics
+//   phi_out  [G][nmom][NENK] collided flux moments
+int multigroup_solve_dev(lb_op* op, const double* qmom, const double* transfer, const lb_solver_opts* opts,
+                         double* phi_out, lb_solve_stats* stats) {
+  const int G = op->G;
+  const int64_t n = op->NENK * op->NI;
+  const int64_t nphi = (int64_t)op->nmom * op->NENK;
+  const size_t nS = (size_t)G * G * op->Nm * (op->L + 1);
+  LB_TRY(op->w_S.reserve(nS * sizeof(double)));
+  LB_HIP(hipMemcpyAsync(op->w_S.p, transfer, nS * sizeof(double), hipMemcpyHostToDevice, op->stream));
+  LB_TRY(op->w_mg.reserve(n * sizeof(double)));
+  LB_TRY(op->w_acc.reserve(nphi * sizeof(double)));
+  double* psi = op->w_mg.as<double>();
+  double* acc = op->w_acc.as<double>();
+  int rc_all = LB_OK;
+  for (int g = 0; g < G; ++g) {
+    LB_TRY(launch_downscatter(op, phi_out, op->w_S.as<double>(), g, G, qmom ? qmom + (int64_t)g * nphi : nullptr,
+                              acc));
+    LB_TRY(launch_moments_to_discrete(op, acc, psi, g, 1, 0, false));
+    lb_solve_stats st{};
+    int rc = boltzmann_matinv_dev(op, psi, g, opts, &st);
+    if (stats) stats[g] = st;
+    if (rc != LB_OK) {
+      if (rc != LB_E_MAXIT && rc != LB_E_BREAKDOWN && rc != LB_E_NAN) return rc;
+      rc_all = rc;
+    }
+    LB_TRY(launch_moments(op, psi, phi_out + (int64_t)g * nphi, 1, op->NENK));
+  }
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return rc_all;
+}
+
 }  // namespace lb

@@ -1,0 +1,70 @@
+!  Driver in the shape of validation/operators/TransportMMS.f90:47-89, running on
+!  the GPU through the LionboltHIP shim.
+!
+!  Usage: transport_mms <input.bin> <output.bin>
+!  input (stream, native endian): NE, NK, NI, L (int64); rg(3,NK), k(3,NI), w(NI)
+!  (real64); node(4,NE), neighbors(4,NE), mat(NE) (int64, 1-based, boundary 0);
+!  sigma (real64), c (real64: scattering ratio of the Boltzmann solve).
+!  output: T^-1 s for the MMS source (NENK*NI real64), then the scalar flux of
+!  BoltzmannOp%MatInv of a unit source (NENK real64).  Prints the RMS MMS error.
+Program transport_mms
+    use iso_c_binding
+    use LionboltHIP
+    Implicit None
+    Type (HIPOperator)               :: OpT, OpL
+    Integer (c_int64_t)              :: NE, NK, NI, L, e, i, kk, sdof, n
+    Real (c_double),     Allocatable :: rg (:,:), kv (:,:), w (:), s (:), strue (:), phi (:), sigs (:,:)
+    Integer (c_int64_t), Allocatable :: node (:,:), nbr (:,:), mat (:)
+    Real (c_double)                  :: sigma, c, r (3), err
+    Real (c_double), Parameter       :: A = 10.0_c_double
+    Character (LEN=512)              :: fin, fout
+    Integer                          :: u
+
+    call get_command_argument(1, fin)
+    call get_command_argument(2, fout)
+    open (newunit=u, file=TRIM(fin), access='stream', form='unformatted', status='old')
+    read (u) NE, NK, NI, L
+    ALLOCATE(rg(3, NK), kv(3, NI), w(NI), node(4, NE), nbr(4, NE), mat(NE))
+    read (u) rg, kv, w, node, nbr, mat, sigma, c
+    close (u)
+
+    ! TransportMMS: psi = exp(-A r^2), source (-2A k.r + sigma) exp(-A r^2)
+    call OpT%BuildSpaceAngle(rg, node, nbr, mat, kv, w, INT(L), 'GMRES')
+    ALLOCATE(sigs(0:L, 1))
+    sigs = 0.0_c_double
+    call OpT%BuildPhysics([sigma], sigs)
+    n = 4 * NE * NI
+    ALLOCATE(s(n), strue(n))
+    do i = 1, NI
+        do e = 1, NE
+            do kk = 1, 4
+                sdof = 4 * (e - 1) + kk
+                r = rg(:, node(kk, e))
+                s(sdof + (i - 1) * 4 * NE) = (-2.0_c_double * A * DOT_PRODUCT(kv(:, i), r) + sigma) &
+                                             * EXP(-A * DOT_PRODUCT(r, r))
+                strue(sdof + (i - 1) * 4 * NE) = EXP(-A * DOT_PRODUCT(r, r))
+            end do
+        end do
+    end do
+    call OpT%TransportMatInv(s)
+    err = SQRT(SUM((s - strue)**2) / n)
+    write (*, '(A,ES12.4)') 'TransportMMS RMS error: ', err
+
+    ! BoltzmannOp%MatInv of a unit isotropic source with scattering ratio c
+    call OpL%BuildSpaceAngle(rg, node, nbr, mat, kv, w, INT(L), 'GMRES')
+    sigs = 0.0_c_double
+    sigs(0, 1) = c
+    call OpL%BuildPhysics([1.0_c_double], sigs)
+    strue = 1.0_c_double
+    call OpL%BoltzmannMatInv(strue)
+    phi = OpL%Fluence(strue)
+    write (*, '(A,I0,A,ES12.4)') 'BoltzmannOp%MatInv iterates: ', OpL%stats%iterates, &
+        '  true residual: ', OpL%stats%true_residual
+
+    open (newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
+    write (u) s, phi
+    close (u)
+    call OpT%Destroy()
+    call OpL%Destroy()
+    print *, 'transport_mms - Successfully terminated.'
+End Program

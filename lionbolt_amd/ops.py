@@ -42,7 +42,7 @@ class Operator:
         self.NENK = 4 * mesh.NE
         self.n = self.NENK * self.NI
         self.nmom = (ang.L + 1) ** 2
-        self._keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat, ang.k, ang.w)]
+        self._keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat - 1, ang.k, ang.w)]
         rg, node, nbr, mat, k, w = self._keep
         self._mesh = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr),
                                    _lib.i64ptr(mat), 0)
@@ -149,6 +149,19 @@ class Operator:
         _lib.check(_lib.lib().lb_moments_to_discrete_dev(self.h, phi.data_ptr(), psi.data_ptr(), g0, G,
                                                          int(accumulate)))
 
+    def multigroup_solve_dev(self, transfer, phi, qmom=None, kind="gmres", space="moment", m=30, rtol=1e-8,
+                             atol=1e-12, maxit=0, raise_on_fail=True):
+        """Group-by-group downscatter solve; ``transfer`` (G, G, Nm, L+1) host array [g', g]."""
+        transfer = np.ascontiguousarray(transfer, dtype=np.float64)
+        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI}[kind],
+                               {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol, 0)
+        stats = (_lib.SolveStats * self.G)()
+        rc = _lib.lib().lb_multigroup_solve_dev(self.h, None if qmom is None else qmom.data_ptr(),
+                                                _lib.dptr(transfer), ctypes.byref(opts), phi.data_ptr(), stats)
+        if raise_on_fail:
+            _lib.check(rc)
+        return [s.as_dict() for s in stats]
+
     def synchronize(self):
         _lib.check(_lib.lib().lb_op_synchronize(self.h))
 
@@ -160,7 +173,7 @@ class Operator:
 
 def schedule_host(mesh, ang):
     """The upwind schedule and normals computed by the library's host half (no GPU needed)."""
-    keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat, ang.k, ang.w)]
+    keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat - 1, ang.k, ang.w)]
     rg, node, nbr, mat, k, w = keep
     md = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr), _lib.i64ptr(mat), 0)
     qd = _lib.QuadDesc(ang.NI, _lib.dptr(k), _lib.dptr(w), ang.L)

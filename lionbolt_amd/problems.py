@@ -1,0 +1,95 @@
+"""Seeded synthetic workloads of the transport-solve path.
+
+Decisions recorded here (the reference never ran these configurations):
+
+* cfg1: 10 cm cube, 20^3 hexes x 6 tets (48,000), LQ S8, one group, P0,
+  sigma_t = 1.0, sigma_s = 0.9 /cm, vacuum; source q_i = 1/(4 pi) on every node
+  of elements whose centroid lies in the central cube [4, 6]^3.
+* cfg2: 40^3 x 6 = 384,000 tets, Gauss-Legendre 8 polar x Chebyshev 32 azimuthal
+  (256 directions), P3, 24 groups, downscatter only.  sigma_t,g log-uniform on
+  [0.02, 0.5] /cm (SplitMix64 stream 1002); the slab z in [4, 6] cm (elements by
+  centroid) is material 2 with 3x all cross sections.  Transfer moments
+  sigma_0(g' -> g) for g >= g' follow 0.5^(g - g') normalised so that the
+  out-scatter of g' is 0.85 sigma_t,g' (the "row sum" read as out-scatter);
+  sigma_l = 0.6^l sigma_0.
+* first-collision beam: a unit mono-directional beam along +z entering the face
+  z = 0 in group 1.  Its uncollided flux at a node is exp(-tau(z)) with the
+  optical depth of the planar slab layout; the first-collision source of group g
+  is sum_l (2l+1)/(4 pi) sigma_l(1 -> g) P_l(k . z) phi_0, i.e. moments
+  sigma_l(1 -> g) Y_lm(z) phi_0 (nodal interpolation, where the reference
+  projects a ray-traced source with the inverse mass matrix, FCSSource.f90:134-215).
+* cfg5: cfg2 with a within-group scattering ratio of 0.99 in every group and no
+  downscatter (scattering-dominated, the stated purpose of the configuration).
+"""
+import math
+
+import numpy as np
+
+from . import angular, mesh, rng
+
+
+def cfg1():
+    m = mesh.jittered_cube(20, a=10.0, jitter=0.2, seed=1001)
+    ang = angular.level_symmetric_s8(0)
+    c = m.centroids()
+    inside = np.all((c >= 4.0) & (c <= 6.0), axis=1)
+    q = np.tile(np.repeat(np.where(inside, 1.0 / (4 * math.pi), 0.0), 4), ang.NI)
+    return m, ang, np.array([1.0]), np.array([[0.9]]), q
+
+
+def cfg2_mesh(n=40):
+    return mesh.jittered_cube(n, a=10.0, jitter=0.2, seed=1001, slab=(4.0, 6.0))
+
+
+def cfg2_angular():
+    return angular.product(8, 32, 3)
+
+
+def cfg2_xs(G=24, L=3, ratio=0.85, within_only=False):
+    """sigma_t (G, 2) and transfer moments S (G, G, 2, L+1) indexed [g', g, mat, l]."""
+    u = rng.uniform(1002, G)
+    st = np.exp(np.log(0.02) + u * (np.log(0.5) - np.log(0.02)))
+    sigt = np.stack([st, 3.0 * st], axis=1)
+    S = np.zeros((G, G, 2, L + 1))
+    for gp in range(G):
+        if within_only:
+            share = np.zeros(G)
+            share[gp] = 1.0
+        else:
+            w = 0.5 ** np.arange(G - gp)
+            share = np.zeros(G)
+            share[gp:] = w / w.sum()
+        for g in range(gp, G):
+            for mat in range(2):
+                s0 = ratio * sigt[gp, mat] * share[g]
+                S[gp, g, mat, :] = s0 * 0.6 ** np.arange(L + 1)
+    return sigt, S
+
+
+def within_group(S):
+    """Within-group moments (G, Nm, L+1) from a transfer table."""
+    G = S.shape[0]
+    return np.stack([S[g, g] for g in range(G)])
+
+
+def beam_uncollided(m, sigt_group0, a=10.0, slab=(4.0, 6.0)):
+    """exp(-tau) at every spatial dof for the +z beam (planar slab layout)."""
+    z = m.dof_coords()[:, 2]
+    thick = np.clip(z - slab[0], 0.0, slab[1] - slab[0])
+    tau = sigt_group0[0] * (z - thick) + sigt_group0[1] * thick
+    return np.exp(-tau)
+
+
+def beam_source_moments(m, ang, S, sigt):
+    """First-collision source moments qmom (G, nmom, NENK) of the +z beam."""
+    G = S.shape[0]
+    L = ang.L
+    phi0 = beam_uncollided(m, sigt[0])
+    y0 = angular.real_sph_harm(L, np.array([[0.0, 0.0, 1.0]]))[:, 0]      # Y_lm(k0)
+    deg = angular.moment_degree(L)
+    mat_dof = np.repeat(m.mat - 1, 4)
+    q = np.empty((G, (L + 1) ** 2, m.NENK))
+    for g in range(G):
+        sig = S[0, g][mat_dof][:, deg].T            # (nmom, NENK)
+        q[g] = sig * y0[:, None] * phi0[None, :]
+    return q, phi0
