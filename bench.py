@@ -15,12 +15,12 @@ The line also carries:
 * cpu_baseline: the FP64 C++/OpenMP oracle (restatement of the reference's
   per-direction OpenMP sweep) timed on this host on a bounded sample: one
   group, 32 of the 256 directions, the full 384k-tet mesh;
-* solve: time to a 1e-6 relative transport residual of the single-group
-  within-group solve (plain GMRES on the device) on the same mesh.
+* solve: time to a 1e-6 relative transport residual of the full 24-group,
+  P3, downscatter problem with the first-collision source of a +z beam
+  (group-by-group within-group GMRES on flux moments, plain, FP64).
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -32,16 +32,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CAG = 64
-
-
-def cfg2_inputs(n=40, G=24):
-    from lionbolt_amd import angular, mesh, rng
-    m = mesh.jittered_cube(n, a=10.0, jitter=0.2, seed=1001, slab=(4.0, 6.0))
-    ang = angular.product(8, 32, 3)
-    u = rng.uniform(1002, G)
-    sigt1 = np.exp(np.log(0.02) + u * (np.log(0.5) - np.log(0.02)))     # log-uniform [0.02, 0.5]
-    sigt = np.stack([sigt1, 3.0 * sigt1], axis=1)                        # slab: 3x density
-    return m, ang, sigt
 
 
 def cpu_baseline(m, ang, ndirs=32, reps=2):
@@ -64,24 +54,102 @@ def cpu_baseline(m, ang, ndirs=32, reps=2):
             "sample": f"1 group x {sub.NI} of 256 directions x {m.NE} tets, FP64 oracle sweep, median of {reps}"}
 
 
+def copy_calibration(gib=4):
+    """A plain device copy of known size (for PMC byte-counter scaling) and its bandwidth."""
+    import torch
+    n = gib * (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": gib * (1 << 30), "bytes_written": gib * (1 << 30),
+            "copy_GBs": 2 * gib * (1 << 30) / dt / 1e9}
+
+
+def solve_leg(m, ang, rtol):
+    """cfg2 multigroup downscatter solve with the +z beam first-collision source."""
+    import torch
+    from lionbolt_amd import angular, problems
+    from lionbolt_amd.ops import Operator
+    G = 24
+    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, problems.within_group(S))
+    # first-collision source moments built on the device (G x nmom x NENK)
+    phi0 = torch.from_numpy(problems.beam_uncollided(m, sigt[0])).cuda()
+    y0 = torch.from_numpy(angular.real_sph_harm(ang.L, np.array([[0.0, 0.0, 1.0]]))[:, 0]).cuda()
+    deg = torch.from_numpy(angular.moment_degree(ang.L)).cuda()
+    mat_dof = torch.from_numpy(np.repeat(m.mat - 1, 4)).cuda()
+    S_d = torch.from_numpy(S).cuda()
+    qmom = torch.empty((G, ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
+    for g in range(G):
+        qmom[g] = S_d[0, g][mat_dof][:, deg].T * y0[:, None] * phi0[None, :]
+    phi = torch.zeros_like(qmom)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    op.close()
+    return {"seconds": wall, "groups": G, "rtol": rtol,
+            "max_true_residual": max(s["true_residual"] for s in stats),
+            "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
+            "method": "plain GMRES(30) on flux moments per group, FP64"}
+
+
+def single_group_leg(m, ang, reps=10):
+    """One-group sweeps of the same mesh and directions (what a within-group solve
+    issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s."""
+    import torch
+    from lionbolt_amd import problems
+    from lionbolt_amd.ops import Operator
+    sigt, _ = problems.cfg2_xs(G=1, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, np.zeros((1, 2, ang.L + 1)))
+    q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(q)
+    res = {}
+    for mode in ("levels", "graph"):
+        op.set_sweep_mode(mode)
+        op.transport_matinv_dev(q, psi, 0, 1)
+        op.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.transport_matinv_dev(q, psi, 0, 1)
+        op.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / reps
+        res[mode] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": m.NE * ang.NI / (ms * 1e-3)}
+    op.close()
+    return res
+
+
 def run(args):
     import torch
+    from lionbolt_amd import problems
     from lionbolt_amd.ops import Operator
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-        sys.exit(2)
     torch.cuda.set_device(0)
+    out_extra = {}
+    if args.calib:
+        out_extra["copy_calibration"] = copy_calibration()
     t0 = time.perf_counter()
-    m, ang, sigt = cfg2_inputs(args.n, args.groups)
+    m = problems.cfg2_mesh(args.n)
+    ang = problems.cfg2_angular()
     G = args.groups
+    sigt, _ = problems.cfg2_xs(G=G, L=ang.L)
     op = Operator(m, ang)
-    sigs = np.zeros((G, 2, ang.L + 1))
-    op.set_physics_groups(sigt, sigs)
+    op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
     setup_s = time.perf_counter() - t0
     cag = m.NE * ang.NI * G
-    nvals = cag * 4
-    q = torch.rand(nvals, dtype=torch.float64, device="cuda")
+    q = torch.rand(cag * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
@@ -96,12 +164,14 @@ def run(args):
     op.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_step = 1000.0 * wall / args.steps
+    ms_step = 1000.0 * wall / max(args.steps, 1)
     value = cag * args.steps / wall
-    ev_ms = float(np.mean(sweep_ms))
+    ev_ms = float(np.mean(sweep_ms)) if sweep_ms else float("nan")
     achieved = BYTES_PER_CAG * cag / (ev_ms * 1e-3) / 1e9
-    if not torch.isfinite(psi).all():
+    if not bool(torch.isfinite(psi[:: 1 << 16]).all()):
         raise RuntimeError("non-finite flux")
+    levels = op.maxlev
+    op.close()
     del q, psi
     torch.cuda.empty_cache()
     out = {
@@ -119,16 +189,29 @@ def run(args):
         "data": "synthetic (seeded jittered-cube mesh and cross sections)",
         "config": {"workload": f"DFEM SN sweep, {m.NE} tets, {ang.NI} directions (GL8 x C32), {G} groups, "
                                f"P{ang.L}, level-synchronous, in HBM",
-                   "tets": m.NE, "directions": ang.NI, "groups": G, "levels_max": op.maxlev,
+                   "tets": m.NE, "directions": ang.NI, "groups": G, "levels_max": levels,
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "sweep_level_kernel", "bytes_per_unit": BYTES_PER_CAG,
                      "event_ms_per_sweep": ev_ms},
     }
+    traffic_file = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    if os.path.exists(traffic_file) and args.n == 40 and G == 24:
+        # HBM bytes per launch from the two PMC passes (FETCH_SIZE, WRITE_SIZE) recorded there
+        with open(traffic_file) as fh:
+            tr = json.load(fh)
+        per_cag = tr["fetch_bytes_per_cag"] + tr["write_bytes_per_cag"]
+        out["roofline"]["traffic"] = per_cag * cag / levels
+        out["roofline"]["traffic_unit"] = "bytes per launch (PMC, " + tr["source"] + ")"
+        out["roofline"]["traffic_bytes_per_unit"] = per_cag
+    out.update(out_extra)
+    if args.single:
+        out["single_group_sweep"] = single_group_leg(m, ang)
+    if args.solve:
+        out["solve"] = solve_leg(m, ang, args.solve_rtol)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, ang)
-    op.close()
     print(json.dumps(out))
 
 
@@ -140,9 +223,13 @@ def main():
     p.add_argument("--n", type=int, default=40, help="hexes per side (40 -> 384,000 tets)")
     p.add_argument("--groups", type=int, default=24)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--no-solve", dest="solve", action="store_false")
+    p.add_argument("--no-single", dest="single", action="store_false")
+    p.add_argument("--solve-rtol", type=float, default=1e-7)
+    p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     args = p.parse_args()
-    if int(os.environ.get("WORLD_SIZE", "1")) > 1 or args.gpus > 1:
-        print(json.dumps({"error": "one GPU per run: nothing here uses a second GPU"}))
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1 or args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no collective path is built)"}))
         sys.exit(2)
     run(args)
 

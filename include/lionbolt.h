@@ -45,6 +45,7 @@ enum {
 enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
 enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
 enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
+enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1 };
 
 typedef struct lb_op lb_op;
 
@@ -104,6 +105,12 @@ int lb_op_set_physics_groups(lb_op* op, int G, int Nm, const double* sigma_t, in
  * reference does (ScatteringAlgebra.f90:30-53); LB_SCATTER_MOMENT applies the
  * identical K = M diag(sigma_l) D (default). */
 int lb_op_set_scatter_mode(lb_op* op, int mode);
+
+/* How a sweep is issued: LB_SWEEP_LEVELS launches one kernel per global level
+ * on the operator's stream; LB_SWEEP_GRAPH (default) captures those launches
+ * once per (source, flux, group range) into a HIP graph and replays it.  No
+ * reference counterpart (the reference sweeps with OpenMP threads). */
+int lb_op_set_sweep_mode(lb_op* op, int mode);
 
 /* TransportOp%MatInv (TransportOperator.f90:72-151): s <- T^-1 s in place. */
 int lb_transport_matinv(lb_op* op, double* s, int64_t n);

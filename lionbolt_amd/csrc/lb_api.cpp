@@ -15,6 +15,11 @@ thread_local std::string g_error;
 
 void set_error(const std::string& msg) { g_error = msg; }
 
+void clear_graphs(lb_op* op) {
+  for (auto& sg : op->graphs) (void)hipGraphExecDestroy(sg.exec);
+  op->graphs.clear();
+}
+
 int DevBuf::reserve(size_t n) {
   if (n <= bytes && p) return LB_OK;
   if (p) {
@@ -81,6 +86,7 @@ int to_host(lb_op* op, double* h, int64_t n) {
 }
 
 int upload_physics(lb_op* op) {
+  clear_graphs(op);  // captured sweeps hold the physics-table pointer
   LB_TRY(op->d_sigt.reserve(op->h_sigt.size() * sizeof(double)));
   LB_TRY(op->d_sigs.reserve(op->h_sigs.size() * sizeof(double)));
   LB_HIP(hipMemcpy(op->d_sigt.p, op->h_sigt.data(), op->h_sigt.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -197,6 +203,17 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
     return LB_E_ARG;
   }
   op->scatter_mode = mode;
+  return LB_OK;
+}
+
+int lb_op_set_sweep_mode(lb_op* op, int mode) {
+  if (!op || (mode != LB_SWEEP_LEVELS && mode != LB_SWEEP_GRAPH)) {
+    set_error("lb_op_set_sweep_mode: invalid mode");
+    return LB_E_ARG;
+  }
+  if (op->stream) (void)hipStreamSynchronize(op->stream);
+  clear_graphs(op);
+  op->sweep_mode = mode;
   return LB_OK;
 }
 
@@ -345,6 +362,7 @@ int lb_last_sweep_ms(lb_op* op, double* ms) {
 void lb_op_destroy(lb_op* op) {
   if (!op) return;
   if (op->stream) (void)hipStreamSynchronize(op->stream);
+  clear_graphs(op);
   for (DevBuf* b : {&op->d_geo, &op->d_nbr, &op->d_perm, &op->d_mat, &op->d_tasks, &op->d_k, &op->d_w, &op->d_D,
                     &op->d_M, &op->d_deg, &op->d_yys, &op->d_sigt, &op->d_sigs, &op->w_a, &op->w_a2, &op->w_b,
                     &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov})

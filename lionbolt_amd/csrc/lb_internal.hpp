@@ -52,6 +52,15 @@ struct LevelTable {
 // Local faces of a tetrahedron (FiniteElementAnalysis.f90:129-132), 0-based.
 constexpr int kFaceNodes[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 
+struct SweepGraph {
+  const double* q;
+  double* psi;
+  int g0, G;
+  hipGraphExec_t exec;
+};
+
+void clear_graphs(lb_op* op);
+
 }  // namespace lb
 
 struct lb_op {
@@ -88,6 +97,8 @@ struct lb_op {
   std::vector<double> h_sigs;       // [Nm][G][L+1]
   lb::DevBuf d_sigt, d_sigs;
   int scatter_mode = LB_SCATTER_MOMENT;
+  int sweep_mode = LB_SWEEP_GRAPH;
+  std::vector<lb::SweepGraph> graphs;
   // runtime
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -318,15 +318,52 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   LevelTable* t = level_table(op, G, &rc);
   if (!t) return rc;
   const int4* work = t->d_entries.as<const int4>();
-  LB_HIP(hipEventRecord(op->ev0, op->stream));
-  for (int l = 0; l < op->maxlev; ++l) {
-    const int nb = t->level_off[l + 1] - t->level_off[l];
-    if (nb == 0) continue;
-    hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l],
-                       op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
-                       op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
-                       op->d_sigt.as<const double>(), op->G, g0, G, op->NE, q, psi);
+  auto enqueue = [&]() {
+    for (int l = 0; l < op->maxlev; ++l) {
+      const int nb = t->level_off[l + 1] - t->level_off[l];
+      if (nb == 0) continue;
+      hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l],
+                         op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
+                         op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
+                         op->d_sigt.as<const double>(), op->G, g0, G, op->NE, q, psi);
+    }
+  };
+  if (op->sweep_mode == LB_SWEEP_GRAPH) {
+    // The level launches of one (source, flux, group range) are captured once into a
+    // graph and replayed; the cache is cleared whenever the physics table moves.
+    hipGraphExec_t exec = nullptr;
+    for (auto& sg : op->graphs)
+      if (sg.q == q && sg.psi == psi && sg.g0 == g0 && sg.G == G) exec = sg.exec;
+    if (!exec) {
+      LB_HIP(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      hipGraph_t graph = nullptr;
+      hipError_t e1 = hipGetLastError();
+      hipError_t e2 = hipStreamEndCapture(op->stream, &graph);
+      if (e1 != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_error(std::string("sweep graph capture failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        return LB_E_HIP;
+      }
+      hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e3 != hipSuccess) {
+        set_error(std::string("sweep graph instantiation failed: ") + hipGetErrorString(e3));
+        return LB_E_HIP;
+      }
+      if (op->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(op->graphs.front().exec);
+        op->graphs.erase(op->graphs.begin());
+      }
+      op->graphs.push_back({q, psi, g0, G, exec});
+    }
+    LB_HIP(hipEventRecord(op->ev0, op->stream));
+    LB_HIP(hipGraphLaunch(exec, op->stream));
+    LB_HIP(hipEventRecord(op->ev1, op->stream));
+    return LB_OK;
   }
+  LB_HIP(hipEventRecord(op->ev0, op->stream));
+  enqueue();
   LB_HIP(hipGetLastError());
   LB_HIP(hipEventRecord(op->ev1, op->stream));
   return LB_OK;

@@ -87,6 +87,10 @@ class Operator:
         m = {"moment": _lib.SCATTER_MOMENT, "dense": _lib.SCATTER_DENSE}[mode]
         _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
 
+    def set_sweep_mode(self, mode):
+        m = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH}[mode]
+        _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, m))
+
     # -- host entry points (reference layout v[sdof + i*NENK]) -------------------
     def _vec(self, x):
         x = np.array(x, dtype=np.float64, copy=True).reshape(-1)

@@ -193,3 +193,24 @@ def test_reference_api_mirror(ops_module):
     L.MatInv(s)
     ref, _ = orc.boltzmann_matinv(np.ones(orc.n))
     assert rel(s.v, ref) <= 1e-7
+
+
+def test_graph_and_level_launch_modes_agree(ops_module):
+    import torch
+    m = M.jittered_cube(5, a=3.0)
+    ang = angular.snprep(3)
+    op = ops_module(m, ang)
+    op.set_physics([0.7], [[0.0] * 4])
+    q = torch.rand(op.n, dtype=torch.float64, device="cuda")
+    outs = {}
+    for mode in ("levels", "graph", "graph"):
+        op.set_sweep_mode(mode)
+        for _ in range(2):                     # replay the captured graph with eager work between
+            psi = torch.empty_like(q)
+            op.transport_matinv_dev(q, psi, 0, 1)
+            op.synchronize()
+            outs.setdefault(mode, []).append(psi.cpu().numpy())
+    ref = outs["levels"][0]
+    for mode, vals in outs.items():
+        for v in vals:
+            assert np.array_equal(v, ref), mode
