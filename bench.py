@@ -117,7 +117,7 @@ def single_group_leg(m, ang, reps=10):
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     res = {}
-    for mode in ("levels", "graph"):
+    for mode in ("levels", "graph", "persistent"):
         op.set_sweep_mode(mode)
         op.transport_matinv_dev(q, psi, 0, 1)
         op.synchronize()
@@ -147,6 +147,7 @@ def run(args):
     sigt, _ = problems.cfg2_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
+    op.set_sweep_mode(args.sweep_mode)
     setup_s = time.perf_counter() - t0
     cag = m.NE * ang.NI * G
     q = torch.rand(cag * 4, dtype=torch.float64, device="cuda")
@@ -170,6 +171,19 @@ def run(args):
     achieved = BYTES_PER_CAG * cag / (ev_ms * 1e-3) / 1e9
     if not bool(torch.isfinite(psi[:: 1 << 16]).all()):
         raise RuntimeError("non-finite flux")
+    other_modes = {}
+    if args.compare_modes:
+        for mode in ("levels", "graph", "persistent"):
+            if mode == args.sweep_mode:
+                continue
+            op.set_sweep_mode(mode)
+            op.transport_matinv_dev(q, psi, 0, G)
+            op.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(3):
+                op.transport_matinv_dev(q, psi, 0, G)
+            op.synchronize()
+            other_modes[mode] = {"ms_per_sweep": 1000.0 * (time.perf_counter() - t1) / 3}
     levels = op.maxlev
     op.close()
     del q, psi
@@ -190,6 +204,7 @@ def run(args):
         "config": {"workload": f"DFEM SN sweep, {m.NE} tets, {ang.NI} directions (GL8 x C32), {G} groups, "
                                f"P{ang.L}, level-synchronous, in HBM",
                    "tets": m.NE, "directions": ang.NI, "groups": G, "levels_max": levels,
+                   "sweep_mode": args.sweep_mode,
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -206,6 +221,8 @@ def run(args):
         out["roofline"]["traffic_unit"] = "bytes per launch (PMC, " + tr["source"] + ")"
         out["roofline"]["traffic_bytes_per_unit"] = per_cag
     out.update(out_extra)
+    if other_modes:
+        out["other_sweep_modes"] = other_modes
     if args.single:
         out["single_group_sweep"] = single_group_leg(m, ang)
     if args.solve:
@@ -227,6 +244,8 @@ def main():
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
+    p.add_argument("--sweep-mode", default="graph", choices=["levels", "graph", "persistent"])
+    p.add_argument("--compare-modes", action="store_true")
     args = p.parse_args()
     if int(os.environ.get("WORLD_SIZE", "1")) > 1 or args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no collective path is built)"}))

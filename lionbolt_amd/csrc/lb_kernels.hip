@@ -10,6 +10,8 @@
 // one cell read consecutive doubles.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lb_internal.hpp"
 
 namespace lb {
@@ -23,51 +25,55 @@ __device__ __forceinline__ bool in_face(int f, int a) {
   return a != 3 - f;
 }
 
-// One lane = one (cell, direction, group) local solve.
-__global__ __launch_bounds__(kBlock) void sweep_level_kernel(
-    const int4* __restrict__ work, const uint32_t* __restrict__ tasks, const double4* __restrict__ geo,
-    const int4* __restrict__ nbr, const uint32_t* __restrict__ perm, const int32_t* __restrict__ mat,
-    const double* __restrict__ kdir, const double* __restrict__ sigt, int Gtot, int g0, int G, int64_t NE,
-    const double* q, double* psi) {
-  const int4 wd = work[blockIdx.x];
-  const int i = wd.x;
-  const int t = threadIdx.x;
-  const int c = t / G;
-  const int g = t - c * G;
-  if (c >= wd.z) return;
-  const uint32_t w = tasks[(int64_t)i * NE + wd.y + c];
+// Arguments shared by the sweep kernels.
+struct SweepArgs {
+  const uint32_t* __restrict__ tasks;
+  const double4* __restrict__ geo;
+  const int4* __restrict__ nbr;
+  const uint32_t* __restrict__ perm;
+  const int32_t* __restrict__ mat;
+  const double* __restrict__ kdir;
+  const double* __restrict__ sigt;
+  int Gtot, g0, G;
+  int64_t NE;
+  const double* q;   // may alias psi (in-place sweep)
+  double* psi;
+};
+
+// One lane = one (cell, direction, group) local solve; w is the packed task word.
+__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, uint32_t w, int g) {
   const int e = int(w & 0xFFFFFFu);
   const unsigned up = (w >> 24) & 0xFu;
   const unsigned out = w >> 28;
 
-  const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
+  const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
   double kn[4], cf[4];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
-    const double4 gf = geo[(int64_t)e * 4 + f];
+    const double4 gf = A.geo[(int64_t)e * 4 + f];
     kn[f] = kx * gf.x + ky * gf.y + kz * gf.z;
     cf[f] = gf.w;
   }
-  const double sig = sigt[(int64_t)mat[e] * Gtot + g0 + g];
-  const int64_t base = ((int64_t)i * NE + e) * 4 * G + g;
+  const double sig = A.sigt[(int64_t)A.mat[e] * A.Gtot + A.g0 + g];
+  const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
   double b[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) b[a] = q[base + a * G];
+  for (int a = 0; a < 4; ++a) b[a] = A.q[base + a * A.G];
 
   // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
   // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
   // node, u = neighbour values at the shared nodes, U their sum.
   if (up) {
-    const int4 nb = nbr[e];
-    const uint32_t pm = perm[e];
+    const int4 nb = A.nbr[e];
+    const uint32_t pm = A.perm[e];
     const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       if (up >> f & 1u) {
-        const int64_t ub = ((int64_t)i * NE + nbv[f]) * 4 * G + g;
-        const double u0 = psi[ub + ((pm >> (6 * f)) & 3u) * G];
-        const double u1 = psi[ub + ((pm >> (6 * f + 2)) & 3u) * G];
-        const double u2 = psi[ub + ((pm >> (6 * f + 4)) & 3u) * G];
+        const int64_t ub = ((int64_t)i * A.NE + nbv[f]) * 4 * A.G + g;
+        const double u0 = A.psi[ub + ((pm >> (6 * f)) & 3u) * A.G];
+        const double u1 = A.psi[ub + ((pm >> (6 * f + 2)) & 3u) * A.G];
+        const double u2 = A.psi[ub + ((pm >> (6 * f + 4)) & 3u) * A.G];
         const double U = u0 + u1 + u2;
         const double coef = -kn[f] * cf[f];
         b[kFaceNodes[f][0]] += coef * (u0 + 0.2 * U);
@@ -150,7 +156,86 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(
     x[r] = s / T[r][r];
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a) psi[base + a * G] = x[a];
+  for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
+}
+
+__global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restrict__ work, SweepArgs A) {
+  const int4 wd = work[blockIdx.x];
+  const int c = threadIdx.x / A.G;
+  const int g = threadIdx.x - c * A.G;
+  if (c >= wd.z) return;
+  solve_cell(A, wd.x, A.tasks[(int64_t)wd.x * A.NE + wd.y + c], g);
+}
+
+// Persistent dataflow sweep.  Workgroups take work-list entries (level-major, so
+// every entry depends only on earlier entries) from an atomic counter; a lane waits
+// until the upstream cells of its cell carry this sweep's epoch in done[dir][cell],
+// solves, and after the workgroup's fluxes are written back at agent scope the
+// cell's flag is set.  A workgroup only waits on entries taken before its own, by
+// workgroups that are already running, so the sweep always drains.  Every wait is
+// bounded: after spin_limit polls, or once another lane has given up, the lane
+// records the failure in sync->err and proceeds, so the kernel always terminates.
+struct SweepSync {
+  uint32_t counter;
+  uint32_t err;
+  uint32_t epoch;
+  uint32_t pad;
+};
+
+__global__ void sweep_sync_reset_kernel(SweepSync* sync) {
+  sync->counter = 0;
+  sync->err = 0;
+  sync->epoch = sync->epoch + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void sweep_persistent_kernel(const int4* __restrict__ work, int nwork,
+                                                                  SweepArgs A, uint32_t* done, SweepSync* sync,
+                                                                  uint32_t spin_limit) {
+  __shared__ int s_idx;
+  const uint32_t epoch = sync->epoch;
+  const int c = threadIdx.x / A.G;
+  const int g = threadIdx.x - c * A.G;
+  while (true) {
+    if (threadIdx.x == 0) s_idx = (int)atomicAdd(&sync->counter, 1u);
+    __syncthreads();
+    const int idx = s_idx;
+    __syncthreads();
+    if (idx >= nwork) break;
+    const int4 wd = work[idx];
+    const int i = wd.x;
+    const bool active = c < wd.z;
+    uint32_t w = 0;
+    if (active) {
+      w = A.tasks[(int64_t)i * A.NE + wd.y + c];
+      const int e = int(w & 0xFFFFFFu);
+      const unsigned up = (w >> 24) & 0xFu;
+      if (up) {
+        const int4 nb = A.nbr[e];
+        const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          if (up >> f & 1u) {
+            const uint32_t* fl = done + (int64_t)i * A.NE + nbv[f];
+            uint32_t polls = 0;
+            while (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+              if (++polls > spin_limit || __hip_atomic_load(&sync->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(&sync->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+              }
+              __builtin_amdgcn_s_sleep(2);
+            }
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (active) solve_cell(A, i, w, g);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (active && g == 0)
+      __hip_atomic_store(done + (int64_t)i * A.NE + int(w & 0xFFFFFFu), epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // phi[lm][col] = sum_i D[lm][i] psi[i][col]; one lane per column (column =
@@ -318,14 +403,43 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   LevelTable* t = level_table(op, G, &rc);
   if (!t) return rc;
   const int4* work = t->d_entries.as<const int4>();
+  SweepArgs A{op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
+              op->d_perm.as<const uint32_t>(),  op->d_mat.as<const int32_t>(),  op->d_k.as<const double>(),
+              op->d_sigt.as<const double>(),    op->G,                          g0,
+              G,                                op->NE,                         q,
+              psi};
+
+  if (op->sweep_mode == LB_SWEEP_PERSISTENT) {
+    if (!op->d_done.p) {
+      LB_TRY(op->d_done.reserve((size_t)op->NI * op->NE * sizeof(uint32_t)));
+      LB_HIP(hipMemset(op->d_done.p, 0, (size_t)op->NI * op->NE * sizeof(uint32_t)));
+      LB_TRY(op->d_sync.reserve(sizeof(SweepSync)));
+      LB_HIP(hipMemset(op->d_sync.p, 0, sizeof(SweepSync)));
+      int dev = 0, ncu = 0, per_cu = 0;
+      LB_HIP(hipGetDevice(&dev));
+      LB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      LB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_persistent_kernel, kBlock, 0));
+      op->persistent_grid = std::max(1, ncu * std::max(1, per_cu));
+    }
+    const int nwork = t->level_off[op->maxlev];
+    const unsigned grid = (unsigned)std::min(nwork, op->persistent_grid);
+    SweepSync* sync = op->d_sync.as<SweepSync>();
+    LB_HIP(hipEventRecord(op->ev0, op->stream));
+    hipLaunchKernelGGL(sweep_sync_reset_kernel, dim3(1), dim3(1), 0, op->stream, sync);
+    hipLaunchKernelGGL(sweep_persistent_kernel, dim3(grid), dim3(kBlock), 0, op->stream, work, nwork, A,
+                       op->d_done.as<uint32_t>(), sync, op->spin_limit);
+    LB_HIP(hipGetLastError());
+    LB_HIP(hipEventRecord(op->ev1, op->stream));
+    // the wait-failure word travels back with the stream; checked at the next synchronisation
+    LB_HIP(hipMemcpyAsync(op->h_sweep_err, &sync->err, sizeof(uint32_t), hipMemcpyDeviceToHost, op->stream));
+    return LB_OK;
+  }
+
   auto enqueue = [&]() {
     for (int l = 0; l < op->maxlev; ++l) {
       const int nb = t->level_off[l + 1] - t->level_off[l];
       if (nb == 0) continue;
-      hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l],
-                         op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
-                         op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
-                         op->d_sigt.as<const double>(), op->G, g0, G, op->NE, q, psi);
+      hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
     }
   };
   if (op->sweep_mode == LB_SWEEP_GRAPH) {
@@ -366,6 +480,17 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   enqueue();
   LB_HIP(hipGetLastError());
   LB_HIP(hipEventRecord(op->ev1, op->stream));
+  return LB_OK;
+}
+
+// After the stream has been synchronised: report a failed persistent-sweep wait.
+int check_sweep_error(lb_op* op) {
+  if (op->h_sweep_err && *op->h_sweep_err) {
+    *op->h_sweep_err = 0;
+    set_error("persistent sweep: an upstream dependency was not satisfied within the wait limit; the flux of "
+              "that sweep is invalid");
+    return LB_E_HIP;
+  }
   return LB_OK;
 }
 
@@ -459,7 +584,7 @@ int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
   *out = *static_cast<double*>(op->w_host.p);
-  return LB_OK;
+  return check_sweep_error(op);
 }
 
 int launch_fluence(lb_op* op, const double* v, double* phi) {
