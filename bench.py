@@ -117,7 +117,7 @@ def single_group_leg(m, ang, reps=10):
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     res = {}
-    for mode in ("levels", "graph", "persistent"):
+    for mode in ("levels", "graph", "direction"):
         op.set_sweep_mode(mode)
         op.transport_matinv_dev(q, psi, 0, 1)
         op.synchronize()
@@ -173,7 +173,7 @@ def run(args):
         raise RuntimeError("non-finite flux")
     other_modes = {}
     if args.compare_modes:
-        for mode in ("levels", "graph", "persistent"):
+        for mode in ("levels", "graph", "direction"):
             if mode == args.sweep_mode:
                 continue
             op.set_sweep_mode(mode)
@@ -244,7 +244,7 @@ def main():
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
-    p.add_argument("--sweep-mode", default="graph", choices=["levels", "graph", "persistent"])
+    p.add_argument("--sweep-mode", default="graph", choices=["levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")
     args = p.parse_args()
     if int(os.environ.get("WORLD_SIZE", "1")) > 1 or args.gpus != 1:

@@ -45,7 +45,7 @@ enum {
 enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
 enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
 enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
-enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_PERSISTENT = 2 };
+enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2 };
 
 typedef struct lb_op lb_op;
 
@@ -109,16 +109,13 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
 /* How a sweep is issued: LB_SWEEP_LEVELS launches one kernel per global level
  * on the operator's stream; LB_SWEEP_GRAPH (default) captures those launches
  * once per (source, flux, group range) into a HIP graph and replays it;
- * LB_SWEEP_PERSISTENT runs one persistent kernel whose workgroups take work in
- * level order and wait on per-(direction, cell) completion flags (bounded
- * waits; a failed wait is reported at the next synchronising call).  No
- * reference counterpart (the reference sweeps with OpenMP threads). */
+ * LB_SWEEP_DIRECTION runs one kernel in which each workgroup owns one
+ * direction and walks its levels with workgroup barriers.  No reference
+ * counterpart (the reference sweeps one direction per OpenMP thread). */
 int lb_op_set_sweep_mode(lb_op* op, int mode);
 
-/* Polls a persistent-sweep lane makes on one upstream flag before it records a
- * failure and gives up (default 2^22).  A failed sweep is reported as LB_E_HIP
- * by the next synchronising call. */
-int lb_op_set_sweep_wait_limit(lb_op* op, uint32_t polls);
+/* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
+int lb_op_set_direction_block(lb_op* op, int threads);
 
 /* TransportOp%MatInv (TransportOperator.f90:72-151): s <- T^-1 s in place. */
 int lb_transport_matinv(lb_op* op, double* s, int64_t n);

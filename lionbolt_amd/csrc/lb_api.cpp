@@ -82,7 +82,7 @@ int to_device(lb_op* op, const double* h, int64_t n) {
 int to_host(lb_op* op, double* h, int64_t n) {
   LB_HIP(hipMemcpyAsync(h, op->w_a.p, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
-  return check_sweep_error(op);
+  return LB_OK;
 }
 
 int upload_physics(lb_op* op) {
@@ -207,26 +207,22 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 }
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
-  if (!op || (mode != LB_SWEEP_LEVELS && mode != LB_SWEEP_GRAPH && mode != LB_SWEEP_PERSISTENT)) {
+  if (!op || (mode != LB_SWEEP_LEVELS && mode != LB_SWEEP_GRAPH && mode != LB_SWEEP_DIRECTION)) {
     set_error("lb_op_set_sweep_mode: invalid mode");
     return LB_E_ARG;
   }
   if (op->stream) (void)hipStreamSynchronize(op->stream);
   clear_graphs(op);
-  if (mode == LB_SWEEP_PERSISTENT && !op->h_sweep_err) {
-    LB_HIP(hipHostMalloc(reinterpret_cast<void**>(&op->h_sweep_err), sizeof(uint32_t), hipHostMallocDefault));
-    *op->h_sweep_err = 0;
-  }
   op->sweep_mode = mode;
   return LB_OK;
 }
 
-int lb_op_set_sweep_wait_limit(lb_op* op, uint32_t polls) {
-  if (!op) {
-    set_error("lb_op_set_sweep_wait_limit: null operator");
+int lb_op_set_direction_block(lb_op* op, int threads) {
+  if (!op || (threads != 256 && threads != 512 && threads != 1024)) {
+    set_error("lb_op_set_direction_block: threads must be 256, 512 or 1024");
     return LB_E_ARG;
   }
-  op->spin_limit = polls;
+  op->dir_block = threads;
   return LB_OK;
 }
 
@@ -357,7 +353,7 @@ int lb_op_synchronize(lb_op* op) {
     return LB_E_ARG;
   }
   LB_HIP(hipStreamSynchronize(op->stream));
-  return check_sweep_error(op);
+  return LB_OK;
 }
 
 int lb_last_sweep_ms(lb_op* op, double* ms) {
@@ -382,9 +378,8 @@ void lb_op_destroy(lb_op* op) {
     b->release();
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);
-  if (op->h_sweep_err) (void)hipHostFree(op->h_sweep_err);
-  op->d_done.release();
-  op->d_sync.release();
+  op->d_lvl_off.release();
+  op->d_nlev.release();
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);
   if (op->stream) (void)hipStreamDestroy(op->stream);

@@ -98,11 +98,8 @@ struct lb_op {
   lb::DevBuf d_sigt, d_sigs;
   int scatter_mode = LB_SCATTER_MOMENT;
   int sweep_mode = LB_SWEEP_GRAPH;
-  lb::DevBuf d_done;                // uint32 [NI][NE] epoch of the last sweep that solved (dir, cell)
-  lb::DevBuf d_sync;                // persistent-sweep counter, error word, epoch
-  uint32_t* h_sweep_err = nullptr;  // pinned copy of the error word
-  int persistent_grid = 0;
-  uint32_t spin_limit = 1u << 22;
+  lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
+  int dir_block = 1024;             // workgroup size of the direction-resident sweep
   std::vector<lb::SweepGraph> graphs;
   // runtime
   hipStream_t stream = nullptr;
@@ -134,7 +131,6 @@ int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, 
 int launch_scale(hipStream_t s, int64_t n, double a, double* x);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
-int check_sweep_error(lb_op* op);
 int launch_fluence(lb_op* op, const double* v, double* phi);
 
 // lb_krylov.cpp

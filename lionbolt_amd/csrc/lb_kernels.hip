@@ -167,74 +167,28 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
   solve_cell(A, wd.x, A.tasks[(int64_t)wd.x * A.NE + wd.y + c], g);
 }
 
-// Persistent dataflow sweep.  Workgroups take work-list entries (level-major, so
-// every entry depends only on earlier entries) from an atomic counter; a lane waits
-// until the upstream cells of its cell carry this sweep's epoch in done[dir][cell],
-// solves, and after the workgroup's fluxes are written back at agent scope the
-// cell's flag is set.  A workgroup only waits on entries taken before its own, by
-// workgroups that are already running, so the sweep always drains.  Every wait is
-// bounded: after spin_limit polls, or once another lane has given up, the lane
-// records the failure in sync->err and proceeds, so the kernel always terminates.
-struct SweepSync {
-  uint32_t counter;
-  uint32_t err;
-  uint32_t epoch;
-  uint32_t pad;
-};
-
-__global__ void sweep_sync_reset_kernel(SweepSync* sync) {
-  sync->counter = 0;
-  sync->err = 0;
-  sync->epoch = sync->epoch + 1;
-}
-
-__global__ __launch_bounds__(kBlock) void sweep_persistent_kernel(const int4* __restrict__ work, int nwork,
-                                                                  SweepArgs A, uint32_t* done, SweepSync* sync,
-                                                                  uint32_t spin_limit) {
-  __shared__ int s_idx;
-  const uint32_t epoch = sync->epoch;
+// Direction-resident sweep: one workgroup owns one direction (of one batch of
+// groups) and walks its levels in order, with a workgroup barrier between
+// levels.  All producers and consumers of a direction's upstream traces run on
+// one CU, so workgroup-scope ordering is enough: no inter-workgroup flags, no
+// agent-scope cache write-back or invalidation, and the traces of the last few
+// levels are re-read from the XCD's L2.  One launch sweeps every direction.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* __restrict__ lvl_off,
+                                                                const int32_t* __restrict__ nlev, int maxlev,
+                                                                SweepArgs A) {
+  const int i = blockIdx.x;
+  const int32_t* off = lvl_off + (int64_t)i * (maxlev + 1);
+  const int nl = nlev[i];
+  const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
-  while (true) {
-    if (threadIdx.x == 0) s_idx = (int)atomicAdd(&sync->counter, 1u);
+  const uint32_t* tk = A.tasks + (int64_t)i * A.NE;
+  for (int l = 0; l < nl; ++l) {
+    const int s1 = off[l + 1];
+    if (c < cpr)
+      for (int s = off[l] + c; s < s1; s += cpr) solve_cell(A, i, tk[s], g);
     __syncthreads();
-    const int idx = s_idx;
-    __syncthreads();
-    if (idx >= nwork) break;
-    const int4 wd = work[idx];
-    const int i = wd.x;
-    const bool active = c < wd.z;
-    uint32_t w = 0;
-    if (active) {
-      w = A.tasks[(int64_t)i * A.NE + wd.y + c];
-      const int e = int(w & 0xFFFFFFu);
-      const unsigned up = (w >> 24) & 0xFu;
-      if (up) {
-        const int4 nb = A.nbr[e];
-        const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          if (up >> f & 1u) {
-            const uint32_t* fl = done + (int64_t)i * A.NE + nbv[f];
-            uint32_t polls = 0;
-            while (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-              if (++polls > spin_limit || __hip_atomic_load(&sync->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                __hip_atomic_store(&sync->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-              }
-              __builtin_amdgcn_s_sleep(2);
-            }
-          }
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (active) solve_cell(A, i, w, g);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (active && g == 0)
-      __hip_atomic_store(done + (int64_t)i * A.NE + int(w & 0xFFFFFFu), epoch, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -399,6 +353,10 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
     set_error("sweep: the group batch must be between 1 and 256");
     return LB_E_ARG;
   }
+  if (op->sweep_mode == LB_SWEEP_DIRECTION && (op->dir_block / G) < 1) {
+    set_error("sweep: the direction-resident block is smaller than the group batch");
+    return LB_E_ARG;
+  }
   int rc = LB_OK;
   LevelTable* t = level_table(op, G, &rc);
   if (!t) return rc;
@@ -409,29 +367,25 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
               G,                                op->NE,                         q,
               psi};
 
-  if (op->sweep_mode == LB_SWEEP_PERSISTENT) {
-    if (!op->d_done.p) {
-      LB_TRY(op->d_done.reserve((size_t)op->NI * op->NE * sizeof(uint32_t)));
-      LB_HIP(hipMemset(op->d_done.p, 0, (size_t)op->NI * op->NE * sizeof(uint32_t)));
-      LB_TRY(op->d_sync.reserve(sizeof(SweepSync)));
-      LB_HIP(hipMemset(op->d_sync.p, 0, sizeof(SweepSync)));
-      int dev = 0, ncu = 0, per_cu = 0;
-      LB_HIP(hipGetDevice(&dev));
-      LB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-      LB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_persistent_kernel, kBlock, 0));
-      op->persistent_grid = std::max(1, ncu * std::max(1, per_cu));
+  if (op->sweep_mode == LB_SWEEP_DIRECTION) {
+    if (!op->d_lvl_off.p) {
+      LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
+      LB_HIP(hipMemcpy(op->d_lvl_off.p, op->h_lvl_off.data(), op->h_lvl_off.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+      LB_TRY(op->d_nlev.reserve(op->h_nlev.size() * sizeof(int32_t)));
+      LB_HIP(hipMemcpy(op->d_nlev.p, op->h_nlev.data(), op->h_nlev.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
-    const int nwork = t->level_off[op->maxlev];
-    const unsigned grid = (unsigned)std::min(nwork, op->persistent_grid);
-    SweepSync* sync = op->d_sync.as<SweepSync>();
     LB_HIP(hipEventRecord(op->ev0, op->stream));
-    hipLaunchKernelGGL(sweep_sync_reset_kernel, dim3(1), dim3(1), 0, op->stream, sync);
-    hipLaunchKernelGGL(sweep_persistent_kernel, dim3(grid), dim3(kBlock), 0, op->stream, work, nwork, A,
-                       op->d_done.as<uint32_t>(), sync, op->spin_limit);
+    const int32_t* lo = op->d_lvl_off.as<const int32_t>();
+    const int32_t* nl = op->d_nlev.as<const int32_t>();
+    if (op->dir_block == 256)
+      hipLaunchKernelGGL(sweep_direction_kernel<256>, dim3(op->NI), dim3(256), 0, op->stream, lo, nl, op->maxlev, A);
+    else if (op->dir_block == 512)
+      hipLaunchKernelGGL(sweep_direction_kernel<512>, dim3(op->NI), dim3(512), 0, op->stream, lo, nl, op->maxlev, A);
+    else
+      hipLaunchKernelGGL(sweep_direction_kernel<1024>, dim3(op->NI), dim3(1024), 0, op->stream, lo, nl, op->maxlev, A);
     LB_HIP(hipGetLastError());
     LB_HIP(hipEventRecord(op->ev1, op->stream));
-    // the wait-failure word travels back with the stream; checked at the next synchronisation
-    LB_HIP(hipMemcpyAsync(op->h_sweep_err, &sync->err, sizeof(uint32_t), hipMemcpyDeviceToHost, op->stream));
     return LB_OK;
   }
 
@@ -480,17 +434,6 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   enqueue();
   LB_HIP(hipGetLastError());
   LB_HIP(hipEventRecord(op->ev1, op->stream));
-  return LB_OK;
-}
-
-// After the stream has been synchronised: report a failed persistent-sweep wait.
-int check_sweep_error(lb_op* op) {
-  if (op->h_sweep_err && *op->h_sweep_err) {
-    *op->h_sweep_err = 0;
-    set_error("persistent sweep: an upstream dependency was not satisfied within the wait limit; the flux of "
-              "that sweep is invalid");
-    return LB_E_HIP;
-  }
   return LB_OK;
 }
 
@@ -584,7 +527,7 @@ int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
   *out = *static_cast<double*>(op->w_host.p);
-  return check_sweep_error(op);
+  return LB_OK;
 }
 
 int launch_fluence(lb_op* op, const double* v, double* phi) {

@@ -88,11 +88,11 @@ class Operator:
         _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
 
     def set_sweep_mode(self, mode):
-        m = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "persistent": _lib.SWEEP_PERSISTENT}[mode]
+        m = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION}[mode]
         _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, m))
 
-    def set_sweep_wait_limit(self, polls):
-        _lib.check(_lib.lib().lb_op_set_sweep_wait_limit(self.h, int(polls)))
+    def set_direction_block(self, threads):
+        _lib.check(_lib.lib().lb_op_set_direction_block(self.h, int(threads)))
 
     # -- host entry points (reference layout v[sdof + i*NENK]) -------------------
     def _vec(self, x):
