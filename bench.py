@@ -117,8 +117,12 @@ def single_group_leg(m, ang, reps=10):
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     res = {}
-    for mode in ("levels", "graph", "direction"):
-        op.set_sweep_mode(mode)
+    for mode in ("levels", "graph", "direction", "direction-256", "direction-512"):
+        if mode.startswith("direction"):
+            op.set_direction_block(int(mode.split("-")[1]) if "-" in mode else 1024)
+            op.set_sweep_mode("direction")
+        else:
+            op.set_sweep_mode(mode)
         op.transport_matinv_dev(q, psi, 0, 1)
         op.synchronize()
         t0 = time.perf_counter()

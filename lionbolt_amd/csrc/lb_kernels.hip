@@ -40,32 +40,60 @@ struct SweepArgs {
   double* psi;
 };
 
-// One lane = one (cell, direction, group) local solve; w is the packed task word.
-__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, uint32_t w, int g) {
-  const int e = int(w & 0xFFFFFFu);
-  const unsigned up = (w >> 24) & 0xFu;
-  const unsigned out = w >> 28;
-
-  const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
+// The part of a cell's solve that does not depend on upstream results: task
+// word, geometry, cross section, source and neighbour data.  Kernels may fetch
+// it one level ahead.
+struct CellPrep {
+  uint32_t w;
   double kn[4], cf[4];
+  double sig;
+  double b[4];
+  int4 nb;
+  uint32_t pm;
+};
+
+__device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, uint32_t w, int g, CellPrep& P) {
+  const int e = int(w & 0xFFFFFFu);
+  P.w = w;
+  const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     const double4 gf = A.geo[(int64_t)e * 4 + f];
-    kn[f] = kx * gf.x + ky * gf.y + kz * gf.z;
-    cf[f] = gf.w;
+    P.kn[f] = kx * gf.x + ky * gf.y + kz * gf.z;
+    P.cf[f] = gf.w;
   }
-  const double sig = A.sigt[(int64_t)A.mat[e] * A.Gtot + A.g0 + g];
+  P.sig = A.sigt[(int64_t)A.mat[e] * A.Gtot + A.g0 + g];
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
-  double b[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) b[a] = A.q[base + a * A.G];
+  for (int a = 0; a < 4; ++a) P.b[a] = A.q[base + a * A.G];
+  if ((w >> 24) & 0xFu) {
+    P.nb = A.nbr[e];
+    P.pm = A.perm[e];
+  }
+}
+
+// The dependent part: upstream traces, local matrix, LU solve, store.
+__device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const CellPrep& P, int g) {
+  const uint32_t w = P.w;
+  const int e = int(w & 0xFFFFFFu);
+  const unsigned up = (w >> 24) & 0xFu;
+  const unsigned out = w >> 28;
+  double kn[4], cf[4], b[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    kn[f] = P.kn[f];
+    cf[f] = P.cf[f];
+    b[f] = P.b[f];
+  }
+  const double sig = P.sig;
+  const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 
   // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
   // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
   // node, u = neighbour values at the shared nodes, U their sum.
   if (up) {
-    const int4 nb = A.nbr[e];
-    const uint32_t pm = A.perm[e];
+    const int4 nb = P.nb;
+    const uint32_t pm = P.pm;
     const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -159,6 +187,12 @@ __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, uint32_t w
   for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
 }
 
+__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, uint32_t w, int g) {
+  CellPrep P;
+  prep_cell(A, i, w, g, P);
+  finish_cell(A, i, P, g);
+}
+
 __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restrict__ work, SweepArgs A) {
   const int4 wd = work[blockIdx.x];
   const int c = threadIdx.x / A.G;
@@ -183,11 +217,26 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
+  const bool lane_ok = c < cpr;
   const uint32_t* tk = A.tasks + (int64_t)i * A.NE;
+  // The first cell of each level for this lane is prepared one level ahead, so
+  // only the upstream-trace loads sit between two barriers.
+  CellPrep P;
+  bool have = false;
+  if (nl > 0 && lane_ok && off[0] + c < off[1]) {
+    prep_cell(A, i, tk[off[0] + c], g, P);
+    have = true;
+  }
   for (int l = 0; l < nl; ++l) {
-    const int s1 = off[l + 1];
-    if (c < cpr)
-      for (int s = off[l] + c; s < s1; s += cpr) solve_cell(A, i, tk[s], g);
+    const int s0 = off[l], s1 = off[l + 1];
+    if (have) finish_cell(A, i, P, g);
+    if (lane_ok)
+      for (int s = s0 + c + cpr; s < s1; s += cpr) solve_cell(A, i, tk[s], g);
+    have = false;
+    if (l + 1 < nl && lane_ok && s1 + c < off[l + 2]) {
+      prep_cell(A, i, tk[s1 + c], g, P);
+      have = true;
+    }
     __syncthreads();
   }
 }
