@@ -231,6 +231,9 @@ def run(args):
         out["single_group_sweep"] = single_group_leg(m, ang)
     if args.solve:
         out["solve"] = solve_leg(m, ang, args.solve_rtol)
+    if args.gemm_study:
+        from lionbolt_amd import gemm_study
+        out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, ang)
     print(json.dumps(out))
@@ -250,6 +253,7 @@ def main():
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="graph", choices=["levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")
+    p.add_argument("--gemm-study", action="store_true", help="scattering-moment products on MFMA, all formats")
     args = p.parse_args()
     if int(os.environ.get("WORLD_SIZE", "1")) > 1 or args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no collective path is built)"}))

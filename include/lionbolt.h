@@ -168,6 +168,27 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
 int lb_multigroup_solve_dev(lb_op* op, const double* d_qmom, const double* transfer, const lb_solver_opts* opts,
                             double* d_phi, lb_solve_stats* stats);
 
+/* ---- scattering-moment products on MFMA, reduced input precision, FP32
+ * accumulation (no reference counterpart: the reference applies the dense
+ * FP64 kernel YYS with dgemm, ScatteringAlgebra.f90:30-53).  Formats:
+ * LB_FMT_FP32, LB_FMT_BF16, LB_FMT_SPLIT_BF16 (x = hi + lo, three products),
+ * LB_FMT_FP16 (power-of-two scale per column).  Reduced operands are stored with
+ * the reduced dimension contiguous per column.  stream: a hipStream_t or NULL. */
+enum { LB_FMT_FP32 = 0, LB_FMT_BF16 = 1, LB_FMT_SPLIT_BF16 = 2, LB_FMT_FP16 = 3 };
+
+/* src [rows][ncols] FP64 -> dst [ncols][rows] in the format (dst_lo: low part of
+ * the split format; inv_scale[ncols]: FP16 column scales 2^-e, written here). */
+int lb_pack_dev(int fmt, int rows, int64_t ncols, const double* src, void* dst, void* dst_lo, float* inv_scale,
+                void* stream);
+
+/* phi[m][col] (FP32) = sum_i D[m][i] psi[col][i]; D device FP64 [nmom][NI], NI = 256, nmom <= 16. */
+int lb_gemm_d_dev(int fmt, int nmom, int NI, const double* D, int64_t ncols, const void* psi, const void* psi_lo,
+                  const float* inv_scale, float* phi, void* stream);
+
+/* psi[col][i] (FP32) = sum_m M[i][m] phi[col][m]; M device FP64 [NI][16], NI = 256. */
+int lb_gemm_m_dev(int fmt, int NI, int nmom, const double* M, int64_t ncols, const void* phi, const void* phi_lo,
+                  const float* inv_scale, float* psi, void* stream);
+
 /* Block until the operator's stream is idle. */
 int lb_op_synchronize(lb_op* op);
 

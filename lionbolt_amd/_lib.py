@@ -69,6 +69,11 @@ SIGNATURES = {
     "lb_moments_to_discrete_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lb_multigroup_solve_dev": (ctypes.c_int, [_vp, _vp, _d, ctypes.POINTER(SolverOpts), _vp,
                                                ctypes.POINTER(SolveStats)]),
+    "lb_pack_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    "lb_gemm_d_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp, _vp,
+                                     _vp, _vp]),
+    "lb_gemm_m_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp, _vp,
+                                     _vp, _vp]),
     "lb_op_synchronize": (ctypes.c_int, [_vp]),
     "lb_last_sweep_ms": (ctypes.c_int, [_vp, _d]),
     "lb_op_destroy": (None, [_vp]),

@@ -378,6 +378,8 @@ void lb_op_destroy(lb_op* op) {
     b->release();
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);
+  op->d_rec.release();
+  op->d_ck.release();
   op->d_lvl_off.release();
   op->d_nlev.release();
   if (op->ev0) (void)hipEventDestroy(op->ev0);

@@ -98,6 +98,8 @@ struct lb_op {
   lb::DevBuf d_sigt, d_sigs;
   int scatter_mode = LB_SCATTER_MOMENT;
   int sweep_mode = LB_SWEEP_GRAPH;
+  lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
+  lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
   int dir_block = 1024;             // workgroup size of the direction-resident sweep
   std::vector<lb::SweepGraph> graphs;

@@ -25,14 +25,22 @@ __device__ __forceinline__ bool in_face(int f, int a) {
   return a != 3 - f;
 }
 
-// Arguments shared by the sweep kernels.
+// Per-(direction, sweep position) record, in sweep order so that the lanes of one
+// level read consecutive records.
+struct TaskRec {
+  uint32_t w;      // cell | upstream mask << 24 | outflow mask << 28
+  int32_t mat;
+  uint32_t perm;   // neighbour local node of each face node, 2 bits each
+  uint32_t pad;
+  int4 nbr;        // neighbour across each local face, -1 on the boundary
+};
+
+// Arguments shared by the sweep kernels.  ck[dir][pos] holds c_f (k . n_f) for the
+// four faces: every term of the local matrix and of the upstream coupling of a
+// linear tet is a multiple of it (closed forms of FEAInnerProducts.f90:311-407).
 struct SweepArgs {
-  const uint32_t* __restrict__ tasks;
-  const double4* __restrict__ geo;
-  const int4* __restrict__ nbr;
-  const uint32_t* __restrict__ perm;
-  const int32_t* __restrict__ mat;
-  const double* __restrict__ kdir;
+  const TaskRec* __restrict__ rec;
+  const double4* __restrict__ ck;
   const double* __restrict__ sigt;
   int Gtot, g0, G;
   int64_t NE;
@@ -40,36 +48,33 @@ struct SweepArgs {
   double* psi;
 };
 
-// The part of a cell's solve that does not depend on upstream results: task
-// word, geometry, cross section, source and neighbour data.  Kernels may fetch
-// it one level ahead.
+// The part of a cell's solve that does not depend on upstream results.  Kernels
+// may fetch it one level ahead.
 struct CellPrep {
   uint32_t w;
-  double kn[4], cf[4];
+  double ck[4];
   double sig;
   double b[4];
   int4 nb;
   uint32_t pm;
 };
 
-__device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, uint32_t w, int g, CellPrep& P) {
-  const int e = int(w & 0xFFFFFFu);
-  P.w = w;
-  const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const double4 gf = A.geo[(int64_t)e * 4 + f];
-    P.kn[f] = kx * gf.x + ky * gf.y + kz * gf.z;
-    P.cf[f] = gf.w;
-  }
-  P.sig = A.sigt[(int64_t)A.mat[e] * A.Gtot + A.g0 + g];
+__device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
+  const int64_t k = (int64_t)i * A.NE + s;
+  const TaskRec r = A.rec[k];
+  const double4 c = A.ck[k];
+  P.w = r.w;
+  P.ck[0] = c.x;
+  P.ck[1] = c.y;
+  P.ck[2] = c.z;
+  P.ck[3] = c.w;
+  P.nb = r.nbr;
+  P.pm = r.perm;
+  P.sig = A.sigt[(int64_t)r.mat * A.Gtot + A.g0 + g];
+  const int e = int(r.w & 0xFFFFFFu);
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 #pragma unroll
   for (int a = 0; a < 4; ++a) P.b[a] = A.q[base + a * A.G];
-  if ((w >> 24) & 0xFu) {
-    P.nb = A.nbr[e];
-    P.pm = A.perm[e];
-  }
 }
 
 // The dependent part: upstream traces, local matrix, LU solve, store.
@@ -78,23 +83,17 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   const int e = int(w & 0xFFFFFFu);
   const unsigned up = (w >> 24) & 0xFu;
   const unsigned out = w >> 28;
-  double kn[4], cf[4], b[4];
+  double b[4];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    kn[f] = P.kn[f];
-    cf[f] = P.cf[f];
-    b[f] = P.b[f];
-  }
-  const double sig = P.sig;
+  for (int a = 0; a < 4; ++a) b[a] = P.b[a];
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 
   // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
   // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
   // node, u = neighbour values at the shared nodes, U their sum.
   if (up) {
-    const int4 nb = P.nb;
+    const int nbv[4] = {P.nb.x, P.nb.y, P.nb.z, P.nb.w};
     const uint32_t pm = P.pm;
-    const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       if (up >> f & 1u) {
@@ -103,7 +102,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
         const double u1 = A.psi[ub + ((pm >> (6 * f + 2)) & 3u) * A.G];
         const double u2 = A.psi[ub + ((pm >> (6 * f + 4)) & 3u) * A.G];
         const double U = u0 + u1 + u2;
-        const double coef = -kn[f] * cf[f];
+        const double coef = -P.ck[f];
         b[kFaceNodes[f][0]] += coef * (u0 + 0.2 * U);
         b[kFaceNodes[f][1]] += coef * (u1 + 0.2 * U);
         b[kFaceNodes[f][2]] += coef * (u2 + 0.2 * U);
@@ -117,15 +116,15 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   double T[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const double r = cf[3 - a] * kn[3 - a];
+    const double r = P.ck[3 - a];
 #pragma unroll
     for (int bb = 0; bb < 4; ++bb) T[a][bb] = r;
-    T[a][a] += sig;
+    T[a][a] += P.sig;
   }
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     if (out >> f & 1u) {
-      const double v = cf[f] * kn[f];
+      const double v = P.ck[f];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int bb = kFaceNodes[f][j];
@@ -138,8 +137,11 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
     }
   }
 
-  // LocalLU (TransportAlgebra.f90:173-246): partial pivoting, row swap when a
-  // strictly larger |A(i,k)| is found, then back substitution.
+  // LocalLU (TransportAlgebra.f90:173-246): partial pivoting with a row swap when a
+  // strictly larger |A(i,k)| is found, elimination, back substitution.  Columns left
+  // of the pivot are not needed afterwards and are not swapped; each pivot is
+  // inverted once and reused by the elimination and the back substitution.
+  double inv[4];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     double mx = fabs(T[k][k]);
@@ -156,7 +158,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
     for (int r = k + 1; r < 4; ++r) {
       if (p == r) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = k; j < 4; ++j) {
           const double tmp = T[k][j];
           T[k][j] = T[r][j];
           T[r][j] = tmp;
@@ -166,31 +168,63 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
         b[r] = tb;
       }
     }
+    inv[k] = 1.0 / T[k][k];
 #pragma unroll
     for (int r = k + 1; r < 4; ++r) {
-      const double factor = T[r][k] / T[k][k];
+      const double factor = T[r][k] * inv[k];
 #pragma unroll
       for (int j = k + 1; j < 4; ++j) T[r][j] = T[r][j] - factor * T[k][j];
       b[r] = b[r] - factor * b[k];
     }
   }
+  inv[3] = 1.0 / T[3][3];
   double x[4];
-  x[3] = b[3] / T[3][3];
+  x[3] = b[3] * inv[3];
 #pragma unroll
   for (int r = 2; r >= 0; --r) {
-    double s = b[r];
+    double sacc = b[r];
 #pragma unroll
-    for (int j = r + 1; j < 4; ++j) s = s - T[r][j] * x[j];
-    x[r] = s / T[r][r];
+    for (int j = r + 1; j < 4; ++j) sacc = sacc - T[r][j] * x[j];
+    x[r] = sacc * inv[r];
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
 }
 
-__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, uint32_t w, int g) {
+__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g) {
   CellPrep P;
-  prep_cell(A, i, w, g, P);
+  prep_cell(A, i, s, g, P);
   finish_cell(A, i, P, g);
+}
+
+// Records and c_f (k . n_f) for every (direction, sweep position).
+__global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t NE, const uint32_t* __restrict__ tasks,
+                                                               const double4* __restrict__ geo,
+                                                               const int4* __restrict__ nbr,
+                                                               const uint32_t* __restrict__ perm,
+                                                               const int32_t* __restrict__ mat,
+                                                               const double* __restrict__ kdir,
+                                                               TaskRec* __restrict__ rec, double4* __restrict__ ck) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (int64_t)NI * NE) return;
+  const int i = int(k / NE);
+  const uint32_t w = tasks[k];
+  const int e = int(w & 0xFFFFFFu);
+  TaskRec r;
+  r.w = w;
+  r.mat = mat[e];
+  r.perm = perm[e];
+  r.pad = 0;
+  r.nbr = nbr[e];
+  rec[k] = r;
+  const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
+  double v[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const double4 gf = geo[(int64_t)e * 4 + f];
+    v[f] = gf.w * (kx * gf.x + ky * gf.y + kz * gf.z);
+  }
+  ck[k] = make_double4(v[0], v[1], v[2], v[3]);
 }
 
 __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restrict__ work, SweepArgs A) {
@@ -198,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   if (c >= wd.z) return;
-  solve_cell(A, wd.x, A.tasks[(int64_t)wd.x * A.NE + wd.y + c], g);
+  solve_cell(A, wd.x, wd.y + c, g);
 }
 
 // Direction-resident sweep: one workgroup owns one direction (of one batch of
@@ -218,23 +252,22 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
-  const uint32_t* tk = A.tasks + (int64_t)i * A.NE;
   // The first cell of each level for this lane is prepared one level ahead, so
   // only the upstream-trace loads sit between two barriers.
   CellPrep P;
   bool have = false;
   if (nl > 0 && lane_ok && off[0] + c < off[1]) {
-    prep_cell(A, i, tk[off[0] + c], g, P);
+    prep_cell(A, i, off[0] + c, g, P);
     have = true;
   }
   for (int l = 0; l < nl; ++l) {
     const int s0 = off[l], s1 = off[l + 1];
     if (have) finish_cell(A, i, P, g);
     if (lane_ok)
-      for (int s = s0 + c + cpr; s < s1; s += cpr) solve_cell(A, i, tk[s], g);
+      for (int s = s0 + c + cpr; s < s1; s += cpr) solve_cell(A, i, s, g);
     have = false;
     if (l + 1 < nl && lane_ok && s1 + c < off[l + 2]) {
-      prep_cell(A, i, tk[s1 + c], g, P);
+      prep_cell(A, i, s1 + c, g, P);
       have = true;
     }
     __syncthreads();
@@ -410,11 +443,18 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   LevelTable* t = level_table(op, G, &rc);
   if (!t) return rc;
   const int4* work = t->d_entries.as<const int4>();
-  SweepArgs A{op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
-              op->d_perm.as<const uint32_t>(),  op->d_mat.as<const int32_t>(),  op->d_k.as<const double>(),
-              op->d_sigt.as<const double>(),    op->G,                          g0,
-              G,                                op->NE,                         q,
-              psi};
+  if (!op->d_rec.p) {
+    const int64_t nrec = op->NI * op->NE;
+    LB_TRY(op->d_rec.reserve(nrec * sizeof(TaskRec)));
+    LB_TRY(op->d_ck.reserve(nrec * sizeof(double4)));
+    hipLaunchKernelGGL(build_records_kernel, dim3(grid_for(nrec)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                       op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
+                       op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
+                       op->d_rec.as<TaskRec>(), op->d_ck.as<double4>());
+    LB_HIP(hipGetLastError());
+  }
+  SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
+              op->G, g0, G, op->NE, q, psi};
 
   if (op->sweep_mode == LB_SWEEP_DIRECTION) {
     if (!op->d_lvl_off.p) {
