@@ -117,8 +117,14 @@ def single_group_leg(m, ang, reps=10):
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     res = {}
-    for mode in ("levels", "graph", "direction", "direction-256", "direction-512"):
-        if mode.startswith("direction"):
+    for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "auto"):
+        if mode == "auto":
+            op.set_sweep_mode("auto")
+            for _ in range(16):
+                if op.sweep_choice(1)[0] != "auto":
+                    break
+                op.transport_matinv_dev(q, psi, 0, 1)
+        elif mode.startswith("direction"):
             op.set_direction_block(int(mode.split("-")[1]) if "-" in mode else 1024)
             op.set_sweep_mode("direction")
         else:
@@ -131,6 +137,8 @@ def single_group_leg(m, ang, reps=10):
         op.synchronize()
         ms = 1000.0 * (time.perf_counter() - t0) / reps
         res[mode] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": m.NE * ang.NI / (ms * 1e-3)}
+        if mode == "auto":
+            res[mode]["choice"] = list(op.sweep_choice(1))
     op.close()
     return res
 
@@ -157,6 +165,17 @@ def run(args):
     q = torch.rand(cag * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     torch.cuda.synchronize()
+    mode, block = args.sweep_mode, 0
+    if args.sweep_mode == "auto":
+        # the autotuner times each launch configuration on its first sweeps (untimed here)
+        for _ in range(16):
+            if op.sweep_choice(G)[0] != "auto":
+                break
+            op.transport_matinv_dev(q, psi, 0, G)
+        op.synchronize()
+        mode, block = op.sweep_choice(G)
+    elif args.sweep_mode == "direction":
+        block = 1024
     for _ in range(args.warmup):
         op.transport_matinv_dev(q, psi, 0, G)
     op.synchronize()
@@ -178,7 +197,7 @@ def run(args):
     other_modes = {}
     if args.compare_modes:
         for mode in ("levels", "graph", "direction"):
-            if mode == args.sweep_mode:
+            if mode == args.sweep_mode or mode == "auto":
                 continue
             op.set_sweep_mode(mode)
             op.transport_matinv_dev(q, psi, 0, G)
@@ -189,6 +208,8 @@ def run(args):
             op.synchronize()
             other_modes[mode] = {"ms_per_sweep": 1000.0 * (time.perf_counter() - t1) / 3}
     levels = op.maxlev
+    launches = 1 if mode == "direction" else levels
+    kernel = f"sweep_direction_kernel<{block}>" if mode == "direction" else "sweep_level_kernel"
     op.close()
     del q, psi
     torch.cuda.empty_cache()
@@ -206,22 +227,24 @@ def run(args):
         "dtype": "f64",
         "data": "synthetic (seeded jittered-cube mesh and cross sections)",
         "config": {"workload": f"DFEM SN sweep, {m.NE} tets, {ang.NI} directions (GL8 x C32), {G} groups, "
-                               f"P{ang.L}, level-synchronous, in HBM",
+                               f"P{ang.L}, in HBM",
                    "tets": m.NE, "directions": ang.NI, "groups": G, "levels_max": levels,
-                   "sweep_mode": args.sweep_mode,
+                   "sweep_mode": args.sweep_mode, "launch": {"mode": mode, "block": block},
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sweep_level_kernel", "bytes_per_unit": BYTES_PER_CAG,
+                     "kernel": kernel, "launches_per_sweep": launches, "bytes_per_unit": BYTES_PER_CAG,
                      "event_ms_per_sweep": ev_ms},
     }
     traffic_file = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    tr = None
     if os.path.exists(traffic_file) and args.n == 40 and G == 24:
-        # HBM bytes per launch from the two PMC passes (FETCH_SIZE, WRITE_SIZE) recorded there
         with open(traffic_file) as fh:
-            tr = json.load(fh)
+            tr = json.load(fh).get(kernel)
+    if tr:
+        # HBM bytes per launch from the two PMC passes (FETCH_SIZE, WRITE_SIZE) recorded there
         per_cag = tr["fetch_bytes_per_cag"] + tr["write_bytes_per_cag"]
-        out["roofline"]["traffic"] = per_cag * cag / levels
+        out["roofline"]["traffic"] = per_cag * cag / launches
         out["roofline"]["traffic_unit"] = "bytes per launch (PMC, " + tr["source"] + ")"
         out["roofline"]["traffic_bytes_per_unit"] = per_cag
     out.update(out_extra)
@@ -251,7 +274,7 @@ def main():
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
-    p.add_argument("--sweep-mode", default="graph", choices=["levels", "graph", "direction"])
+    p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")
     p.add_argument("--gemm-study", action="store_true", help="scattering-moment products on MFMA, all formats")
     args = p.parse_args()

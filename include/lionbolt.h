@@ -45,7 +45,7 @@ enum {
 enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
 enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
 enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
-enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2 };
+enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2, LB_SWEEP_AUTO = 3 };
 
 typedef struct lb_op lb_op;
 
@@ -107,12 +107,21 @@ int lb_op_set_physics_groups(lb_op* op, int G, int Nm, const double* sigma_t, in
 int lb_op_set_scatter_mode(lb_op* op, int mode);
 
 /* How a sweep is issued: LB_SWEEP_LEVELS launches one kernel per global level
- * on the operator's stream; LB_SWEEP_GRAPH (default) captures those launches
- * once per (source, flux, group range) into a HIP graph and replays it;
+ * on the operator's stream; LB_SWEEP_GRAPH captures those launches once per
+ * (source, flux, group range) into a HIP graph and replays it;
  * LB_SWEEP_DIRECTION runs one kernel in which each workgroup owns one
- * direction and walks its levels with workgroup barriers.  No reference
- * counterpart (the reference sweeps one direction per OpenMP thread). */
+ * direction and walks its levels with workgroup barriers; LB_SWEEP_AUTO
+ * (default) times the graph mode and the direction-resident mode at 256, 512
+ * and 1024 threads on the first sweeps of each group-batch size (two sweeps
+ * each, synchronising after each) and then keeps the fastest.  All modes give
+ * bit-identical results.  No reference counterpart (the reference sweeps one
+ * direction per OpenMP thread). */
 int lb_op_set_sweep_mode(lb_op* op, int mode);
+
+/* The launch configuration LB_SWEEP_AUTO settled on for a group batch of G:
+ * *mode is LB_SWEEP_GRAPH or LB_SWEEP_DIRECTION (with *threads its block), or
+ * LB_SWEEP_AUTO while that batch size is still being timed. */
+int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads);
 
 /* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
 int lb_op_set_direction_block(lb_op* op, int threads);

@@ -207,13 +207,29 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 }
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
-  if (!op || (mode != LB_SWEEP_LEVELS && mode != LB_SWEEP_GRAPH && mode != LB_SWEEP_DIRECTION)) {
+  if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_AUTO) {
     set_error("lb_op_set_sweep_mode: invalid mode");
     return LB_E_ARG;
   }
   if (op->stream) (void)hipStreamSynchronize(op->stream);
   clear_graphs(op);
   op->sweep_mode = mode;
+  return LB_OK;
+}
+
+int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads) {
+  if (!op || !mode || !threads || G < 1) {
+    set_error("lb_op_sweep_choice: invalid argument");
+    return LB_E_ARG;
+  }
+  auto it = op->tune.find(G);
+  if (it == op->tune.end() || it->second.choice < 0) {
+    *mode = LB_SWEEP_AUTO;
+    *threads = 0;
+  } else {
+    *mode = kTuneConfig[it->second.choice].mode;
+    *threads = kTuneConfig[it->second.choice].block;
+  }
   return LB_OK;
 }
 

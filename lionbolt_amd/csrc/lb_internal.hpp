@@ -59,6 +59,20 @@ struct SweepGraph {
   hipGraphExec_t exec;
 };
 
+// Launch configurations the sweep autotuner times, and its state per group batch.
+struct SweepConfig {
+  int mode, block;
+};
+constexpr int kTuneConfigs = 4;
+constexpr SweepConfig kTuneConfig[kTuneConfigs] = {
+    {LB_SWEEP_GRAPH, 0}, {LB_SWEEP_DIRECTION, 256}, {LB_SWEEP_DIRECTION, 512}, {LB_SWEEP_DIRECTION, 1024}};
+constexpr int kTuneTrials = 2;
+struct SweepTune {
+  int trials = 0;                        // timed sweeps so far (config = trials % kTuneConfigs)
+  float best_ms[kTuneConfigs] = {1e30f, 1e30f, 1e30f, 1e30f};
+  int choice = -1;                       // index into kTuneConfig once settled
+};
+
 void clear_graphs(lb_op* op);
 
 }  // namespace lb
@@ -97,7 +111,8 @@ struct lb_op {
   std::vector<double> h_sigs;       // [Nm][G][L+1]
   lb::DevBuf d_sigt, d_sigs;
   int scatter_mode = LB_SCATTER_MOMENT;
-  int sweep_mode = LB_SWEEP_GRAPH;
+  int sweep_mode = LB_SWEEP_AUTO;
+  std::map<int, lb::SweepTune> tune;  // autotuner state per group-batch size
   lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
   lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep

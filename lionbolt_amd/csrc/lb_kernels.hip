@@ -430,12 +430,9 @@ inline unsigned grid_stride(int64_t n) {
 
 }  // namespace
 
-int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
-  if (G < 1 || G > kBlock) {
-    set_error("sweep: the group batch must be between 1 and 256");
-    return LB_E_ARG;
-  }
-  if (op->sweep_mode == LB_SWEEP_DIRECTION && (op->dir_block / G) < 1) {
+// One sweep issued in a fixed launch configuration (mode, direction block).
+static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, int mode, int block) {
+  if (mode == LB_SWEEP_DIRECTION && (block / G) < 1) {
     set_error("sweep: the direction-resident block is smaller than the group batch");
     return LB_E_ARG;
   }
@@ -456,7 +453,7 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
               op->G, g0, G, op->NE, q, psi};
 
-  if (op->sweep_mode == LB_SWEEP_DIRECTION) {
+  if (mode == LB_SWEEP_DIRECTION) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
       LB_HIP(hipMemcpy(op->d_lvl_off.p, op->h_lvl_off.data(), op->h_lvl_off.size() * sizeof(int32_t),
@@ -467,9 +464,9 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
     LB_HIP(hipEventRecord(op->ev0, op->stream));
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
-    if (op->dir_block == 256)
+    if (block == 256)
       hipLaunchKernelGGL(sweep_direction_kernel<256>, dim3(op->NI), dim3(256), 0, op->stream, lo, nl, op->maxlev, A);
-    else if (op->dir_block == 512)
+    else if (block == 512)
       hipLaunchKernelGGL(sweep_direction_kernel<512>, dim3(op->NI), dim3(512), 0, op->stream, lo, nl, op->maxlev, A);
     else
       hipLaunchKernelGGL(sweep_direction_kernel<1024>, dim3(op->NI), dim3(1024), 0, op->stream, lo, nl, op->maxlev, A);
@@ -485,7 +482,7 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
       hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
     }
   };
-  if (op->sweep_mode == LB_SWEEP_GRAPH) {
+  if (mode == LB_SWEEP_GRAPH) {
     // The level launches of one (source, flux, group range) are captured once into a
     // graph and replayed; the cache is cleared whenever the physics table moves.
     hipGraphExec_t exec = nullptr;
@@ -523,6 +520,37 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   enqueue();
   LB_HIP(hipGetLastError());
   LB_HIP(hipEventRecord(op->ev1, op->stream));
+  return LB_OK;
+}
+
+int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
+  if (G < 1 || G > kBlock) {
+    set_error("sweep: the group batch must be between 1 and 256");
+    return LB_E_ARG;
+  }
+  if (op->sweep_mode != LB_SWEEP_AUTO) return launch_sweep_as(op, q, psi, g0, G, op->sweep_mode, op->dir_block);
+  SweepTune& tn = op->tune[G];
+  if (tn.choice >= 0)
+    return launch_sweep_as(op, q, psi, g0, G, kTuneConfig[tn.choice].mode, kTuneConfig[tn.choice].block);
+  // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
+  // sweep timed alone between the operator's events; configurations whose block
+  // cannot hold the group batch are skipped.  Results are identical in every mode.
+  int c = tn.trials % kTuneConfigs;
+  while (kTuneConfig[c].mode == LB_SWEEP_DIRECTION && kTuneConfig[c].block < G) {
+    ++tn.trials;
+    c = tn.trials % kTuneConfigs;
+  }
+  LB_TRY(launch_sweep_as(op, q, psi, g0, G, kTuneConfig[c].mode, kTuneConfig[c].block));
+  LB_HIP(hipEventSynchronize(op->ev1));
+  float ms = 0.f;
+  LB_HIP(hipEventElapsedTime(&ms, op->ev0, op->ev1));
+  tn.best_ms[c] = std::min(tn.best_ms[c], ms);
+  if (++tn.trials >= kTuneTrials * kTuneConfigs) {
+    int best = 0;
+    for (int i = 1; i < kTuneConfigs; ++i)
+      if (tn.best_ms[i] < tn.best_ms[best]) best = i;
+    tn.choice = best;
+  }
   return LB_OK;
 }
 

@@ -87,9 +87,19 @@ class Operator:
         m = {"moment": _lib.SCATTER_MOMENT, "dense": _lib.SCATTER_DENSE}[mode]
         _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
 
+    _SWEEP_MODES = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION,
+                    "auto": _lib.SWEEP_AUTO}
+
     def set_sweep_mode(self, mode):
-        m = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION}[mode]
-        _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, m))
+        _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, self._SWEEP_MODES[mode]))
+
+    def sweep_choice(self, G=1):
+        """(mode name, direction block) the autotuner settled on for a group batch of G,
+        or ("auto", 0) while it is still timing."""
+        mode, threads = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().lb_op_sweep_choice(self.h, int(G), ctypes.byref(mode), ctypes.byref(threads)))
+        name = {v: k for k, v in self._SWEEP_MODES.items()}[mode.value]
+        return name, threads.value
 
     def set_direction_block(self, threads):
         _lib.check(_lib.lib().lb_op_set_direction_block(self.h, int(threads)))

@@ -49,3 +49,31 @@ def test_direction_resident_in_place_and_solver():
     assert sa["iterates"] == sb["iterates"]
     assert np.array_equal(a, b)
     assert np.array_equal(t1, t2)
+
+
+@pytest.mark.parametrize("G", [1, 24])
+def test_autotuned_sweeps_equal_level_launches(G):
+    """The autotuner times every configuration on live sweeps; every sweep it
+    issues (timed or settled) matches the level launches bit for bit."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(5, a=4.0, seed=1011, slab=(1.5, 2.5))
+    ang = angular.product(4, 8, 1)
+    op = Operator(m, ang)
+    sigt = np.stack([np.linspace(0.2, 1.1, G), np.linspace(0.5, 3.5, G)], axis=1)
+    op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
+    q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
+    op.set_sweep_mode("levels")
+    ref = torch.empty_like(q)
+    op.transport_matinv_dev(q, ref, 0, G)
+    op.synchronize()
+    ref = ref.cpu().numpy()
+    op.set_sweep_mode("auto")
+    assert op.sweep_choice(G) == ("auto", 0)
+    for _ in range(10):
+        psi = torch.full_like(q, float("nan"))
+        op.transport_matinv_dev(q, psi, 0, G)
+        op.synchronize()
+        assert np.array_equal(psi.cpu().numpy(), ref)
+    mode, threads = op.sweep_choice(G)
+    assert (mode, threads) in {("graph", 0), ("direction", 256), ("direction", 512), ("direction", 1024)}
