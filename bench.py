@@ -196,17 +196,17 @@ def run(args):
         raise RuntimeError("non-finite flux")
     other_modes = {}
     if args.compare_modes:
-        for mode in ("levels", "graph", "direction"):
-            if mode == args.sweep_mode or mode == "auto":
+        for other in ("levels", "graph", "direction"):
+            if other == args.sweep_mode:
                 continue
-            op.set_sweep_mode(mode)
+            op.set_sweep_mode(other)
             op.transport_matinv_dev(q, psi, 0, G)
             op.synchronize()
             t1 = time.perf_counter()
             for _ in range(3):
                 op.transport_matinv_dev(q, psi, 0, G)
             op.synchronize()
-            other_modes[mode] = {"ms_per_sweep": 1000.0 * (time.perf_counter() - t1) / 3}
+            other_modes[other] = {"ms_per_sweep": 1000.0 * (time.perf_counter() - t1) / 3}
     levels = op.maxlev
     launches = 1 if mode == "direction" else levels
     kernel = f"sweep_direction_kernel<{block}>" if mode == "direction" else "sweep_level_kernel"
