@@ -17,7 +17,12 @@ The line also carries:
   group, 32 of the 256 directions, the full 384k-tet mesh;
 * solve: time to a 1e-6 relative transport residual of the full 24-group,
   P3, downscatter problem with the first-collision source of a +z beam
-  (group-by-group within-group GMRES on flux moments, plain, FP64).
+  (group-by-group within-group GMRES on flux moments, plain, FP64);
+* reference_method_gpu: the same problem with the reference's own method
+  (psi-space GMRES(10) per group), for its iteration counts;
+* cfg1_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
+* cpu_baseline also carries the oracle's cfg1 solve time and a projected
+  (lower-bound) CPU time of the 24-group solve by the reference's method.
 """
 import argparse
 import json
@@ -54,6 +59,44 @@ def cpu_baseline(m, ang, ndirs=32, reps=2):
             "sample": f"1 group x {sub.NI} of 256 directions x {m.NE} tets, FP64 oracle sweep, median of {reps}"}
 
 
+def cpu_solve_baseline(ref_method):
+    """The oracle's end-to-end solves on this host.
+
+    cfg1 (48,000 tets, S8, P0 c = 0.9, GMRES(30) at 1e-8) is run to convergence.
+    The 24-group cfg2 solve is projected: the reference's own method (psi-space
+    GMRES(10) with the dense direction x direction scattering kernel) at the
+    iteration counts that method takes on the GPU in this run, times the oracle's
+    full-size sweep and dense-K times measured here.  The projection counts only
+    sweeps and K applications (no Krylov vector work), so it is a lower bound."""
+    from lionbolt_amd import problems, angular
+    from oracle import oracle
+    out = {}
+    m, ang, sigt, sigs, q = problems.cfg1()
+    op = oracle.OracleOp(m, ang)
+    op.set_physics(sigt, sigs)
+    t0 = time.perf_counter()
+    _, st = op.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)
+    out["cfg1_solve"] = {"seconds": time.perf_counter() - t0, "iterates": st["iters"], "status": st["status"],
+                         "method": "psi-space GMRES(30), dense K, tol 1e-8"}
+    if ref_method:
+        m2 = problems.cfg2_mesh(10)                 # dense K timed on 6,000 tets x 256 directions, scaled by NE
+        ang2 = problems.cfg2_angular()
+        op2 = oracle.OracleOp(m2, ang2)
+        op2.set_physics([0.3, 0.9], [[0.2, 0.1, 0.05, 0.02], [0.6, 0.3, 0.15, 0.06]])
+        x = np.random.default_rng(1003).random(op2.n)
+        op2.scatter_matvec(x)
+        t0 = time.perf_counter()
+        op2.scatter_matvec(x)
+        k_s = (time.perf_counter() - t0) * (384000 / m2.NE)
+        sweep_s = ref_method["sweep_s_full"]
+        sweeps = ref_method["sweeps"]
+        proj = sweeps * (sweep_s + k_s)
+        out["cfg2_solve_projected"] = {
+            "seconds": proj, "sweeps_and_K": sweeps, "cpu_sweep_s": sweep_s, "cpu_dense_K_s": k_s,
+            "note": "lower bound: sweeps + dense K at the reference method's measured iteration counts"}
+    return out
+
+
 def copy_calibration(gib=4):
     """A plain device copy of known size (for PMC byte-counter scaling) and its bandwidth."""
     import torch
@@ -83,15 +126,7 @@ def solve_leg(m, ang, rtol):
     sigt, S = problems.cfg2_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
-    # first-collision source moments built on the device (G x nmom x NENK)
-    phi0 = torch.from_numpy(problems.beam_uncollided(m, sigt[0])).cuda()
-    y0 = torch.from_numpy(angular.real_sph_harm(ang.L, np.array([[0.0, 0.0, 1.0]]))[:, 0]).cuda()
-    deg = torch.from_numpy(angular.moment_degree(ang.L)).cuda()
-    mat_dof = torch.from_numpy(np.repeat(m.mat - 1, 4)).cuda()
-    S_d = torch.from_numpy(S).cuda()
-    qmom = torch.empty((G, ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
-    for g in range(G):
-        qmom[g] = S_d[0, g][mat_dof][:, deg].T * y0[:, None] * phi0[None, :]
+    qmom = _beam_qmom(m, ang, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -103,6 +138,68 @@ def solve_leg(m, ang, rtol):
             "max_true_residual": max(s["true_residual"] for s in stats),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
             "method": "plain GMRES(30) on flux moments per group, FP64"}
+
+
+def reference_method_leg(m, ang, rtol):
+    """The reference's solver method on the GPU: psi-space GMRES(10) per group from
+    x0 = q/sigma_t(1) (BoltzmannOperator.f90:106-151), same multigroup problem and
+    stopping tolerance as the solve leg; K applied in moment form (the same
+    operator as the dense kernel).  Gives the iteration counts the CPU projection
+    uses, and the time of the reference method on the GPU."""
+    import torch
+    from lionbolt_amd import angular, problems
+    from lionbolt_amd.ops import Operator
+    G = 24
+    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, problems.within_group(S))
+    qmom = _beam_qmom(m, ang, sigt, S)
+    phi = torch.zeros_like(qmom)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="gmres", space="psi", m=10, rtol=rtol)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    op.close()
+    return {"seconds": wall, "rtol": rtol, "iterates": [s["iterates"] for s in stats],
+            "sweeps": sum(s["sweeps"] for s in stats),
+            "max_true_residual": max(s["true_residual"] for s in stats),
+            "method": "psi-space GMRES(10) per group (the reference's method), FP64"}
+
+
+def _beam_qmom(m, ang, sigt, S):
+    """First-collision source moments of the +z beam, per group (G x nmom x NENK)."""
+    import torch
+    from lionbolt_amd import angular, problems
+    G = sigt.shape[0]
+    phi0 = torch.from_numpy(problems.beam_uncollided(m, sigt[0])).cuda()
+    y0 = torch.from_numpy(angular.real_sph_harm(ang.L, np.array([[0.0, 0.0, 1.0]]))[:, 0]).cuda()
+    deg = torch.from_numpy(angular.moment_degree(ang.L)).cuda()
+    mat_dof = torch.from_numpy(np.repeat(m.mat - 1, 4)).cuda()
+    S_d = torch.from_numpy(S).cuda()
+    qmom = torch.empty((G, ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
+    for g in range(G):
+        qmom[g] = S_d[0, g][mat_dof][:, deg].T * y0[:, None] * phi0[None, :]
+    return qmom
+
+
+def cfg1_leg():
+    """cfg1 end to end on the GPU through the host entry point (vectors cross PCIe):
+    psi-space GMRES(30) at 1e-8 as the config states, and the moment-space solve."""
+    from lionbolt_amd import problems
+    from lionbolt_amd.ops import Operator
+    m, ang, sigt, sigs, q = problems.cfg1()
+    op = Operator(m, ang)
+    op.set_physics(sigt, sigs)
+    op.boltzmann_matinv(q, kind="gmres", space="moment", m=30, rtol=1e-8)    # warm-up (autotuner, graphs)
+    res = {}
+    for space in ("psi", "moment"):
+        t0 = time.perf_counter()
+        _, st = op.boltzmann_matinv(q, kind="gmres", space=space, m=30, rtol=1e-8)
+        res[space] = {"seconds": time.perf_counter() - t0, "iterates": st["iterates"],
+                      "true_residual": st["true_residual"]}
+    op.close()
+    return res
 
 
 def single_group_leg(m, ang, reps=10):
@@ -254,11 +351,27 @@ def run(args):
         out["single_group_sweep"] = single_group_leg(m, ang)
     if args.solve:
         out["solve"] = solve_leg(m, ang, args.solve_rtol)
+    ref = None
+    if args.ref_method:
+        out["reference_method_gpu"] = reference_method_leg(m, ang, 1e-6)
+        ref = {"sweeps": out["reference_method_gpu"]["sweeps"],
+               "sweep_s_full": None}
+    if args.cfg1:
+        out["cfg1_gpu"] = cfg1_leg()
     if args.gemm_study:
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(m, ang)
+        if ref is not None:
+            ref["sweep_s_full"] = m.NE * ang.NI / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(cpu_solve_baseline(ref))
+        proj = out["cpu_baseline"].get("cfg2_solve_projected")
+        if proj and "solve" in out:
+            proj["gpu_solve_s"] = out["solve"]["seconds"]
+            proj["speedup_vs_gpu_solve"] = proj["seconds"] / out["solve"]["seconds"]
+        if "cfg1_gpu" in out:
+            out["cfg1_gpu"]["speedup_vs_cpu"] = out["cpu_baseline"]["cfg1_solve"]["seconds"] / out["cfg1_gpu"]["psi"]["seconds"]
     print(json.dumps(out))
 
 
@@ -273,6 +386,9 @@ def main():
     p.add_argument("--no-solve", dest="solve", action="store_false")
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
+    p.add_argument("--no-ref-method", dest="ref_method", action="store_false",
+                   help="skip the reference-method (psi-space GMRES(10)) solve on the GPU")
+    p.add_argument("--no-cfg1", dest="cfg1", action="store_false")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")

@@ -214,3 +214,22 @@ def test_graph_and_level_launch_modes_agree(ops_module):
     for mode, vals in outs.items():
         for v in vals:
             assert np.array_equal(v, ref), mode
+
+
+def test_cfg1_full_solve_matches_oracle(ops_module):
+    """48,000 tets, S8, P0 c = 0.9, central-cube source, GMRES(30) at 1e-8: the
+    reference's own CPU-runnable configuration, solved end to end."""
+    from lionbolt_amd import problems
+    m, ang, sigt, sigs, q = problems.cfg1()
+    lib_op, orc = make_pair(ops_module, m, ang, sigt, sigs)
+    a, st = lib_op.boltzmann_matinv(q, kind="gmres", space="psi", m=30, rtol=1e-8)
+    b, ost = orc.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)
+    assert st["status"] == 0 and ost["status"] == 0
+    assert st["iterates"] == ost["iters"]
+    assert st["true_residual"] <= 1e-8 * 1.01
+    phi_a, phi_b = lib_op.fluence(a), orc.fluence(b)
+    assert rel(phi_a, phi_b) <= 1e-5
+    # the moment-space solve reaches the same flux
+    c, sc = lib_op.boltzmann_matinv(q, kind="gmres", space="moment", m=30, rtol=1e-8)
+    assert sc["status"] == 0 and sc["true_residual"] <= 1e-7
+    assert rel(lib_op.fluence(c), phi_b) <= 1e-5
