@@ -21,8 +21,12 @@ The line also carries:
 * reference_method_gpu: the same problem with the reference's own method
   (psi-space GMRES(10) per group), for its iteration counts;
 * cfg1_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
-* cpu_baseline also carries the oracle's cfg1 solve time and a projected
-  (lower-bound) CPU time of the 24-group solve by the reference's method.
+* batch_cfg4: 64 uneven single-group S8 beam problems (2k-56k tets) swept as
+  one batch (persistent work-queue kernel, autotuned) and solved with
+  per-problem GMRES in lockstep;
+* cpu_baseline also carries the oracle's cfg1 solve time, a projected
+  (lower-bound) CPU time of the 24-group solve by the reference's method and
+  the oracle's sweep rate on the batch problems.
 """
 import argparse
 import json
@@ -78,6 +82,21 @@ def cpu_solve_baseline(ref_method):
     _, st = op.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)
     out["cfg1_solve"] = {"seconds": time.perf_counter() - t0, "iterates": st["iters"], "status": st["status"],
                          "method": "psi-space GMRES(30), dense K, tol 1e-8"}
+    # cfg4: the oracle sweeps 4 of the 64 batch problems (all 80 directions) one
+    # after another; rate in cell-angle sweeps/s
+    probs = problems.cfg4()
+    s8 = angular.level_symmetric_s8()
+    tot_ca, tot_s = 0, 0.0
+    for pr in probs[:4]:
+        o = oracle.OracleOp(pr["mesh"], s8)
+        o.set_physics([pr["sigt"]], [[pr["c"] * pr["sigt"]]])
+        x = np.random.default_rng(1004).random(o.n)
+        t0 = time.perf_counter()
+        o.sweep_only(x)
+        tot_s += time.perf_counter() - t0
+        tot_ca += pr["mesh"].NE * s8.NI
+    out["cfg4_sweep"] = {"cell_angle_sweeps_per_s": tot_ca / tot_s,
+                         "sample": "problems 1-4 of the 64, one sweep each, all 80 directions"}
     if ref_method:
         m2 = problems.cfg2_mesh(10)                 # dense K timed on 6,000 tets x 256 directions, scaled by NE
         ang2 = problems.cfg2_angular()
@@ -202,6 +221,92 @@ def cfg1_leg():
     return res
 
 
+def batch_leg(reps=10, rtol=1e-6):
+    """cfg4: 64 uneven single-group S8 beam problems as one batch.  Sweep rate of the
+    batched sweep per launch mode (the autotuned choice included), the same 64
+    problems swept one operator at a time, and the batched per-problem GMRES."""
+    import torch
+    from lionbolt_amd import angular, problems
+    from lionbolt_amd.ops import Operator
+    um, eoff, sigt, sigs, qmom, probs = problems.cfg4_batch()
+    ang = angular.level_symmetric_s8()
+    ca = um.NE * ang.NI
+    op = Operator(um, ang)
+    op.set_physics(sigt, sigs)
+    op.set_problems(eoff)
+    q = torch.rand(op.n, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(q)
+
+    def timed(k):
+        op.transport_matinv_dev(q, psi, 0, 1)
+        op.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            op.transport_matinv_dev(q, psi, 0, 1)
+        op.synchronize()
+        return 1000.0 * (time.perf_counter() - t0) / k
+
+    res = {"problems": len(probs), "tets": um.NE, "directions": ang.NI,
+           "tets_min": int(min(p["mesh"].NE for p in probs)), "tets_max": int(max(p["mesh"].NE for p in probs))}
+    modes = {}
+    op.set_sweep_mode("auto")
+    for _ in range(32):
+        if op.sweep_choice(1)[0] != "auto":
+            break
+        op.transport_matinv_dev(q, psi, 0, 1)
+    op.synchronize()
+    choice = op.sweep_choice(1)
+    ms = timed(reps)
+    modes["auto"] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3), "choice": list(choice)}
+    for name, mode, block, per_cu in (("levels", "levels", 0, 0), ("graph", "graph", 0, 0),
+                                      ("persistent-1024x2", "persistent", 1024, 2),
+                                      ("persistent-256x8", "persistent", 256, 8),
+                                      ("direction-1024", "direction", 1024, 0)):
+        if block:
+            op.set_direction_block(block)
+        if per_cu:
+            op.set_workgroups_per_cu(per_cu)
+        op.set_sweep_mode(mode)
+        ms = timed(reps)
+        modes[name] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3)}
+    res["sweep"] = modes
+    # the same problems swept one operator at a time (each with its own autotuned launch)
+    singles = []
+    for pr in probs:
+        o = Operator(pr["mesh"], ang)
+        o.set_physics([pr["sigt"]], [[pr["c"] * pr["sigt"]]])
+        qq = torch.rand(o.n, dtype=torch.float64, device="cuda")
+        for _ in range(10):
+            o.transport_matinv_dev(qq, qq, 0, 1)
+        singles.append((o, qq))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        for o, qq in singles:
+            o.transport_matinv_dev(qq, qq, 0, 1)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / 3
+    res["one_operator_per_problem"] = {"ms_per_sweep_of_all": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3)}
+    for o, _ in singles:
+        o.close()
+    del singles
+    # batched solve from the first-collision sources
+    op.set_sweep_mode("auto")
+    qm = torch.from_numpy(np.ascontiguousarray(qmom.reshape(-1))).cuda()
+    phi = torch.zeros_like(qm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = op.batch_solve_dev(qm, phi, m=30, rtol=rtol)
+    torch.cuda.synchronize()
+    res["solve"] = {"seconds": time.perf_counter() - t0, "rtol": rtol,
+                    "iterates_max": max(s["iterates"] for s in st),
+                    "iterates_mean": float(np.mean([s["iterates"] for s in st])),
+                    "sweeps": st[0]["sweeps"], "max_true_residual": max(s["true_residual"] for s in st),
+                    "method": "per-problem flux-moment GMRES(30) in lockstep, one sweep launch per step, FP64"}
+    op.close()
+    return res
+
+
 def single_group_leg(m, ang, reps=10):
     """One-group sweeps of the same mesh and directions (what a within-group solve
     issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s."""
@@ -270,7 +375,7 @@ def run(args):
                 break
             op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
-        mode, block = op.sweep_choice(G)
+        mode, block, _ = op.sweep_choice(G)
     elif args.sweep_mode == "direction":
         block = 1024
     for _ in range(args.warmup):
@@ -358,6 +463,8 @@ def run(args):
                "sweep_s_full": None}
     if args.cfg1:
         out["cfg1_gpu"] = cfg1_leg()
+    if args.batch:
+        out["batch_cfg4"] = batch_leg()
     if args.gemm_study:
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
@@ -389,6 +496,7 @@ def main():
     p.add_argument("--no-ref-method", dest="ref_method", action="store_false",
                    help="skip the reference-method (psi-space GMRES(10)) solve on the GPU")
     p.add_argument("--no-cfg1", dest="cfg1", action="store_false")
+    p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-problem batch (cfg4)")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")

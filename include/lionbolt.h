@@ -45,7 +45,7 @@ enum {
 enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
 enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
 enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
-enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2, LB_SWEEP_AUTO = 3 };
+enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2, LB_SWEEP_AUTO = 3, LB_SWEEP_PERSISTENT = 4 };
 
 typedef struct lb_op lb_op;
 
@@ -119,9 +119,35 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
 int lb_op_set_sweep_mode(lb_op* op, int mode);
 
 /* The launch configuration LB_SWEEP_AUTO settled on for a group batch of G:
- * *mode is LB_SWEEP_GRAPH or LB_SWEEP_DIRECTION (with *threads its block), or
- * LB_SWEEP_AUTO while that batch size is still being timed. */
-int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads);
+ * *mode is LB_SWEEP_GRAPH, LB_SWEEP_DIRECTION or LB_SWEEP_PERSISTENT (with
+ * *threads its workgroup size and *per_cu the persistent grid's workgroups per
+ * compute unit), or LB_SWEEP_AUTO while that batch size is still being timed. */
+int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads, int* per_cu);
+
+/* Workgroups per compute unit of the persistent sweep's grid (1..8, with
+ * threads x per_cu <= 2048); the workgroup size is lb_op_set_direction_block's. */
+int lb_op_set_workgroups_per_cu(lb_op* op, int per_cu);
+
+/* Batches of independent problems (no reference counterpart: the reference
+ * solves one problem per run).  The operator's mesh is the disjoint union of P
+ * problems in consecutive element ranges [elem_off[p], elem_off[p+1]) (0-based,
+ * elem_off[0] = 0, elem_off[P] = NE) with no face between ranges; give each
+ * problem its own material to give it its own cross sections.  Enables
+ * LB_SWEEP_PERSISTENT: one launch of a fixed grid that pulls (problem,
+ * direction) items from a work queue, largest problems first, replayed from a
+ * HIP graph; LB_SWEEP_AUTO then times it (several workgroup sizes and grid
+ * occupancies) against the level launches. */
+int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off);
+
+/* Solves every problem of the batch, single group: flux-moment GMRES(m) per
+ * problem, all problems in lockstep with one sweep launch per Arnoldi step
+ * (converged problems leave the sweep).  d_qmom [nmom][NENK]: moments of each
+ * problem's external source (discrete source = M qmom); d_phi [nmom][NENK]:
+ * collided flux moments out.  stats[P]: per-problem status, Arnoldi steps and
+ * true transport residual.  opts: restart, maxit, rtol, atol (kind and space
+ * are ignored). */
+int lb_batch_solve_dev(lb_op* op, const double* d_qmom, const lb_solver_opts* opts, double* d_phi,
+                       lb_solve_stats* stats);
 
 /* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
 int lb_op_set_direction_block(lb_op* op, int threads);

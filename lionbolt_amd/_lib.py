@@ -15,7 +15,7 @@ ERRORS = {-1: "LB_E_ARG", -2: "LB_E_STAGNANT", -3: "LB_E_NOT_READY", -4: "LB_E_H
 SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
-SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO = 0, 1, 2, 3
+SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT = 0, 1, 2, 3, 4
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)
@@ -53,7 +53,11 @@ SIGNATURES = {
     "lb_op_set_physics_groups": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d]),
     "lb_op_set_scatter_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_op_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "lb_op_sweep_choice": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "lb_op_sweep_choice": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "lb_op_set_workgroups_per_cu": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lb_op_set_problems": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "lb_batch_solve_dev": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(SolverOpts), _vp, ctypes.POINTER(SolveStats)]),
     "lb_op_set_direction_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_transport_matinv": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),
     "lb_scatter_matvec": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),

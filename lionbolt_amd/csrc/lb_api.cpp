@@ -207,7 +207,7 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 }
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
-  if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_AUTO) {
+  if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_PERSISTENT) {
     set_error("lb_op_set_sweep_mode: invalid mode");
     return LB_E_ARG;
   }
@@ -217,20 +217,60 @@ int lb_op_set_sweep_mode(lb_op* op, int mode) {
   return LB_OK;
 }
 
-int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads) {
-  if (!op || !mode || !threads || G < 1) {
+int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads, int* per_cu) {
+  if (!op || !mode || !threads || !per_cu || G < 1) {
     set_error("lb_op_sweep_choice: invalid argument");
     return LB_E_ARG;
   }
-  auto it = op->tune.find(G);
-  if (it == op->tune.end() || it->second.choice < 0) {
+  const bool batched = op->batch.P > 0;
+  auto& tune = batched ? op->btune : op->tune;
+  const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
+  auto it = tune.find(G);
+  if (it == tune.end() || it->second.choice < 0) {
     *mode = LB_SWEEP_AUTO;
     *threads = 0;
+    *per_cu = 0;
   } else {
-    *mode = kTuneConfig[it->second.choice].mode;
-    *threads = kTuneConfig[it->second.choice].block;
+    *mode = table[it->second.choice].mode;
+    *threads = table[it->second.choice].block;
+    *per_cu = table[it->second.choice].per_cu;
   }
   return LB_OK;
+}
+
+int lb_op_set_workgroups_per_cu(lb_op* op, int per_cu) {
+  if (!op || per_cu < 1 || per_cu > 8) {
+    set_error("lb_op_set_workgroups_per_cu: per_cu must be 1..8");
+    return LB_E_ARG;
+  }
+  op->per_cu = per_cu;
+  return LB_OK;
+}
+
+int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off) {
+  if (!op || op->host_only) {
+    set_error("lb_op_set_problems: null operator");
+    return LB_E_ARG;
+  }
+  return set_problems(op, P, elem_off);
+}
+
+int lb_batch_solve_dev(lb_op* op, const double* d_qmom, const lb_solver_opts* opts, double* d_phi,
+                       lb_solve_stats* stats) {
+  LB_TRY(check_ready(op, "lb_batch_solve_dev"));
+  if (op->batch.P == 0) {
+    set_error("lb_batch_solve_dev: no problem set (lb_op_set_problems)");
+    return LB_E_ARG;
+  }
+  if (op->G != 1) {
+    set_error("lb_batch_solve_dev: the batched solve is single-group");
+    return LB_E_ARG;
+  }
+  if (!d_qmom || !d_phi) {
+    set_error("lb_batch_solve_dev: null vector");
+    return LB_E_ARG;
+  }
+  return batch_solve_dev(op, d_qmom, opts, d_phi, stats);
 }
 
 int lb_op_set_direction_block(lb_op* op, int threads) {
@@ -390,12 +430,22 @@ void lb_op_destroy(lb_op* op) {
   clear_graphs(op);
   for (DevBuf* b : {&op->d_geo, &op->d_nbr, &op->d_perm, &op->d_mat, &op->d_tasks, &op->d_k, &op->d_w, &op->d_D,
                     &op->d_M, &op->d_deg, &op->d_yys, &op->d_sigt, &op->d_sigs, &op->w_a, &op->w_a2, &op->w_b,
-                    &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov})
+                    &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov, &op->w_mg, &op->w_acc, &op->w_S})
     b->release();
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);
   op->d_rec.release();
   op->d_ck.release();
+  for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec, &op->batch.d_ck,
+                    &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
+                    &op->batch.d_out})
+    b->release();
+  for (auto& kv : op->batch.seg) {
+    kv.second.d_tiles.release();
+    kv.second.d_tile_off.release();
+  }
+  if (op->batch.h_io) (void)hipHostFree(op->batch.h_io);
+  op->batch.h_io = nullptr;
   op->d_lvl_off.release();
   op->d_nlev.release();
   if (op->ev0) (void)hipEventDestroy(op->ev0);

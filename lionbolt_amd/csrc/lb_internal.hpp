@@ -55,22 +55,52 @@ constexpr int kFaceNodes[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 struct SweepGraph {
   const double* q;
   double* psi;
-  int g0, G;
+  int g0, G, mode, block, per_cu;
   hipGraphExec_t exec;
 };
 
 // Launch configurations the sweep autotuner times, and its state per group batch.
+// A single problem chooses among the graph-replayed level launches and the
+// direction-resident kernel; a batch of independent problems among the level
+// launches and the persistent work-queue kernel (block, workgroups per CU).
 struct SweepConfig {
-  int mode, block;
+  int mode, block, per_cu;
 };
 constexpr int kTuneConfigs = 4;
 constexpr SweepConfig kTuneConfig[kTuneConfigs] = {
-    {LB_SWEEP_GRAPH, 0}, {LB_SWEEP_DIRECTION, 256}, {LB_SWEEP_DIRECTION, 512}, {LB_SWEEP_DIRECTION, 1024}};
+    {LB_SWEEP_GRAPH, 0, 0}, {LB_SWEEP_DIRECTION, 256, 0}, {LB_SWEEP_DIRECTION, 512, 0}, {LB_SWEEP_DIRECTION, 1024, 0}};
+constexpr int kBatchTuneConfigs = 8;
+constexpr SweepConfig kBatchTuneConfig[kBatchTuneConfigs] = {
+    {LB_SWEEP_GRAPH, 0, 0},         {LB_SWEEP_PERSISTENT, 256, 4},  {LB_SWEEP_PERSISTENT, 256, 8},
+    {LB_SWEEP_PERSISTENT, 512, 2},  {LB_SWEEP_PERSISTENT, 512, 4},  {LB_SWEEP_PERSISTENT, 1024, 1},
+    {LB_SWEEP_PERSISTENT, 1024, 2}, {LB_SWEEP_DIRECTION, 1024, 0}};
 constexpr int kTuneTrials = 2;
 struct SweepTune {
-  int trials = 0;                        // timed sweeps so far (config = trials % kTuneConfigs)
-  float best_ms[kTuneConfigs] = {1e30f, 1e30f, 1e30f, 1e30f};
-  int choice = -1;                       // index into kTuneConfig once settled
+  int trials = 0;                        // timed sweeps so far (config = trials % number of configs)
+  float best_ms[8] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
+  int choice = -1;                       // index into the config table once settled
+};
+
+// Contiguous (problem, row) ranges of a [rows][NENK] vector, split into tiles, for
+// the per-problem reductions and updates of batched solves.
+struct SegTable {
+  std::vector<int64_t> tiles;            // 4 per tile: problem, start, length, 0
+  std::vector<int32_t> tile_off;         // [P + 1] tiles of problem p
+  DevBuf d_tiles, d_tile_off;
+};
+
+// A mesh made of independent problems in consecutive element ranges.
+struct BatchTable {
+  int P = 0;
+  std::vector<int64_t> eoff;             // [P + 1] element offsets
+  std::vector<int32_t> pmat;             // material of each problem's first element
+  std::vector<int32_t> items;            // 4 per (problem, direction) item: p, i, level base, level count
+  std::vector<int32_t> lvl;              // absolute level offsets (sweep positions in batch order)
+  std::vector<uint32_t> tasks;           // [NI][NE] task words, problem-major inside each direction
+  DevBuf d_items, d_lvl, d_tasks, d_rec, d_ck, d_active, d_counter, d_coef, d_part, d_out;
+  double* h_io = nullptr;                // pinned host staging for per-problem scalars
+  std::map<int, SegTable> seg;           // by rows
+  int ncu = 0;
 };
 
 void clear_graphs(lb_op* op);
@@ -116,7 +146,11 @@ struct lb_op {
   lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
   lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
-  int dir_block = 1024;             // workgroup size of the direction-resident sweep
+  int dir_block = 1024;             // workgroup size of the direction-resident and persistent sweeps
+  int per_cu = 2;                   // persistent sweep: resident workgroups per compute unit
+  lb::BatchTable batch;             // independent problems (set by lb_op_set_problems)
+  std::map<int, lb::SweepTune> btune;  // autotuner state of batched sweeps, per group-batch size
+  std::vector<int32_t> h_nbr;       // int32 [NE][4] neighbours (-1 boundary)
   std::vector<lb::SweepGraph> graphs;
   // runtime
   hipStream_t stream = nullptr;
@@ -149,6 +183,14 @@ int launch_scale(hipStream_t s, int64_t n, double a, double* x);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int launch_fluence(lb_op* op, const double* v, double* phi);
+
+// lb_batch.hip
+int set_problems(lb_op* op, int P, const int64_t* eoff);
+SegTable* seg_table(lb_op* op, int rows, int* rc);
+int seg_dot(lb_op* op, SegTable* t, const double* x, const double* y, double* out);
+int seg_axpy(lb_op* op, SegTable* t, const double* coef, const double* x, double* y);
+int seg_scale(lb_op* op, SegTable* t, const double* coef, double* x);
+int batch_solve_dev(lb_op* op, const double* qmom, const lb_solver_opts* opts, double* phi, lb_solve_stats* stats);
 
 // lb_krylov.cpp
 int boltzmann_matvec_dev(lb_op* op, double* x, int g0);

@@ -235,25 +235,16 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
   solve_cell(A, wd.x, wd.y + c, g);
 }
 
-// Direction-resident sweep: one workgroup owns one direction (of one batch of
-// groups) and walks its levels in order, with a workgroup barrier between
-// levels.  All producers and consumers of a direction's upstream traces run on
-// one CU, so workgroup-scope ordering is enough: no inter-workgroup flags, no
-// agent-scope cache write-back or invalidation, and the traces of the last few
-// levels are re-read from the XCD's L2.  One launch sweeps every direction.
+// Walks the levels of one direction (positions off[0..nl]) with the whole
+// workgroup: BLOCK / G cells per round, a workgroup barrier between levels.  The
+// first cell of each level for this lane is prepared one level ahead, so only
+// the upstream-trace loads sit between two barriers.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* __restrict__ lvl_off,
-                                                                const int32_t* __restrict__ nlev, int maxlev,
-                                                                SweepArgs A) {
-  const int i = blockIdx.x;
-  const int32_t* off = lvl_off + (int64_t)i * (maxlev + 1);
-  const int nl = nlev[i];
+__device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl) {
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
-  // The first cell of each level for this lane is prepared one level ahead, so
-  // only the upstream-trace loads sit between two barriers.
   CellPrep P;
   bool have = false;
   if (nl > 0 && lane_ok && off[0] + c < off[1]) {
@@ -272,6 +263,48 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
     }
     __syncthreads();
   }
+}
+
+// Direction-resident sweep: one workgroup owns one direction (of one batch of
+// groups) and walks its levels in order.  All producers and consumers of a
+// direction's upstream traces run on one CU, so workgroup-scope ordering is
+// enough: no inter-workgroup flags, no agent-scope cache write-back or
+// invalidation, and the traces of the last few levels are re-read from the
+// XCD's L2.  One launch sweeps every direction.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* __restrict__ lvl_off,
+                                                                const int32_t* __restrict__ nlev, int maxlev,
+                                                                SweepArgs A) {
+  const int i = blockIdx.x;
+  walk_levels<BLOCK>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+}
+
+// Persistent sweep of a batch of independent problems: a fixed grid (a few
+// workgroups per CU) pulls (problem, direction) items from a global counter, in
+// the order of the item table (largest problems first), and walks each item's
+// levels like the direction-resident kernel.  Items of inactive problems
+// (converged in a batched solve) are skipped.  Every workgroup leaves the loop
+// once the counter passes the item count.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sweep_persistent_kernel(const int4* __restrict__ items, int nitems,
+                                                                 const int32_t* __restrict__ lvl,
+                                                                 const int32_t* __restrict__ active,
+                                                                 unsigned int* __restrict__ counter, SweepArgs A) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const int4 d = items[it];
+    if (active[d.x] == 0) continue;
+    walk_levels<BLOCK>(A, d.y, lvl + d.z, d.w);
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned int* counter) {
+  if (threadIdx.x == 0) counter[0] = 0u;
 }
 
 // phi[lm][col] = sum_i D[lm][i] psi[i][col]; one lane per column (column =
@@ -430,29 +463,101 @@ inline unsigned grid_stride(int64_t n) {
 
 }  // namespace
 
-// One sweep issued in a fixed launch configuration (mode, direction block).
-static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, int mode, int block) {
-  if (mode == LB_SWEEP_DIRECTION && (block / G) < 1) {
-    set_error("sweep: the direction-resident block is smaller than the group batch");
+// Records and c_f (k . n_f) of every (direction, sweep position) for a task order.
+static int ensure_records(lb_op* op, DevBuf& rec, DevBuf& ck, const DevBuf& tasks) {
+  if (rec.p) return LB_OK;
+  const int64_t nrec = op->NI * op->NE;
+  LB_TRY(rec.reserve(nrec * sizeof(TaskRec)));
+  LB_TRY(ck.reserve(nrec * sizeof(double4)));
+  hipLaunchKernelGGL(build_records_kernel, dim3(grid_for(nrec)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                     tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
+                     op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
+                     rec.as<TaskRec>(), ck.as<double4>());
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+// Replays the graph captured for this (source, flux, group range, launch
+// configuration), capturing enqueue() first if there is none.  The cache holds 8
+// graphs and is cleared whenever the physics table or the problem set moves.
+template <class F>
+static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg, F enqueue) {
+  hipGraphExec_t exec = nullptr;
+  for (auto& sg : op->graphs)
+    if (sg.q == q && sg.psi == psi && sg.g0 == g0 && sg.G == G && sg.mode == cfg.mode && sg.block == cfg.block &&
+        sg.per_cu == cfg.per_cu)
+      exec = sg.exec;
+  if (!exec) {
+    LB_HIP(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    hipGraph_t graph = nullptr;
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipStreamEndCapture(op->stream, &graph);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      set_error(std::string("sweep graph capture failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+      return LB_E_HIP;
+    }
+    hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) {
+      set_error(std::string("sweep graph instantiation failed: ") + hipGetErrorString(e3));
+      return LB_E_HIP;
+    }
+    if (op->graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(op->graphs.front().exec);
+      op->graphs.erase(op->graphs.begin());
+    }
+    op->graphs.push_back({q, psi, g0, G, cfg.mode, cfg.block, cfg.per_cu, exec});
+  }
+  LB_HIP(hipEventRecord(op->ev0, op->stream));
+  LB_HIP(hipGraphLaunch(exec, op->stream));
+  LB_HIP(hipEventRecord(op->ev1, op->stream));
+  return LB_OK;
+}
+
+// One sweep issued in a fixed launch configuration.
+static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
+  const int mode = cfg.mode, block = cfg.block;
+  if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_PERSISTENT) &&
+      !((block == 256 || block == 512 || block == 1024) && block / G >= 1)) {
+    set_error("sweep: the workgroup size must be 256, 512 or 1024 and hold the group batch");
     return LB_E_ARG;
   }
-  int rc = LB_OK;
-  LevelTable* t = level_table(op, G, &rc);
-  if (!t) return rc;
-  const int4* work = t->d_entries.as<const int4>();
-  if (!op->d_rec.p) {
-    const int64_t nrec = op->NI * op->NE;
-    LB_TRY(op->d_rec.reserve(nrec * sizeof(TaskRec)));
-    LB_TRY(op->d_ck.reserve(nrec * sizeof(double4)));
-    hipLaunchKernelGGL(build_records_kernel, dim3(grid_for(nrec)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                       op->d_tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
-                       op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
-                       op->d_rec.as<TaskRec>(), op->d_ck.as<double4>());
-    LB_HIP(hipGetLastError());
+  if (mode == LB_SWEEP_PERSISTENT) {
+    if (op->batch.P == 0) {
+      set_error("sweep: the persistent sweep needs the problem set (lb_op_set_problems)");
+      return LB_E_ARG;
+    }
+    if (cfg.per_cu < 1 || cfg.per_cu > 8 || block * cfg.per_cu > 2048) {
+      set_error("sweep: persistent workgroups per CU must be 1..8 with at most 2048 threads per CU");
+      return LB_E_ARG;
+    }
+    BatchTable& B = op->batch;
+    LB_TRY(ensure_records(op, B.d_rec, B.d_ck, B.d_tasks));
+    SweepArgs A{B.d_rec.as<const TaskRec>(), B.d_ck.as<const double4>(), op->d_sigt.as<const double>(),
+                op->G, g0, G, op->NE, q, psi};
+    const int4* items = B.d_items.as<const int4>();
+    const int nitems = (int)(B.items.size() / 4);
+    const int32_t* lvl = B.d_lvl.as<const int32_t>();
+    const int32_t* act = B.d_active.as<const int32_t>();
+    unsigned int* ctr = B.d_counter.as<unsigned int>();
+    const dim3 grid((unsigned)(B.ncu * cfg.per_cu));
+    return replay_graph(op, q, psi, g0, G, cfg, [&]() {
+      hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, op->stream, ctr);
+      if (block == 256)
+        hipLaunchKernelGGL(sweep_persistent_kernel<256>, grid, dim3(256), 0, op->stream, items, nitems, lvl, act, ctr, A);
+      else if (block == 512)
+        hipLaunchKernelGGL(sweep_persistent_kernel<512>, grid, dim3(512), 0, op->stream, items, nitems, lvl, act, ctr, A);
+      else
+        hipLaunchKernelGGL(sweep_persistent_kernel<1024>, grid, dim3(1024), 0, op->stream, items, nitems, lvl, act, ctr,
+                           A);
+    });
   }
+
+  LB_TRY(ensure_records(op, op->d_rec, op->d_ck, op->d_tasks));
   SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
               op->G, g0, G, op->NE, q, psi};
-
   if (mode == LB_SWEEP_DIRECTION) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
@@ -475,6 +580,10 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     return LB_OK;
   }
 
+  int rc = LB_OK;
+  LevelTable* t = level_table(op, G, &rc);
+  if (!t) return rc;
+  const int4* work = t->d_entries.as<const int4>();
   auto enqueue = [&]() {
     for (int l = 0; l < op->maxlev; ++l) {
       const int nb = t->level_off[l + 1] - t->level_off[l];
@@ -482,40 +591,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
     }
   };
-  if (mode == LB_SWEEP_GRAPH) {
-    // The level launches of one (source, flux, group range) are captured once into a
-    // graph and replayed; the cache is cleared whenever the physics table moves.
-    hipGraphExec_t exec = nullptr;
-    for (auto& sg : op->graphs)
-      if (sg.q == q && sg.psi == psi && sg.g0 == g0 && sg.G == G) exec = sg.exec;
-    if (!exec) {
-      LB_HIP(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-      enqueue();
-      hipGraph_t graph = nullptr;
-      hipError_t e1 = hipGetLastError();
-      hipError_t e2 = hipStreamEndCapture(op->stream, &graph);
-      if (e1 != hipSuccess || e2 != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        set_error(std::string("sweep graph capture failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
-        return LB_E_HIP;
-      }
-      hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e3 != hipSuccess) {
-        set_error(std::string("sweep graph instantiation failed: ") + hipGetErrorString(e3));
-        return LB_E_HIP;
-      }
-      if (op->graphs.size() >= 8) {
-        (void)hipGraphExecDestroy(op->graphs.front().exec);
-        op->graphs.erase(op->graphs.begin());
-      }
-      op->graphs.push_back({q, psi, g0, G, exec});
-    }
-    LB_HIP(hipEventRecord(op->ev0, op->stream));
-    LB_HIP(hipGraphLaunch(exec, op->stream));
-    LB_HIP(hipEventRecord(op->ev1, op->stream));
-    return LB_OK;
-  }
+  if (mode == LB_SWEEP_GRAPH) return replay_graph(op, q, psi, g0, G, cfg, enqueue);
   LB_HIP(hipEventRecord(op->ev0, op->stream));
   enqueue();
   LB_HIP(hipGetLastError());
@@ -528,26 +604,29 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
     set_error("sweep: the group batch must be between 1 and 256");
     return LB_E_ARG;
   }
-  if (op->sweep_mode != LB_SWEEP_AUTO) return launch_sweep_as(op, q, psi, g0, G, op->sweep_mode, op->dir_block);
-  SweepTune& tn = op->tune[G];
-  if (tn.choice >= 0)
-    return launch_sweep_as(op, q, psi, g0, G, kTuneConfig[tn.choice].mode, kTuneConfig[tn.choice].block);
+  if (op->sweep_mode != LB_SWEEP_AUTO)
+    return launch_sweep_as(op, q, psi, g0, G, SweepConfig{op->sweep_mode, op->dir_block, op->per_cu});
+  const bool batched = op->batch.P > 0;
+  const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
+  const int nconf = batched ? kBatchTuneConfigs : kTuneConfigs;
+  SweepTune& tn = batched ? op->btune[G] : op->tune[G];
+  if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice]);
   // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
   // sweep timed alone between the operator's events; configurations whose block
   // cannot hold the group batch are skipped.  Results are identical in every mode.
-  int c = tn.trials % kTuneConfigs;
-  while (kTuneConfig[c].mode == LB_SWEEP_DIRECTION && kTuneConfig[c].block < G) {
+  int c = tn.trials % nconf;
+  while (table[c].block != 0 && table[c].block < G) {
     ++tn.trials;
-    c = tn.trials % kTuneConfigs;
+    c = tn.trials % nconf;
   }
-  LB_TRY(launch_sweep_as(op, q, psi, g0, G, kTuneConfig[c].mode, kTuneConfig[c].block));
+  LB_TRY(launch_sweep_as(op, q, psi, g0, G, table[c]));
   LB_HIP(hipEventSynchronize(op->ev1));
   float ms = 0.f;
   LB_HIP(hipEventElapsedTime(&ms, op->ev0, op->ev1));
   tn.best_ms[c] = std::min(tn.best_ms[c], ms);
-  if (++tn.trials >= kTuneTrials * kTuneConfigs) {
+  if (++tn.trials >= kTuneTrials * nconf) {
     int best = 0;
-    for (int i = 1; i < kTuneConfigs; ++i)
+    for (int i = 1; i < nconf; ++i)
       if (tn.best_ms[i] < tn.best_ms[best]) best = i;
     tn.choice = best;
   }

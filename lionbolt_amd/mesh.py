@@ -218,3 +218,23 @@ def read_gmsh41(path):
     uniq = sorted(set(mat.tolist()))
     mat = np.searchsorted(np.array(uniq), mat) + 1
     return TetMesh(xyz, node, mat)
+
+
+def union(meshes):
+    """Disjoint union of meshes for a batch of independent problems.
+
+    Nodes and elements are concatenated in order; neighbours are shifted by the
+    element offset; element materials become ``p + 1`` for problem ``p`` (so each
+    problem has its own cross sections).  Returns the mesh and the element
+    offsets ``elem_off`` (length P + 1)."""
+    rg, node, nbr, mat, eoff = [], [], [], [], [0]
+    noff = 0
+    for p, m in enumerate(meshes):
+        rg.append(m.rg)
+        node.append(m.node + noff)
+        nbr.append(np.where(m.nbr >= 0, m.nbr + eoff[-1], -1))
+        mat.append(np.full(m.NE, p + 1, dtype=np.int64))
+        noff += m.NK
+        eoff.append(eoff[-1] + m.NE)
+    return (TetMesh(np.concatenate(rg), np.concatenate(node), np.concatenate(mat), np.concatenate(nbr)),
+            np.array(eoff, dtype=np.int64))

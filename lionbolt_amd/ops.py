@@ -88,18 +88,38 @@ class Operator:
         _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
 
     _SWEEP_MODES = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION,
-                    "auto": _lib.SWEEP_AUTO}
+                    "auto": _lib.SWEEP_AUTO, "persistent": _lib.SWEEP_PERSISTENT}
 
     def set_sweep_mode(self, mode):
         _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, self._SWEEP_MODES[mode]))
 
     def sweep_choice(self, G=1):
-        """(mode name, direction block) the autotuner settled on for a group batch of G,
-        or ("auto", 0) while it is still timing."""
-        mode, threads = ctypes.c_int(), ctypes.c_int()
-        _lib.check(_lib.lib().lb_op_sweep_choice(self.h, int(G), ctypes.byref(mode), ctypes.byref(threads)))
+        """(mode name, workgroup size, persistent workgroups per CU) the autotuner settled
+        on for a group batch of G, or ("auto", 0, 0) while it is still timing."""
+        mode, threads, per_cu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().lb_op_sweep_choice(self.h, int(G), ctypes.byref(mode), ctypes.byref(threads),
+                                                 ctypes.byref(per_cu)))
         name = {v: k for k, v in self._SWEEP_MODES.items()}[mode.value]
-        return name, threads.value
+        return name, threads.value, per_cu.value
+
+    def set_workgroups_per_cu(self, per_cu):
+        _lib.check(_lib.lib().lb_op_set_workgroups_per_cu(self.h, int(per_cu)))
+
+    def set_problems(self, elem_off):
+        """Declare the mesh as independent problems in element ranges [elem_off[p], elem_off[p+1])."""
+        off = np.ascontiguousarray(elem_off, dtype=np.int64)
+        _lib.check(_lib.lib().lb_op_set_problems(self.h, off.size - 1, _lib.i64ptr(off)))
+        self.P = off.size - 1
+
+    def batch_solve_dev(self, qmom, phi, m=30, rtol=1e-8, atol=1e-12, maxit=0, raise_on_fail=True):
+        """Per-problem flux-moment GMRES(m) of the whole batch in lockstep (single group).
+        ``qmom``/``phi``: cuda float64 tensors of nmom * NENK entries."""
+        opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
+        stats = (_lib.SolveStats * self.P)()
+        rc = _lib.lib().lb_batch_solve_dev(self.h, qmom.data_ptr(), ctypes.byref(opts), phi.data_ptr(), stats)
+        if raise_on_fail:
+            _lib.check(rc)
+        return [s.as_dict() for s in stats]
 
     def set_direction_block(self, threads):
         _lib.check(_lib.lib().lb_op_set_direction_block(self.h, int(threads)))

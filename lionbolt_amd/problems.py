@@ -18,6 +18,16 @@ Decisions recorded here (the reference never ran these configurations):
   is sum_l (2l+1)/(4 pi) sigma_l(1 -> g) P_l(k . z) phi_0, i.e. moments
   sigma_l(1 -> g) Y_lm(z) phi_0 (nodal interpolation, where the reference
   projects a ray-traced source with the inverse mass matrix, FCSSource.f90:134-215).
+* cfg4: 64 single-group problems (SplitMix64 stream 1004, six values per
+  problem): tets drawn uniform on [2,000, 60,000] and quantised to the jittered
+  cube 6 n^3 with n = round((N/6)^(1/3)) clipped to [7, 21]; edge length
+  U[5, 20] cm; sigma_t U[0.05, 1.0] /cm; c = sigma_s/sigma_t U[0.5, 0.95]; P0;
+  LQ S8; a broad collimated beam of unit flux along
+  k0 = (sin t cos f, sin t sin f, cos t) with cos t U[0.5, 1.0] and f U[0, 2 pi)
+  entering every face it illuminates; jitter stream 1001 + 7919 (p + 1).  Its
+  uncollided flux is exp(-sigma_t l) with l the distance from the node back along
+  -k0 to the cube surface; the first-collision source is isotropic,
+  sigma_s phi_0 / (4 pi).
 * cfg5: cfg2 with a within-group scattering ratio of 0.99 in every group and no
   downscatter (scattering-dominated, the stated purpose of the configuration).
 """
@@ -93,3 +103,48 @@ def beam_source_moments(m, ang, S, sigt):
         sig = S[0, g][mat_dof][:, deg].T            # (nmom, NENK)
         q[g] = sig * y0[:, None] * phi0[None, :]
     return q, phi0
+
+
+def cube_beam_uncollided(m, a, k0, sigt):
+    """exp(-sigma_t l) at every spatial dof of a homogeneous cube [0, a]^3 lit by a
+    broad parallel beam along k0: l = distance from the node back along -k0 to the
+    cube surface."""
+    x = m.dof_coords()
+    k0 = np.asarray(k0, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        back = np.where(k0 > 0, x / k0, np.where(k0 < 0, (a - x) / -k0, np.inf))
+    ell = np.clip(back.min(axis=1), 0.0, None)
+    return np.exp(-sigt * ell)
+
+
+def cfg4(nprob=64, seed=1004):
+    """The 64 uneven single-group beam problems (see the module docstring)."""
+    u = rng.uniform(seed, 6 * nprob).reshape(nprob, 6)
+    out = []
+    for p in range(nprob):
+        target = 2000.0 + 58000.0 * u[p, 0]
+        n = int(np.clip(round((target / 6.0) ** (1.0 / 3.0)), 7, 21))
+        a = 5.0 + 15.0 * u[p, 1]
+        sigt = 0.05 + 0.95 * u[p, 2]
+        c = 0.5 + 0.45 * u[p, 3]
+        mu = 0.5 + 0.5 * u[p, 4]
+        fi = 2.0 * math.pi * u[p, 5]
+        st = math.sqrt(1.0 - mu * mu)
+        k0 = np.array([st * math.cos(fi), st * math.sin(fi), mu])
+        out.append({"n": n, "a": a, "sigt": sigt, "c": c, "k0": k0,
+                    "mesh": mesh.jittered_cube(n, a=a, jitter=0.2, seed=1001 + 7919 * (p + 1))})
+    return out
+
+
+def cfg4_batch(nprob=64, seed=1004, probs=None):
+    """The cfg4 problems as one batch: union mesh, element offsets, per-problem
+    (material) sigma_t and P0 sigma_s, and the first-collision source moments
+    qmom (1, NENK) of the union (P0: q = Y00 sigma_s phi_0, so M qmom = sigma_s phi_0/(4 pi))."""
+    probs = cfg4(nprob, seed) if probs is None else probs
+    um, eoff = mesh.union([p["mesh"] for p in probs])
+    sigt = np.array([p["sigt"] for p in probs])
+    sigs = np.array([[p["c"] * p["sigt"]] for p in probs])
+    y00 = 1.0 / math.sqrt(4.0 * math.pi)
+    qmom = np.concatenate([y00 * p["c"] * p["sigt"] * cube_beam_uncollided(p["mesh"], p["a"], p["k0"], p["sigt"])
+                           for p in probs])[None, :]
+    return um, eoff, sigt, sigs, qmom, probs

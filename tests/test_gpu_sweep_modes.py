@@ -69,11 +69,11 @@ def test_autotuned_sweeps_equal_level_launches(G):
     op.synchronize()
     ref = ref.cpu().numpy()
     op.set_sweep_mode("auto")
-    assert op.sweep_choice(G) == ("auto", 0)
+    assert op.sweep_choice(G) == ("auto", 0, 0)
     for _ in range(10):
         psi = torch.full_like(q, float("nan"))
         op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         assert np.array_equal(psi.cpu().numpy(), ref)
-    mode, threads = op.sweep_choice(G)
+    mode, threads, _ = op.sweep_choice(G)
     assert (mode, threads) in {("graph", 0), ("direction", 256), ("direction", 512), ("direction", 1024)}
