@@ -16,6 +16,7 @@ thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
 void clear_graphs(lb_op* op) {
+  if (op->stream && !op->graphs.empty()) (void)hipStreamSynchronize(op->stream);  // none may be in flight
   for (auto& sg : op->graphs) (void)hipGraphExecDestroy(sg.exec);
   op->graphs.clear();
 }

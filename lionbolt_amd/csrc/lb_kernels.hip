@@ -505,6 +505,7 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
       return LB_E_HIP;
     }
     if (op->graphs.size() >= 8) {
+      LB_HIP(hipStreamSynchronize(op->stream));   // the evicted graph may still be in flight
       (void)hipGraphExecDestroy(op->graphs.front().exec);
       op->graphs.erase(op->graphs.begin());
     }
