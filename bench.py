@@ -21,6 +21,10 @@ The line also carries:
 * reference_method_gpu: the same problem with the reference's own method
   (psi-space GMRES(10) per group), for its iteration counts;
 * cfg1_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
+* learned_pc_cfg5: the 24-group problem at within-group c = 0.99 solved with
+  plain flux-moment GMRES(30) and with flexible GMRES(30) preconditioned by the
+  per-cell-node network (trained first on the solver's residual/error pairs
+  from 32 smaller problems; training time reported separately);
 * batch_cfg4: 64 uneven single-group S8 beam problems (2k-56k tets) swept as
   one batch (persistent work-queue kernel, autotuned) and solved with
   per-problem GMRES in lockstep;
@@ -307,6 +311,46 @@ def batch_leg(reps=10, rtol=1e-6):
     return res
 
 
+def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
+    """cfg5: the 24-group P3 problem with within-group c = 0.99, solved group by
+    group with plain flux-moment GMRES(30) and with flexible GMRES(30) right-
+    preconditioned by the per-cell-node network, trained first on the solver's own
+    residual/error pairs from 32 smaller seeded problems."""
+    import torch
+    from lionbolt_amd import learned_pc, problems
+    from lionbolt_amd.ops import Operator
+    X, Y, data = learned_pc.build_training_set()
+    params, tinfo = learned_pc.train(X, Y)
+    del X, Y
+    G = 24
+    sigt, S = problems.cfg5_xs(G=G, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, problems.within_group(S))
+    op.set_preconditioner(params, fmt)
+    qmom = _beam_qmom(m, ang, sigt, S)
+    res = {"training": {"rows": data["rows"], "solves": len(data["solves"]),
+                        "data_seconds": data["seconds"], **tinfo}}
+    for kind in ("gmres", "fgmres"):
+        phi = torch.zeros_like(qmom)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = op.multigroup_solve_dev(S, phi, qmom=qmom, kind=kind, space="moment", m=30, rtol=rtol)
+        torch.cuda.synchronize()
+        res[kind] = {"seconds": time.perf_counter() - t0, "iterates": [s["iterates"] for s in st],
+                     "total_iterates": sum(s["iterates"] for s in st), "sweeps": sum(s["sweeps"] for s in st),
+                     "max_true_residual": max(s["true_residual"] for s in st)}
+        if kind == "fgmres":
+            res[kind]["fallbacks"] = [s["fallback"] for s in st]
+            res[kind]["preconditioner"] = fmt
+        if kind == "gmres":
+            ref = phi.clone()
+        else:
+            res["flux_rel_diff"] = float((phi[:, 0] - ref[:, 0]).abs().max() / ref[:, 0].abs().max())
+    res["speedup_fgmres_vs_gmres"] = res["gmres"]["seconds"] / res["fgmres"]["seconds"]
+    op.close()
+    return res
+
+
 def single_group_leg(m, ang, reps=10):
     """One-group sweeps of the same mesh and directions (what a within-group solve
     issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s."""
@@ -465,6 +509,8 @@ def run(args):
         out["cfg1_gpu"] = cfg1_leg()
     if args.batch:
         out["batch_cfg4"] = batch_leg()
+    if args.learned_pc:
+        out["learned_pc_cfg5"] = learned_pc_leg(m, ang)
     if args.gemm_study:
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
@@ -497,6 +543,8 @@ def main():
                    help="skip the reference-method (psi-space GMRES(10)) solve on the GPU")
     p.add_argument("--no-cfg1", dest="cfg1", action="store_false")
     p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-problem batch (cfg4)")
+    p.add_argument("--no-learned-pc", dest="learned_pc", action="store_false",
+                   help="skip the learned-preconditioner solve (cfg5)")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction"])
     p.add_argument("--compare-modes", action="store_true")

@@ -70,7 +70,8 @@ typedef struct {
 } lb_quad_desc;
 
 typedef struct {
-  int kind;              /* LB_SOLVER_GMRES (reference default), LB_SOLVER_SI */
+  int kind;              /* LB_SOLVER_GMRES (reference default), LB_SOLVER_SI, LB_SOLVER_FGMRES
+                            (flux moments, learned right preconditioner, lb_op_set_preconditioner) */
   int space;             /* LB_SPACE_PSI: Krylov on psi as the reference; LB_SPACE_MOMENT: on flux moments */
   int restart;           /* GMRES(m); the reference hard-codes 10 (BoltzmannOperator.f90:135) */
   int maxit;             /* <= 0: no limit (SPARSKIT.f90:89) */
@@ -86,6 +87,8 @@ typedef struct {
   double final_norm;     /* last printed residual norm (Arnoldi estimate / SI increment) */
   double true_residual;  /* ||T^-1 q - (I - T^-1 K) psi|| / ||T^-1 q||, recomputed in FP64 */
   double seconds;        /* wall time of the solve */
+  int fallback;          /* LB_SOLVER_FGMRES: Arnoldi step after which the learned preconditioner
+                            was dropped for stagnation (0: never) */
 } lb_solve_stats;
 
 /* Op%Build(mesh) + Op%Build(mesh, angular): normals, geometry and the upwind
@@ -138,6 +141,55 @@ int lb_op_set_workgroups_per_cu(lb_op* op, int per_cu);
  * HIP graph; LB_SWEEP_AUTO then times it (several workgroup sizes and grid
  * occupancies) against the level launches. */
 int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off);
+
+/* ---- learned right preconditioner (no reference counterpart) -------------
+ * A per-cell-node MLP 18 -> 64 -> 64 -> 64 -> 16 (tanh hidden layers): inputs the
+ * 16 flux moments of a residual at one spatial dof (divided by the residual's
+ * root-mean-square s), log(sigma_t h) and sigma_s0/sigma_t of the dof's element
+ * (h = (6 V)^(1/3)); outputs the error's 16 moments there (times s).  Used as
+ * the right preconditioner z = P(v) of flexible GMRES(m) on flux moments
+ * (LB_SOLVER_FGMRES); when the residual estimate falls by less than half over
+ * 10 Arnoldi steps, or P returns a non-finite vector, the solve continues as
+ * plain GMRES.  Parameters: W1[18][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+ * W4[64][16] b4[16], FP32, W[i][j] from input i to output j. */
+#define LB_MLP_PARAMS 10576
+
+/* Sets the network; fmt LB_FMT_BF16 (MFMA inference) or LB_FMT_FP32. */
+int lb_op_set_preconditioner(lb_op* op, const float* params, int nparams, int fmt);
+
+/* d_z [16][NENK] = P(d_r [16][NENK]) for group g0 (single-group moment vectors). */
+int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0);
+
+/* The network's inputs for d_r: d_feat [NENK][18] (FP32) and the scale s. */
+int lb_mlp_features_dev(lb_op* op, const double* d_r, int g0, float* d_feat, double* scale);
+
+/* Trains the network in FP32 by minibatch Adam (batch <= 4096 rows drawn with
+ * SplitMix64 stream `seed`) on device rows d_X [N][18] -> d_Y [N][16] (mean
+ * squared error); params [LB_MLP_PARAMS] in (initial) and out (trained) on the
+ * host; loss_hist [steps] (host, may be NULL). */
+int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* params, int nparams, int steps,
+                     int batch, float lr, uint64_t seed, float* loss_hist);
+
+/* Gradient of the mean squared error over rows 0..B-1 (B <= 4096) at params (host
+ * in), grad [LB_MLP_PARAMS] (host out), without an update. */
+int lb_mlp_grad_dev(const float* d_X, const float* d_Y, int B, const float* params, int nparams, float* grad,
+                    float* loss);
+
+/* Within-group flux-moment system (I - D T^-1 M S) phi = D T^-1 q of group g0
+ * (single-group moment vectors [nmom][NENK]): d_b = D T^-1 d_q (d_q: discrete
+ * source, NENK*NI), and d_y = d_x - D T^-1 M S d_x. */
+int lb_moment_rhs_dev(lb_op* op, const double* d_q, double* d_b, int g0);
+/* Discrete source of single-group source moments: d_q [NI][NENK] = M d_qmom (no
+ * cross sections applied). */
+int lb_discrete_source_dev(lb_op* op, const double* d_qmom, double* d_q);
+int lb_moment_matvec_dev(lb_op* op, const double* d_x, double* d_y, int g0);
+
+/* Flux-moment GMRES(m) of group g0 from x0 = b / sigma_t(1) that records the
+ * iterate x_k after every Arnoldi step (d_snaps [max_snaps][nmom*NENK]) and
+ * returns the solution in d_phi: residual/error pairs for training the
+ * preconditioner come from the solver itself. */
+int lb_gmres_snapshots_dev(lb_op* op, const double* d_q, const lb_solver_opts* opts, double* d_phi,
+                           double* d_snaps, int max_snaps, int* nsnaps, lb_solve_stats* stats);
 
 /* Solves every problem of the batch, single group: flux-moment GMRES(m) per
  * problem, all problems in lockstep with one sweep launch per Arnoldi step

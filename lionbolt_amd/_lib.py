@@ -21,6 +21,9 @@ _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)
 _i32 = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
+_f = ctypes.POINTER(ctypes.c_float)
+FMT_FP32, FMT_BF16, FMT_SPLIT_BF16, FMT_FP16 = 0, 1, 2, 3
+MLP_PARAMS = 10576
 
 
 class MeshDesc(ctypes.Structure):
@@ -40,7 +43,7 @@ class SolverOpts(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterates", ctypes.c_int), ("matvecs", ctypes.c_int),
                 ("sweeps", ctypes.c_int), ("final_norm", ctypes.c_double), ("true_residual", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("fallback", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -58,6 +61,17 @@ SIGNATURES = {
     "lb_op_set_workgroups_per_cu": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_op_set_problems": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
     "lb_batch_solve_dev": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(SolverOpts), _vp, ctypes.POINTER(SolveStats)]),
+    "lb_op_set_preconditioner": (ctypes.c_int, [_vp, _f, ctypes.c_int, ctypes.c_int]),
+    "lb_mlp_apply_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_mlp_features_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_double)]),
+    "lb_mlp_train_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_uint64, _f]),
+    "lb_mlp_grad_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _f, ctypes.c_int, _f, _f]),
+    "lb_moment_rhs_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_moment_matvec_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_discrete_source_dev": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lb_gmres_snapshots_dev": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(SolverOpts), _vp, _vp, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(SolveStats)]),
     "lb_op_set_direction_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_transport_matinv": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),
     "lb_scatter_matvec": (ctypes.c_int, [_vp, _d, ctypes.c_int64]),

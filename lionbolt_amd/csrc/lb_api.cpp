@@ -256,6 +256,70 @@ int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off) {
   return set_problems(op, P, elem_off);
 }
 
+int lb_op_set_preconditioner(lb_op* op, const float* params, int nparams, int fmt) {
+  if (!op || op->host_only) {
+    set_error("lb_op_set_preconditioner: null operator");
+    return LB_E_ARG;
+  }
+  return mlp_set(op, params, nparams, fmt);
+}
+
+int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0) {
+  LB_TRY(check_ready(op, "lb_mlp_apply_dev"));
+  if (!d_r || !d_z || g0 < 0 || g0 >= op->G) {
+    set_error("lb_mlp_apply_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  return mlp_apply(op, d_r, d_z, g0);
+}
+
+int lb_mlp_features_dev(lb_op* op, const double* d_r, int g0, float* d_feat, double* scale) {
+  LB_TRY(check_ready(op, "lb_mlp_features_dev"));
+  if (!d_r || !d_feat || g0 < 0 || g0 >= op->G) {
+    set_error("lb_mlp_features_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  return mlp_features(op, d_r, g0, d_feat, scale);
+}
+
+int lb_moment_rhs_dev(lb_op* op, const double* d_q, double* d_b, int g0) {
+  LB_TRY(check_ready(op, "lb_moment_rhs_dev"));
+  if (!d_q || !d_b || g0 < 0 || g0 >= op->G) {
+    set_error("lb_moment_rhs_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  return moment_rhs_dev(op, d_q, d_b, g0);
+}
+
+int lb_discrete_source_dev(lb_op* op, const double* d_qmom, double* d_q) {
+  LB_TRY(check_ready(op, "lb_discrete_source_dev"));
+  if (!d_qmom || !d_q) {
+    set_error("lb_discrete_source_dev: null vector");
+    return LB_E_ARG;
+  }
+  return launch_moments_to_discrete(op, d_qmom, d_q, 0, 1, 0, false);
+}
+
+int lb_moment_matvec_dev(lb_op* op, const double* d_x, double* d_y, int g0) {
+  LB_TRY(check_ready(op, "lb_moment_matvec_dev"));
+  if (!d_x || !d_y || g0 < 0 || g0 >= op->G) {
+    set_error("lb_moment_matvec_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  if (d_x != d_y) LB_TRY(launch_copy(op->stream, op->nmom * op->NENK, d_x, d_y));
+  return moment_matvec_dev(op, d_y, g0);
+}
+
+int lb_gmres_snapshots_dev(lb_op* op, const double* d_q, const lb_solver_opts* opts, double* d_phi,
+                           double* d_snaps, int max_snaps, int* nsnaps, lb_solve_stats* stats) {
+  LB_TRY(check_ready(op, "lb_gmres_snapshots_dev"));
+  if (!d_q || !d_phi || (max_snaps > 0 && !d_snaps) || max_snaps < 0 || op->G != 1) {
+    set_error("lb_gmres_snapshots_dev: invalid argument (single-group operators only)");
+    return LB_E_ARG;
+  }
+  return gmres_snapshots_dev(op, d_q, opts, d_phi, d_snaps, max_snaps, nsnaps, stats);
+}
+
 int lb_batch_solve_dev(lb_op* op, const double* d_qmom, const lb_solver_opts* opts, double* d_phi,
                        lb_solve_stats* stats) {
   LB_TRY(check_ready(op, "lb_batch_solve_dev"));
@@ -441,6 +505,7 @@ void lb_op_destroy(lb_op* op) {
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
     b->release();
+  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat}) b->release();
   for (auto& kv : op->batch.seg) {
     kv.second.d_tiles.release();
     kv.second.d_tile_off.release();

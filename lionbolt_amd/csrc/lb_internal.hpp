@@ -151,6 +151,14 @@ struct lb_op {
   lb::BatchTable batch;             // independent problems (set by lb_op_set_problems)
   std::map<int, lb::SweepTune> btune;  // autotuner state of batched sweeps, per group-batch size
   std::vector<int32_t> h_nbr;       // int32 [NE][4] neighbours (-1 boundary)
+  std::vector<double> h_detJ;       // [NE] 6 x volume
+  // learned right preconditioner (per cell-node MLP), see lb_mlp.hip
+  bool mlp = false;
+  int mlp_fmt = 1;                  // LB_FMT_BF16 (MFMA) or LB_FMT_FP32
+  lb::DevBuf d_mlp_w;               // FP32 parameters, natural layout
+  lb::DevBuf d_mlp_pack;            // BF16 weights in MFMA lane order
+  lb::DevBuf d_elem_h;              // [NE] element size (6 V)^(1/3)
+  lb::DevBuf w_feat;                // training/feature scratch
   std::vector<lb::SweepGraph> graphs;
   // runtime
   hipStream_t stream = nullptr;
@@ -184,6 +192,11 @@ int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int launch_fluence(lb_op* op, const double* v, double* phi);
 
+// lb_mlp.hip
+int mlp_set(lb_op* op, const float* params, int nparams, int fmt);
+int mlp_apply(lb_op* op, const double* r, double* z, int g0);
+int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale);
+
 // lb_batch.hip
 int set_problems(lb_op* op, int P, const int64_t* eoff);
 SegTable* seg_table(lb_op* op, int rows, int* rc);
@@ -198,5 +211,9 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* opt
 int scatter_dev(lb_op* op, const double* x, double* y, int g0, int G);
 int multigroup_solve_dev(lb_op* op, const double* qmom, const double* transfer, const lb_solver_opts* opts,
                         double* phi_out, lb_solve_stats* stats);
+int moment_matvec_dev(lb_op* op, double* v, int g0);
+int moment_rhs_dev(lb_op* op, const double* q, double* b, int g0);
+int gmres_snapshots_dev(lb_op* op, const double* q, const lb_solver_opts* o, double* phi, double* snaps,
+                        int max_snaps, int* nsnaps, lb_solve_stats* st);
 
 }  // namespace lb

@@ -210,7 +210,233 @@ int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* s
 
 int sweep_inplace(lb_op* op, double* v, int g0) { return launch_sweep(op, v, v, g0, 1); }
 
+// Givens rotation of Golub & van Loan as SPARSKIT's givens (iters.f:3225-3260):
+// (x, y) -> (c, s) with c x + s y = r, -s x + c y = 0.
+void givens(double x, double y, double* c, double* s) {
+  if (x == 0.0 && y == 0.0) {
+    *c = 1.0;
+    *s = 0.0;
+  } else if (std::fabs(y) > std::fabs(x)) {
+    const double t = x / y;
+    *s = std::copysign(1.0 / std::sqrt(1.0 + t * t), y);
+    *c = t * *s;
+  } else {
+    const double t = y / x;
+    *c = std::copysign(1.0 / std::sqrt(1.0 + t * t), x);
+    *s = t * *c;
+  }
+}
+
+using Precond = std::function<int(const double*, double*)>;
+using Snapshot = std::function<int(const double* sol, const double* Z, int k, const std::vector<double>& y)>;
+
+struct FgmresResult {
+  int status = 0, iterates = 0, matvecs = 0, fallback = 0;
+  double final_norm = 0.0;
+};
+
+// Flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14 (1993) 461-469): right
+// preconditioner z_k = P(v_k), which may change from step to step, and
+// x = x0 + Z y.  Modified Gram-Schmidt with mgsro's selective
+// re-orthogonalisation (0.98 test), Givens rotations, stopping rule
+// ||r|| <= rtol ||rhs|| + atol and an explicit residual at every restart, as
+// gmres() above.  Fallback: when the residual estimate has fallen by less than
+// half over the last 10 Arnoldi steps, or P returns a non-finite vector, P is
+// replaced by the identity (plain GMRES) for the rest of the solve.
+int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
+           double* Z, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
+           FgmresResult* res) {
+  hipStream_t s = op->stream;
+  auto vec = [&](int j) { return V + (int64_t)j * n; };
+  auto zvec = [&](int j) { return Z + (int64_t)j * n; };
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hist;
+  auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
+  constexpr int kWindow = 10;
+  constexpr double kDrop = 0.5;
+  double r2 = 0.0;
+  LB_TRY(dot(op, n, rhs, rhs, &r2));
+  const double eps = rtol * std::sqrt(r2) + atol;
+  bool use_pc = P != nullptr;
+  int iters = 0, nmv = 0, status = LB_E_MAXIT, fallback = 0;
+  double resn = 0.0;
+  auto solve_y = [&](int k, std::vector<double>& y) {
+    y.assign(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double a = g[i];
+      for (int j = i + 1; j < k; ++j) a -= h(i, j) * y[j];
+      y[i] = h(i, i) != 0.0 ? a / h(i, i) : 0.0;
+    }
+  };
+  bool done = false;
+  while (!done) {
+    double* v0 = vec(0);
+    LB_TRY(launch_copy(s, n, sol, v0));
+    LB_TRY(apply(v0));
+    ++nmv;
+    LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    double b2 = 0.0;
+    LB_TRY(dot(op, n, v0, v0, &b2));
+    const double beta = std::sqrt(b2);
+    resn = beta;
+    if (std::isnan(beta)) {
+      status = LB_E_NAN;
+      break;
+    }
+    if (beta <= eps) {
+      status = 0;
+      break;
+    }
+    if (maxit > 0 && iters >= maxit) break;
+    LB_TRY(launch_scale(s, n, 1.0 / beta, v0));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    while (k < m) {
+      double* zk = zvec(k);
+      if (use_pc) {
+        LB_TRY((*P)(vec(k), zk));
+        double zz = 0.0;
+        LB_TRY(dot(op, n, zk, zk, &zz));
+        if (!std::isfinite(zz) || zz == 0.0) {
+          use_pc = false;
+          fallback = iters > 0 ? iters : 1;
+        }
+      }
+      if (!use_pc) LB_TRY(launch_copy(s, n, vec(k), zk));
+      double* w = vec(k + 1);
+      LB_TRY(launch_copy(s, n, zk, w));
+      LB_TRY(apply(w));
+      ++nmv;
+      double nrm0 = 0.0;
+      LB_TRY(dot(op, n, w, w, &nrm0));
+      double thr = nrm0 * 0.98;
+      for (int i = 0; i <= k; ++i) {
+        double f = 0.0;
+        LB_TRY(dot(op, n, w, vec(i), &f));
+        h(i, k) = f;
+        LB_TRY(launch_axpby(s, n, -f, vec(i), 1.0, w, w));
+        if (f * f > thr) {
+          LB_TRY(dot(op, n, w, vec(i), &f));
+          h(i, k) += f;
+          LB_TRY(launch_axpby(s, n, -f, vec(i), 1.0, w, w));
+        }
+        nrm0 = std::max(0.0, nrm0 - h(i, k) * h(i, k));
+        thr = nrm0 * 0.98;
+      }
+      double w2 = 0.0;
+      LB_TRY(dot(op, n, w, w, &w2));
+      const double hn = std::sqrt(w2);
+      h(k + 1, k) = hn;
+      if (hn > 0.0) LB_TRY(launch_scale(s, n, 1.0 / hn, w));
+      for (int i = 0; i < k; ++i) {
+        const double a = h(i, k), b = h(i + 1, k);
+        h(i, k) = c[i] * a + sn[i] * b;
+        h(i + 1, k) = c[i] * b - sn[i] * a;
+      }
+      givens(h(k, k), h(k + 1, k), &c[k], &sn[k]);
+      h(k, k) = c[k] * h(k, k) + sn[k] * h(k + 1, k);
+      h(k + 1, k) = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = c[k] * g[k];
+      resn = std::fabs(g[k + 1]);
+      ++k;
+      ++iters;
+      hist.push_back(resn);
+      print_row(verbose, iters, resn);
+      if (snap) {
+        std::vector<double> y;
+        solve_y(k, y);
+        LB_TRY((*snap)(sol, Z, k, y));
+      }
+      if (std::isnan(resn)) {
+        status = LB_E_NAN;
+        done = true;
+        break;
+      }
+      if (use_pc && (int)hist.size() > kWindow && resn > kDrop * hist[hist.size() - 1 - kWindow]) {
+        use_pc = false;
+        fallback = iters;
+      }
+      if (resn <= eps || !(hn > 0.0) || (maxit > 0 && iters >= maxit)) break;
+    }
+    if (status == LB_E_NAN) break;
+    std::vector<double> y;
+    solve_y(k, y);
+    for (int i = 0; i < k; ++i) LB_TRY(launch_axpby(s, n, y[i], zvec(i), 1.0, sol, sol));
+  }
+  res->status = status;
+  res->iterates = iters;
+  res->matvecs = nmv;
+  res->fallback = fallback;
+  res->final_norm = resn;
+  return LB_OK;
+}
+
 }  // namespace
+
+// v <- v - D T^-1 M S v: the within-group operator on flux moments of group g0.
+int moment_matvec_dev(lb_op* op, double* v, int g0) {
+  const int64_t n = op->NENK * op->NI, nphi = op->NENK * op->nmom;
+  LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+  LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
+  double* psi_t = op->w_a2.as<double>();
+  double* tphi = op->w_phi.as<double>();
+  LB_TRY(launch_moments_to_discrete(op, v, psi_t, g0, 1, 0));
+  LB_TRY(sweep_inplace(op, psi_t, g0));
+  LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+  return launch_axpby(op->stream, nphi, 1.0, v, -1.0, tphi, v);
+}
+
+// b = D T^-1 q for a discrete source q of group g0.
+int moment_rhs_dev(lb_op* op, const double* q, double* b, int g0) {
+  const int64_t n = op->NENK * op->NI;
+  LB_TRY(op->w_c.reserve(n * sizeof(double)));
+  double* t = op->w_c.as<double>();
+  LB_TRY(launch_copy(op->stream, n, q, t));
+  LB_TRY(sweep_inplace(op, t, g0));
+  return launch_moments(op, t, b, 1, op->NENK);
+}
+
+// Flux-moment GMRES(m) that records every iterate (training data for the
+// learned preconditioner).
+int gmres_snapshots_dev(lb_op* op, const double* q, const lb_solver_opts* o, double* phi, double* snaps,
+                        int max_snaps, int* nsnaps, lb_solve_stats* st) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int64_t nphi = op->NENK * op->nmom;
+  lb_solver_opts opt = o ? *o : lb_solver_opts{LB_SOLVER_GMRES, LB_SPACE_MOMENT, 30, 0, 1e-8, 1e-12, 0};
+  const int m = opt.restart > 0 ? opt.restart : 30;
+  LB_TRY(op->w_krylov.reserve((size_t)(2 * m + 2) * nphi * sizeof(double)));
+  double* V = op->w_krylov.as<double>();
+  double* Z = V + (int64_t)(m + 1) * nphi;
+  double* rhs = Z + (int64_t)m * nphi;
+  LB_TRY(moment_rhs_dev(op, q, rhs, 0));
+  LB_TRY(launch_copy(op->stream, nphi, rhs, phi));
+  LB_TRY(launch_scale(op->stream, nphi, 1.0 / op->h_sigt[0], phi));
+  int count = 0;
+  Snapshot snap = [&](const double* sol, const double* Zb, int k, const std::vector<double>& y) -> int {
+    if (count >= max_snaps) return LB_OK;
+    double* x = snaps + (int64_t)count * nphi;
+    LB_TRY(launch_copy(op->stream, nphi, sol, x));
+    for (int i = 0; i < k; ++i) LB_TRY(launch_axpby(op->stream, nphi, y[i], Zb + (int64_t)i * nphi, 1.0, x, x));
+    ++count;
+    return LB_OK;
+  };
+  Apply A = [&](double* v) { return moment_matvec_dev(op, v, 0); };
+  FgmresResult r;
+  LB_TRY(fgmres(op, nphi, A, nullptr, rhs, phi, V, Z, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &snap, &r));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  if (nsnaps) *nsnaps = count;
+  if (st) {
+    *st = lb_solve_stats{};
+    st->status = r.status;
+    st->iterates = r.iterates;
+    st->matvecs = r.matvecs;
+    st->sweeps = r.matvecs + 1;
+    st->final_norm = r.final_norm;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return r.status == 0 ? LB_OK : r.status;
+}
 
 int scatter_dev(lb_op* op, const double* x, double* y, int g0, int G) {
   if (op->scatter_mode == LB_SCATTER_DENSE) {
@@ -245,6 +471,10 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
   const int64_t nphi = op->NENK * op->nmom;
   lb_solver_opts opt = o ? *o : lb_solver_opts{LB_SOLVER_GMRES, LB_SPACE_PSI, 10, 0, 1e-8, 1e-12, 0};
   if (opt.restart < 1) opt.restart = 10;
+  if (opt.kind != LB_SOLVER_GMRES && opt.kind != LB_SOLVER_SI && opt.kind != LB_SOLVER_FGMRES) {
+    set_error("BoltzmannOp%MatInv: unknown solver kind");
+    return LB_E_ARG;
+  }
   lb_solve_stats stats{};
   int sweeps = 0;
   hipStream_t str = op->stream;
@@ -296,7 +526,7 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     }
     stats.iterates = iter;
     stats.matvecs = iter - 1;
-  } else if (opt.space == LB_SPACE_PSI) {
+  } else if (opt.space == LB_SPACE_PSI && opt.kind != LB_SOLVER_FGMRES) {
     const int m = opt.restart;
     LB_TRY(op->w_krylov.reserve((size_t)(m + 2) * n * sizeof(double)));
     double* V = op->w_krylov.as<double>();
@@ -314,6 +544,39 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     stats.iterates = r.iterates;
     stats.matvecs = r.matvecs;
     stats.final_norm = r.final_norm;
+  } else if (opt.kind == LB_SOLVER_FGMRES) {
+    // Flux-moment flexible GMRES with the learned right preconditioner (see the
+    // moment-space branch below for the system).
+    if (!op->mlp) {
+      set_error("FGMRES : no preconditioner set (lb_op_set_preconditioner)");
+      return LB_E_NOT_READY;
+    }
+    const int m = opt.restart;
+    LB_TRY(op->w_krylov.reserve((size_t)(2 * m + 3) * nphi * sizeof(double)));
+    LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+    double* V = op->w_krylov.as<double>();
+    double* Z = V + (int64_t)(m + 1) * nphi;
+    double* phi = Z + (int64_t)m * nphi;
+    double* rhs = phi + nphi;
+    LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
+    LB_TRY(launch_copy(str, nphi, rhs, phi));
+    LB_TRY(launch_scale(str, nphi, 1.0 / sig1, phi));
+    Apply A = [&](double* v) -> int {
+      ++sweeps;
+      return moment_matvec_dev(op, v, g0);
+    };
+    Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g0); };
+    FgmresResult r;
+    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r));
+    LB_TRY(launch_copy(str, n, q, s));
+    LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
+    LB_TRY(sweep_inplace(op, s, g0));
+    ++sweeps;
+    stats.status = r.status;
+    stats.iterates = r.iterates;
+    stats.matvecs = r.matvecs;
+    stats.final_norm = r.final_norm;
+    stats.fallback = r.fallback;
   } else {
     // Flux-moment GMRES: with phi = D psi the within-group system
     // (I - T^-1 M S D) psi = T^-1 q is equivalent to (I - D T^-1 M S) phi = D T^-1 q,

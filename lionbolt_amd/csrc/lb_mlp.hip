@@ -1,0 +1,583 @@
+// Learned right preconditioner: a per-cell-node MLP 18 -> 64 -> 64 -> 64 -> 16
+// (tanh on the hidden layers) that maps a residual's 16 flux moments at one
+// spatial dof, log(sigma_t h) and sigma_s0 / sigma_t of the dof's element, to an
+// estimate of the error's 16 moments there.  The residual is normalised by its
+// root-mean-square s before the network and the output is scaled back by s, so
+// P(a r) = a P(r) for a > 0.  No reference counterpart: the reference has plain
+// GMRES and source iteration only.
+//
+// Parameters (FP32, 10,576): W1 [18][64], b1 [64], W2 [64][64], b2, W3 [64][64], b3,
+// W4 [64][16], b4 [16]; W[i][j] connects input i to output j.
+//
+// Inference runs on MFMA (v_mfma_f32_16x16x32_bf16): a wave takes 16 dofs, the
+// weights are the A operand (rows = output neurons), the activations the B
+// operand (columns = dofs).  The result of a layer leaves lane l holding neurons
+// 16 t + 4 (l / 16) + r (tiles t = 0..3, r = 0..3) of dof l % 16; since A and B
+// only need to agree on which input neuron each K slot carries, those registers
+// are fed to the next layer as they are and the weights are packed in the same
+// slot order (host side, once).  An FP32 path (one thread per dof, weights in
+// LDS) serves as the precision reference.
+//
+// Training (FP32): minibatch Adam over (feature, target) rows with small GEMM
+// kernels (one thread per output element, sums over the batch in a fixed order,
+// so gradients are deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "lb_internal.hpp"
+
+namespace lb {
+
+namespace {
+
+constexpr int kIn = 18, kH = 64, kOut = 16;
+constexpr int kW1 = 0, kB1 = kW1 + kIn * kH, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
+              kB3 = kW3 + kH * kH, kW4 = kB3 + kH, kB4 = kW4 + kH * kOut, kParams = kB4 + kOut;
+static_assert(kParams == LB_MLP_PARAMS, "parameter count");
+// packed BF16 weights, [tile][step][lane][8]
+constexpr int kP1 = 0, kP2 = kP1 + 4 * 1 * 64 * 8, kP3 = kP2 + 4 * 2 * 64 * 8, kP4 = kP3 + 4 * 2 * 64 * 8,
+              kPackLen = kP4 + 1 * 2 * 64 * 8;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct FeatArgs {
+  const double* r;         // [16][NENK] residual moments (single group)
+  int64_t NENK;
+  double inv_s;            // 1 / rms(r)
+  const int32_t* mat;      // [NE]
+  const double* sigt;      // [Nm][Gtot]
+  const double* sigs;      // [Nm][Gtot][L+1]
+  int Gtot, g0, L;
+  const double* elem_h;    // [NE]
+};
+
+__device__ __forceinline__ float feature(const FeatArgs& F, int64_t sd, int f) {
+  if (f < 16) return (float)(F.r[(int64_t)f * F.NENK + sd] * F.inv_s);
+  const int64_t e = sd >> 2;
+  const int m = F.mat[e];
+  const double st = F.sigt[(int64_t)m * F.Gtot + F.g0];
+  if (f == 16) return (float)log(st * F.elem_h[e]);
+  if (f == 17) return (float)(F.sigs[((int64_t)m * F.Gtot + F.g0) * (F.L + 1)] / st);
+  return 0.f;
+}
+
+__global__ __launch_bounds__(256) void features_kernel(FeatArgs F, float* __restrict__ out) {
+  const int64_t sd = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (sd >= F.NENK) return;
+#pragma unroll
+  for (int f = 0; f < kIn; ++f) out[sd * kIn + f] = feature(F, sd, f);
+}
+
+__device__ __forceinline__ float act(float x) { return tanhf(x); }
+
+// FP32 reference inference: one thread per dof.
+__global__ __launch_bounds__(256) void mlp_fp32_kernel(FeatArgs F, const float* __restrict__ W, double s,
+                                                       double* __restrict__ z) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += 256) w[i] = W[i];
+  __syncthreads();
+  const int64_t sd = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (sd >= F.NENK) return;
+  float x[kIn], h[kH], g[kH];
+#pragma unroll
+  for (int f = 0; f < kIn; ++f) x[f] = feature(F, sd, f);
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kB1 + j];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) a += x[i] * w[kW1 + i * kH + j];
+    h[j] = act(a);
+  }
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kB2 + j];
+    for (int i = 0; i < kH; ++i) a += h[i] * w[kW2 + i * kH + j];
+    g[j] = act(a);
+  }
+  for (int j = 0; j < kH; ++j) {
+    float a = w[kB3 + j];
+    for (int i = 0; i < kH; ++i) a += g[i] * w[kW3 + i * kH + j];
+    h[j] = act(a);
+  }
+  for (int j = 0; j < kOut; ++j) {
+    float a = w[kB4 + j];
+    for (int i = 0; i < kH; ++i) a += h[i] * w[kW4 + i * kOut + j];
+    z[(int64_t)j * F.NENK + sd] = s * (double)a;
+  }
+}
+
+// BF16 MFMA inference: each wave loops over tiles of 16 dofs.
+__global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16* __restrict__ P,
+                                                       const float* __restrict__ W, double s,
+                                                       double* __restrict__ z) {
+  const int lane = threadIdx.x & 63;
+  const int kb = lane >> 4, col = lane & 15;
+  const int wave = threadIdx.x >> 6;
+  const bf16x8* pk = reinterpret_cast<const bf16x8*>(P);
+  bf16x8 w1[4], w2[4][2], w3[4][2], w4[2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    w1[t] = pk[(kP1 >> 3) + t * 64 + lane];
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      w2[t][st] = pk[(kP2 >> 3) + (t * 2 + st) * 64 + lane];
+      w3[t][st] = pk[(kP3 >> 3) + (t * 2 + st) * 64 + lane];
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < 2; ++st) w4[st] = pk[(kP4 >> 3) + st * 64 + lane];
+  float b1[4][4], b2[4][4], b3[4][4], b4[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      b1[t][r] = W[kB1 + 16 * t + 4 * kb + r];
+      b2[t][r] = W[kB2 + 16 * t + 4 * kb + r];
+      b3[t][r] = W[kB3 + 16 * t + 4 * kb + r];
+    }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) b4[r] = W[kB4 + 4 * kb + r];
+
+  const int64_t ntiles = (F.NENK + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t sd = tile * 16 + col;
+    const bool ok = sd < F.NENK;
+    bf16x8 xb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f = 8 * kb + q;
+      xb[q] = (__bf16)((ok && f < kIn) ? feature(F, sd, f) : 0.f);
+    }
+    f32x4 a[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a[t] = f32x4{b1[t][0], b1[t][1], b1[t][2], b1[t][3]};
+      a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], xb, a[t], 0, 0, 0);
+    }
+    bf16x8 hb[2];
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a[t] = f32x4{b2[t][0], b2[t][1], b2[t][2], b2[t][3]};
+      a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], hb[0], a[t], 0, 0, 0);
+      a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], hb[1], a[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a[t] = f32x4{b3[t][0], b3[t][1], b3[t][2], b3[t][3]};
+      a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], hb[0], a[t], 0, 0, 0);
+      a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], hb[1], a[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+    f32x4 o = f32x4{b4[0], b4[1], b4[2], b4[3]};
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[0], hb[0], o, 0, 0, 0);
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[1], hb[1], o, 0, 0, 0);
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) z[(int64_t)(4 * kb + r) * F.NENK + sd] = s * (double)o[r];
+    }
+  }
+}
+
+// ---- training kernels (FP32) ------------------------------------------------
+
+// out[b][j] = act?(sum_k in[row(b)][k] W[k][j] + bias[j]); row(b) = idx ? idx[b] : b.
+__global__ __launch_bounds__(256) void dense_fwd_kernel(int B, int K, int J, const float* __restrict__ in,
+                                                        const int32_t* __restrict__ idx, const float* __restrict__ W,
+                                                        const float* __restrict__ bias, int use_act,
+                                                        float* __restrict__ out) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= B * J) return;
+  const int b = t / J, j = t - b * J;
+  const int64_t row = idx ? (int64_t)idx[b] : (int64_t)b;
+  const float* x = in + row * K;
+  float a = bias[j];
+  for (int k = 0; k < K; ++k) a += x[k] * W[k * J + j];
+  out[t] = use_act ? act(a) : a;
+}
+
+// dy = 2 (yhat - y) / (B * kOut); part[block] = sum of squared errors / (B * kOut)
+__global__ __launch_bounds__(256) void loss_grad_kernel(int B, const float* __restrict__ yhat,
+                                                        const float* __restrict__ Y, const int32_t* __restrict__ idx,
+                                                        float* __restrict__ dy, float* __restrict__ part) {
+  __shared__ float sh[256];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  float e2 = 0.f;
+  if (t < B * kOut) {
+    const int b = t / kOut, j = t - b * kOut;
+    const float d = yhat[t] - Y[(int64_t)idx[b] * kOut + j];
+    dy[t] = 2.f * d / (float)(B * kOut);
+    e2 = d * d / (float)(B * kOut);
+  }
+  sh[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void sum_kernel(int n, const float* __restrict__ part, float* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    float a = 0.f;
+    for (int i = 0; i < n; ++i) a += part[i];
+    out[0] = a;
+  }
+}
+
+// dW[k][j] = sum_b in[row(b)][k] dout[b][j] (threads k*J + j), db[j] = sum_b dout[b][j] (threads K*J + j).
+__global__ __launch_bounds__(256) void dense_bwd_w_kernel(int B, int K, int J, const float* __restrict__ in,
+                                                          const int32_t* __restrict__ idx,
+                                                          const float* __restrict__ dout, float* __restrict__ dW,
+                                                          float* __restrict__ db) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < K * J) {
+    const int k = t / J, j = t - k * J;
+    float a = 0.f;
+    for (int b = 0; b < B; ++b) {
+      const int64_t row = idx ? (int64_t)idx[b] : (int64_t)b;
+      a += in[row * K + k] * dout[b * J + j];
+    }
+    dW[t] = a;
+  } else if (t < K * J + J) {
+    const int j = t - K * J;
+    float a = 0.f;
+    for (int b = 0; b < B; ++b) a += dout[b * J + j];
+    db[j] = a;
+  }
+}
+
+// din[b][k] = (sum_j dout[b][j] W[k][j]) (1 - h[b][k]^2), h = tanh output of the layer below.
+__global__ __launch_bounds__(256) void dense_bwd_x_kernel(int B, int K, int J, const float* __restrict__ dout,
+                                                          const float* __restrict__ W, const float* __restrict__ h,
+                                                          float* __restrict__ din) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= B * K) return;
+  const int b = t / K, k = t - b * K;
+  float a = 0.f;
+  for (int j = 0; j < J; ++j) a += dout[b * J + j] * W[k * J + j];
+  const float hv = h[t];
+  din[t] = a * (1.f - hv * hv);
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(int n, float* __restrict__ w, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                                   float b2, float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+inline unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+uint16_t bf16_rne(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Input neuron carried by K slot q of lane group kb in step st: layer 1 reads
+// the features in order; deeper layers read the registers the layer below left.
+int slot_input(int layer, int kb, int st, int q) {
+  if (layer == 1) return 8 * kb + q;
+  return 16 * (2 * st + (q >> 2)) + 4 * kb + (q & 3);
+}
+
+int feat_args(lb_op* op, const double* r, int g0, FeatArgs* F) {
+  if (op->nmom != 16) {
+    set_error("learned preconditioner: the network takes 16 flux moments (P3)");
+    return LB_E_ARG;
+  }
+  if (!op->d_elem_h.p) {
+    std::vector<double> h(op->NE);
+    for (int64_t e = 0; e < op->NE; ++e) h[e] = std::cbrt(op->h_detJ[e]);
+    LB_TRY(upload(op, op->d_elem_h, h.data(), h.size() * sizeof(double)));
+  }
+  const int64_t nphi = op->nmom * op->NENK;
+  double rr = 0.0;
+  LB_TRY(dot(op, nphi, r, r, &rr));
+  const double rms = std::sqrt(rr / (double)nphi);
+  *F = FeatArgs{r, op->NENK, rms > 0.0 ? 1.0 / rms : 0.0, op->d_mat.as<const int32_t>(),
+                op->d_sigt.as<const double>(), op->d_sigs.as<const double>(), op->G, g0, op->L,
+                op->d_elem_h.as<const double>()};
+  return LB_OK;
+}
+
+}  // namespace
+
+int mlp_set(lb_op* op, const float* params, int nparams, int fmt) {
+  if (!params || nparams != kParams || (fmt != LB_FMT_BF16 && fmt != LB_FMT_FP32)) {
+    set_error("lb_op_set_preconditioner: expects LB_MLP_PARAMS FP32 values and LB_FMT_BF16 or LB_FMT_FP32");
+    return LB_E_ARG;
+  }
+  if (op->nmom != 16) {
+    set_error("lb_op_set_preconditioner: the network takes 16 flux moments (P3 quadrature sets only)");
+    return LB_E_ARG;
+  }
+  for (int i = 0; i < nparams; ++i)
+    if (!std::isfinite(params[i])) {
+      set_error("lb_op_set_preconditioner: non-finite parameter");
+      return LB_E_ARG;
+    }
+  std::vector<uint16_t> pk(kPackLen, 0);
+  auto pack = [&](int layer, int off, int wofs, int K, int J) {
+    const int tiles = J / 16, steps = (layer == 1) ? 1 : 2;
+    for (int t = 0; t < tiles; ++t)
+      for (int st = 0; st < steps; ++st)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int q = 0; q < 8; ++q) {
+            const int in = slot_input(layer, lane >> 4, st, q);
+            const int out = 16 * t + (lane & 15);
+            const float v = in < K ? params[wofs + in * J + out] : 0.f;
+            pk[off + ((t * steps + st) * 64 + lane) * 8 + q] = bf16_rne(v);
+          }
+  };
+  pack(1, kP1, kW1, kIn, kH);
+  pack(2, kP2, kW2, kH, kH);
+  pack(3, kP3, kW3, kH, kH);
+  pack(4, kP4, kW4, kH, kOut);
+  LB_TRY(op->d_mlp_w.reserve(kParams * sizeof(float)));
+  LB_HIP(hipMemcpy(op->d_mlp_w.p, params, kParams * sizeof(float), hipMemcpyHostToDevice));
+  LB_TRY(op->d_mlp_pack.reserve(kPackLen * sizeof(uint16_t)));
+  LB_HIP(hipMemcpy(op->d_mlp_pack.p, pk.data(), kPackLen * sizeof(uint16_t), hipMemcpyHostToDevice));
+  op->mlp = true;
+  op->mlp_fmt = fmt;
+  return LB_OK;
+}
+
+int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale) {
+  FeatArgs F;
+  LB_TRY(feat_args(op, r, g0, &F));
+  hipLaunchKernelGGL(features_kernel, dim3(blocks(op->NENK)), dim3(256), 0, op->stream, F, feat);
+  LB_HIP(hipGetLastError());
+  if (scale) *scale = F.inv_s > 0.0 ? 1.0 / F.inv_s : 0.0;
+  return LB_OK;
+}
+
+int mlp_apply(lb_op* op, const double* r, double* z, int g0) {
+  if (!op->mlp) {
+    set_error("learned preconditioner: no network set (lb_op_set_preconditioner)");
+    return LB_E_NOT_READY;
+  }
+  FeatArgs F;
+  LB_TRY(feat_args(op, r, g0, &F));
+  const double s = F.inv_s > 0.0 ? 1.0 / F.inv_s : 0.0;
+  if (op->mlp_fmt == LB_FMT_FP32) {
+    hipLaunchKernelGGL(mlp_fp32_kernel, dim3(blocks(op->NENK)), dim3(256), 0, op->stream, F,
+                       op->d_mlp_w.as<const float>(), s, z);
+  } else {
+    const int64_t tiles = (op->NENK + 15) / 16;
+    const unsigned grid = (unsigned)std::min<int64_t>((tiles + 3) / 4, 8192);
+    hipLaunchKernelGGL(mlp_bf16_kernel, dim3(grid), dim3(256), 0, op->stream, F, op->d_mlp_pack.as<const __bf16>(),
+                       op->d_mlp_w.as<const float>(), s, z);
+  }
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+}  // namespace lb
+
+using namespace lb;
+
+extern "C" {
+
+// Minibatch Adam on (X [N][18], Y [N][16]) device rows; params in/out on the host.
+// At most 4096 rows per batch, so the loss partials (B * 16 / 256 blocks) fit the
+// 256-entry buffer.
+int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* params, int nparams, int steps, int batch,
+                     float lr, uint64_t seed, float* loss_hist) {
+  if (!d_X || !d_Y || !params || nparams != kParams || N < 1 || N > INT32_MAX || steps < 1 || batch < 1 ||
+      batch > 4096 || !(lr > 0.f)) {
+    set_error("lb_mlp_train_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  hipStream_t st = nullptr;
+  LB_HIP(hipStreamCreate(&st));
+  const int B = batch;
+  // every buffer in one allocation: params, grads, Adam m and v, activations, indices, loss
+  const size_t nf = (size_t)4 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 256 + steps;
+  const size_t bytes = nf * sizeof(float) + (size_t)steps * B * sizeof(int32_t);
+  void* mem = nullptr;
+  hipError_t e = hipMalloc(&mem, bytes);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(st);
+    set_error(std::string("lb_mlp_train_dev: hipMalloc failed: ") + hipGetErrorString(e));
+    return LB_E_HIP;
+  }
+  float* w = static_cast<float*>(mem);
+  float* g = w + kParams;
+  float* am = g + kParams;
+  float* av = am + kParams;
+  float* h1 = av + kParams;
+  float* h2 = h1 + (size_t)B * kH;
+  float* h3 = h2 + (size_t)B * kH;
+  float* yh = h3 + (size_t)B * kH;
+  float* d3 = yh + (size_t)B * kOut;
+  float* d2 = d3 + (size_t)B * kH;
+  float* d1 = d2 + (size_t)B * kH;
+  float* dy = d1 + (size_t)B * kH;
+  float* part = dy + (size_t)B * kOut;
+  float* loss = part + 256;
+  int32_t* idx = reinterpret_cast<int32_t*>(loss + steps);
+  int rc = LB_OK;
+  auto fail = [&](hipError_t err, const char* what) {
+    set_error(std::string("lb_mlp_train_dev: ") + what + ": " + hipGetErrorString(err));
+    rc = LB_E_HIP;
+  };
+  // minibatch rows: SplitMix64 stream `seed`
+  std::vector<int32_t> h_idx((size_t)steps * B);
+  for (size_t j = 0; j < h_idx.size(); ++j) {
+    uint64_t zz = seed + (uint64_t)(j + 1) * 0x9E3779B97F4A7C15ull;
+    zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+    zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+    zz = zz ^ (zz >> 31);
+    h_idx[j] = (int32_t)(zz % (uint64_t)N);
+  }
+  if ((e = hipMemcpyAsync(idx, h_idx.data(), h_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(w, params, kParams * sizeof(float), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemsetAsync(am, 0, 2 * kParams * sizeof(float), st)) != hipSuccess)
+    fail(e, "upload");
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float bc1 = 1.f, bc2 = 1.f;
+  for (int s = 0; s < steps && rc == LB_OK; ++s) {
+    const int32_t* ix = idx + (size_t)s * B;
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix,
+                       w + kW1, w + kB1, 1, h1);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
+                       w + kW2, w + kB2, 1, h2);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
+                       w + kW3, w + kB3, 1, h3);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                       w + kW4, w + kB4, 0, yh);
+    const unsigned lb = blocks((int64_t)B * kOut);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, ix, dy, part);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss + s);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                       dy, g + kW4, g + kB4);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
+                       h3, d3);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
+                       g + kW3, g + kB3);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
+                       h2, d2);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
+                       g + kW2, g + kB2);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
+                       h1, d1);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix, d1,
+                       g + kW1, g + kB1);
+    bc1 *= b1;
+    bc2 *= b2;
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks(kParams)), dim3(256), 0, st, kParams, w, g, am, av, lr, b1, b2, eps,
+                       1.f - bc1, 1.f - bc2);
+    if ((e = hipGetLastError()) != hipSuccess) fail(e, "launch");
+  }
+  if (rc == LB_OK) {
+    if ((e = hipMemcpyAsync(params, w, kParams * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      fail(e, "download");
+    else if (loss_hist &&
+             (e = hipMemcpyAsync(loss_hist, loss, steps * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      fail(e, "download");
+    else if ((e = hipStreamSynchronize(st)) != hipSuccess)
+      fail(e, "synchronize");
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(mem);
+  (void)hipStreamDestroy(st);
+  return rc;
+}
+
+// One gradient of the mean squared error over rows 0..B-1 (no update), for tests.
+int lb_mlp_grad_dev(const float* d_X, const float* d_Y, int B, const float* params, int nparams, float* grad,
+                    float* loss_out) {
+  if (!d_X || !d_Y || !params || !grad || nparams != kParams || B < 1 || B > 4096) {
+    set_error("lb_mlp_grad_dev: invalid argument");
+    return LB_E_ARG;
+  }
+  void* mem = nullptr;
+  const size_t nf = (size_t)2 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 257;
+  LB_HIP(hipMalloc(&mem, nf * sizeof(float) + (size_t)B * sizeof(int32_t)));
+  float* w = static_cast<float*>(mem);
+  float* g = w + kParams;
+  float* h1 = g + kParams;
+  float* h2 = h1 + (size_t)B * kH;
+  float* h3 = h2 + (size_t)B * kH;
+  float* yh = h3 + (size_t)B * kH;
+  float* d3 = yh + (size_t)B * kOut;
+  float* d2 = d3 + (size_t)B * kH;
+  float* d1 = d2 + (size_t)B * kH;
+  float* dy = d1 + (size_t)B * kH;
+  float* part = dy + (size_t)B * kOut;
+  float* loss = part + 256;
+  int32_t* idx = reinterpret_cast<int32_t*>(loss + 1);
+  std::vector<int32_t> h_idx(B);
+  for (int b = 0; b < B; ++b) h_idx[b] = b;
+  hipStream_t st = nullptr;
+  hipError_t e = hipSuccess;
+  int rc = LB_OK;
+  if ((e = hipMemcpy(idx, h_idx.data(), B * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(w, params, kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+    set_error(std::string("lb_mlp_grad_dev: ") + hipGetErrorString(e));
+    rc = LB_E_HIP;
+  }
+  if (rc == LB_OK) {
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx,
+                       w + kW1, w + kB1, 1, h1);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
+                       w + kW2, w + kB2, 1, h2);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
+                       w + kW3, w + kB3, 1, h3);
+    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                       w + kW4, w + kB4, 0, yh);
+    const unsigned lb = blocks((int64_t)B * kOut);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, idx, dy, part);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                       dy, g + kW4, g + kB4);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
+                       h3, d3);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
+                       g + kW3, g + kB3);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
+                       h2, d2);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
+                       g + kW2, g + kB2);
+    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
+                       h1, d1);
+    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx, d1,
+                       g + kW1, g + kB1);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
+        (e = hipMemcpy(grad, g, kParams * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess ||
+        (loss_out && (e = hipMemcpy(loss_out, loss, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)) {
+      set_error(std::string("lb_mlp_grad_dev: ") + hipGetErrorString(e));
+      rc = LB_E_HIP;
+    }
+  }
+  (void)hipDeviceSynchronize();
+  (void)hipFree(mem);
+  return rc;
+}
+
+}  // extern "C"

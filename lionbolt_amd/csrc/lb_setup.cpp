@@ -170,6 +170,7 @@ int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int
   LB_TRY(upload(op, op->d_geo, geo.data(), geo.size() * sizeof(double)));
   LB_TRY(upload(op, op->d_nbr, nbr32.data(), nbr32.size() * sizeof(int32_t)));
   op->h_nbr = std::move(nbr32);
+  op->h_detJ = detJ;
   LB_TRY(upload(op, op->d_perm, perm.data(), perm.size() * sizeof(uint32_t)));
   LB_TRY(upload(op, op->d_mat, op->h_mat.data(), op->h_mat.size() * sizeof(int32_t)));
   return compute_schedule(op, nb.data());

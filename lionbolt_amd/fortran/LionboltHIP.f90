@@ -38,6 +38,7 @@ Module LionboltHIP
     Type, Bind(C), Public :: lb_solve_stats
         Integer (c_int) :: status, iterates, matvecs, sweeps
         Real (c_double) :: final_norm, true_residual, seconds
+        Integer (c_int) :: fallback
     End Type
 
     Interface

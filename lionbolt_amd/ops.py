@@ -149,7 +149,7 @@ class Operator:
     def boltzmann_matinv(self, s, kind="gmres", space="psi", m=10, rtol=1e-8, atol=1e-12, maxit=0, verbose=False,
                          raise_on_fail=True):
         x = self._vec(s)
-        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI}[kind],
+        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI, "fgmres": _lib.SOLVER_FGMRES}[kind],
                                {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol,
                                int(verbose))
         st = _lib.SolveStats()
@@ -190,7 +190,7 @@ class Operator:
                              atol=1e-12, maxit=0, raise_on_fail=True):
         """Group-by-group downscatter solve; ``transfer`` (G, G, Nm, L+1) host array [g', g]."""
         transfer = np.ascontiguousarray(transfer, dtype=np.float64)
-        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI}[kind],
+        opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI, "fgmres": _lib.SOLVER_FGMRES}[kind],
                                {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol, 0)
         stats = (_lib.SolveStats * self.G)()
         rc = _lib.lib().lb_multigroup_solve_dev(self.h, None if qmom is None else qmom.data_ptr(),
@@ -198,6 +198,43 @@ class Operator:
         if raise_on_fail:
             _lib.check(rc)
         return [s.as_dict() for s in stats]
+
+    # -- learned right preconditioner and the within-group moment system ---------
+    def set_preconditioner(self, params, fmt="bf16"):
+        """Per-cell-node MLP parameters (``_lib.MLP_PARAMS`` float32) for FGMRES."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        f = {"bf16": _lib.FMT_BF16, "fp32": _lib.FMT_FP32}[fmt]
+        _lib.check(_lib.lib().lb_op_set_preconditioner(self.h, p.ctypes.data_as(_lib._f), p.size, f))
+
+    def mlp_apply_dev(self, r, z, g0=0):
+        _lib.check(_lib.lib().lb_mlp_apply_dev(self.h, r.data_ptr(), z.data_ptr(), int(g0)))
+
+    def mlp_features_dev(self, r, g0=0):
+        import torch
+        feat = torch.empty((self.NENK, 18), dtype=torch.float32, device=r.device)
+        s = ctypes.c_double()
+        _lib.check(_lib.lib().lb_mlp_features_dev(self.h, r.data_ptr(), int(g0), feat.data_ptr(), ctypes.byref(s)))
+        return feat, s.value
+
+    def moment_rhs_dev(self, q, b, g0=0):
+        _lib.check(_lib.lib().lb_moment_rhs_dev(self.h, q.data_ptr(), b.data_ptr(), int(g0)))
+
+    def discrete_source_dev(self, qmom, q):
+        _lib.check(_lib.lib().lb_discrete_source_dev(self.h, qmom.data_ptr(), q.data_ptr()))
+
+    def moment_matvec_dev(self, x, y, g0=0):
+        _lib.check(_lib.lib().lb_moment_matvec_dev(self.h, x.data_ptr(), y.data_ptr(), int(g0)))
+
+    def gmres_snapshots_dev(self, q, phi, snaps, m=30, rtol=1e-10, atol=1e-12, maxit=0):
+        """Flux-moment GMRES(m) recording the iterate after every Arnoldi step into
+        ``snaps`` (max_snaps x nmom*NENK); returns (count, stats)."""
+        opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
+        st = _lib.SolveStats()
+        cnt = ctypes.c_int()
+        _lib.check(_lib.lib().lb_gmres_snapshots_dev(self.h, q.data_ptr(), ctypes.byref(opts), phi.data_ptr(),
+                                                     snaps.data_ptr(), snaps.shape[0], ctypes.byref(cnt),
+                                                     ctypes.byref(st)))
+        return cnt.value, st.as_dict()
 
     def synchronize(self):
         _lib.check(_lib.lib().lb_op_synchronize(self.h))

@@ -28,8 +28,16 @@ Decisions recorded here (the reference never ran these configurations):
   uncollided flux is exp(-sigma_t l) with l the distance from the node back along
   -k0 to the cube surface; the first-collision source is isotropic,
   sigma_s phi_0 / (4 pi).
-* cfg5: cfg2 with a within-group scattering ratio of 0.99 in every group and no
-  downscatter (scattering-dominated, the stated purpose of the configuration).
+* cfg5: cfg2's mesh, directions, groups and sigma_t with a within-group
+  scattering ratio sigma_s0(g -> g) / sigma_t,g = 0.99 in every group (the
+  scattering-dominated regime the configuration asks for) and a weak downscatter
+  tail of 0.005 sigma_t,g' (shares 0.5^(g - g') as cfg2) so that the +z beam's
+  first-collision source reaches every group; sigma_l = 0.6^l sigma_0.
+* learned-preconditioner training problems: 32 jittered cubes (10 cm, n = 10..16
+  hexes per side, jitter stream 2000 + p, the 3x slab) with one group each,
+  sigma_t log-uniform on [0.02, 0.5] /cm and the cfg5 ratios (SplitMix64 stream
+  2000); sources: the +z beam's first-collision source and a uniform isotropic
+  source.
 """
 import math
 
@@ -148,3 +156,31 @@ def cfg4_batch(nprob=64, seed=1004, probs=None):
     qmom = np.concatenate([y00 * p["c"] * p["sigt"] * cube_beam_uncollided(p["mesh"], p["a"], p["k0"], p["sigt"])
                            for p in probs])[None, :]
     return um, eoff, sigt, sigs, qmom, probs
+
+
+def cfg5_xs(G=24, L=3, within=0.99, down=0.005):
+    """cfg5 cross sections: sigma_t (G, 2) of cfg2 and transfer moments (G, G, 2, L+1)."""
+    sigt, _ = cfg2_xs(G=G, L=L)
+    S = np.zeros((G, G, 2, L + 1))
+    for gp in range(G):
+        w = 0.5 ** np.arange(1, G - gp)
+        for mat in range(2):
+            S[gp, gp, mat, :] = within * sigt[gp, mat] * 0.6 ** np.arange(L + 1)
+            if gp + 1 < G:
+                for g in range(gp + 1, G):
+                    S[gp, g, mat, :] = down * sigt[gp, mat] * (w[g - gp - 1] / w.sum()) * 0.6 ** np.arange(L + 1)
+    return sigt, S
+
+
+def pc_training_problems(nmesh=32, seed=2000, L=3, within=0.99):
+    """Small single-group problems for training the learned preconditioner."""
+    u = rng.uniform(seed, 2 * nmesh).reshape(nmesh, 2)
+    out = []
+    for p in range(nmesh):
+        n = 10 + int(u[p, 0] * 7)
+        st = float(np.exp(np.log(0.02) + u[p, 1] * (np.log(0.5) - np.log(0.02))))
+        sigt = np.array([st, 3.0 * st])
+        sigs = np.stack([within * sigt[0] * 0.6 ** np.arange(L + 1), within * sigt[1] * 0.6 ** np.arange(L + 1)])
+        out.append({"n": n, "sigt": sigt, "sigs": sigs,
+                    "mesh": mesh.jittered_cube(n, a=10.0, jitter=0.2, seed=seed + p, slab=(4.0, 6.0))})
+    return out
