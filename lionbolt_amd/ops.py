@@ -24,6 +24,13 @@ MeshClass = TetMesh
 AngularClass = AngularSet
 
 
+def _torch_done():
+    """The library works on its own non-blocking stream: wait for the caller's
+    current torch stream so that tensors it just produced are complete."""
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
 class XSType:
     """Cross sections of one group: ``t(Nm)`` and Legendre moments ``s(0:L, Nm)``."""
 
@@ -114,6 +121,7 @@ class Operator:
     def batch_solve_dev(self, qmom, phi, m=30, rtol=1e-8, atol=1e-12, maxit=0, raise_on_fail=True):
         """Per-problem flux-moment GMRES(m) of the whole batch in lockstep (single group).
         ``qmom``/``phi``: cuda float64 tensors of nmom * NENK entries."""
+        _torch_done()
         opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
         stats = (_lib.SolveStats * self.P)()
         rc = _lib.lib().lb_batch_solve_dev(self.h, qmom.data_ptr(), ctypes.byref(opts), phi.data_ptr(), stats)
@@ -177,18 +185,24 @@ class Operator:
 
     # -- device entry points (torch tensors on cuda, group-batched layout) --------
     def transport_matinv_dev(self, q, psi, g0=0, G=1):
+        _torch_done()
         _lib.check(_lib.lib().lb_transport_matinv_dev(self.h, q.data_ptr(), psi.data_ptr(), g0, G))
 
     def moments_dev(self, psi, phi, G=1):
+        _torch_done()
         _lib.check(_lib.lib().lb_moments_dev(self.h, psi.data_ptr(), phi.data_ptr(), G))
+        self.synchronize()
 
     def moments_to_discrete_dev(self, phi, psi, g0=0, G=1, accumulate=False):
+        _torch_done()
         _lib.check(_lib.lib().lb_moments_to_discrete_dev(self.h, phi.data_ptr(), psi.data_ptr(), g0, G,
                                                          int(accumulate)))
+        self.synchronize()
 
     def multigroup_solve_dev(self, transfer, phi, qmom=None, kind="gmres", space="moment", m=30, rtol=1e-8,
                              atol=1e-12, maxit=0, raise_on_fail=True):
         """Group-by-group downscatter solve; ``transfer`` (G, G, Nm, L+1) host array [g', g]."""
+        _torch_done()
         transfer = np.ascontiguousarray(transfer, dtype=np.float64)
         opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI, "fgmres": _lib.SOLVER_FGMRES}[kind],
                                {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol, 0)
@@ -207,27 +221,38 @@ class Operator:
         _lib.check(_lib.lib().lb_op_set_preconditioner(self.h, p.ctypes.data_as(_lib._f), p.size, f))
 
     def mlp_apply_dev(self, r, z, g0=0):
+        _torch_done()
         _lib.check(_lib.lib().lb_mlp_apply_dev(self.h, r.data_ptr(), z.data_ptr(), int(g0)))
+        self.synchronize()
 
     def mlp_features_dev(self, r, g0=0):
+        _torch_done()
         import torch
         feat = torch.empty((self.NENK, 18), dtype=torch.float32, device=r.device)
         s = ctypes.c_double()
         _lib.check(_lib.lib().lb_mlp_features_dev(self.h, r.data_ptr(), int(g0), feat.data_ptr(), ctypes.byref(s)))
+        self.synchronize()
         return feat, s.value
 
     def moment_rhs_dev(self, q, b, g0=0):
+        _torch_done()
         _lib.check(_lib.lib().lb_moment_rhs_dev(self.h, q.data_ptr(), b.data_ptr(), int(g0)))
+        self.synchronize()
 
     def discrete_source_dev(self, qmom, q):
+        _torch_done()
         _lib.check(_lib.lib().lb_discrete_source_dev(self.h, qmom.data_ptr(), q.data_ptr()))
+        self.synchronize()
 
     def moment_matvec_dev(self, x, y, g0=0):
+        _torch_done()
         _lib.check(_lib.lib().lb_moment_matvec_dev(self.h, x.data_ptr(), y.data_ptr(), int(g0)))
+        self.synchronize()
 
     def gmres_snapshots_dev(self, q, phi, snaps, m=30, rtol=1e-10, atol=1e-12, maxit=0):
         """Flux-moment GMRES(m) recording the iterate after every Arnoldi step into
         ``snaps`` (max_snaps x nmom*NENK); returns (count, stats)."""
+        _torch_done()
         opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
         st = _lib.SolveStats()
         cnt = ctypes.c_int()
