@@ -58,7 +58,8 @@ def test_training_gradient_matches_autograd():
     params = learned_pc.init_params(11)
     grad = np.empty_like(params)
     loss = np.empty(1, dtype=np.float32)
-    _lib.check(_lib.lib().lb_mlp_grad_dev(X.cuda().data_ptr(), Y.cuda().data_ptr(), B,
+    Xd, Yd = X.cuda(), Y.cuda()                 # keep the device copies alive across the call
+    _lib.check(_lib.lib().lb_mlp_grad_dev(Xd.data_ptr(), Yd.data_ptr(), B,
                                           params.ctypes.data_as(_lib._f), params.size,
                                           grad.ctypes.data_as(_lib._f), loss.ctypes.data_as(_lib._f)))
     P = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in learned_pc.unpack(params).items()}
