@@ -235,16 +235,21 @@ struct FgmresResult {
   double final_norm = 0.0;
 };
 
-// Flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14 (1993) 461-469): right
-// preconditioner z_k = P(v_k), which may change from step to step, and
-// x = x0 + Z y.  Modified Gram-Schmidt with mgsro's selective
-// re-orthogonalisation (0.98 test), Givens rotations, stopping rule
-// ||r|| <= rtol ||rhs|| + atol and an explicit residual at every restart, as
-// gmres() above.  Fallback: when the residual estimate has fallen by less than
-// half over the last 10 Arnoldi steps, or P returns a non-finite vector, P is
-// replaced by the identity (plain GMRES) for the rest of the solve.
+// Flexible GMRES(m) (Saad, SIAM J. Sci. Comput. 14 (1993) 461-469): any
+// search direction z_k may enter the basis, A Z_k = V_{k+1} H_k still holds, and
+// x = x0 + Z y minimises the residual over span(Z).  The learned preconditioner
+// maps residuals to errors, so it is applied to the current residual r_k (the
+// GMRESR / flexible GCR choice of van der Vorst & Vuik, 1994): z_k = P(r_k).
+// r_k is carried along at the cost of one update per step,
+// r_k = s_k^2 r_{k-1} + c_k rho_k v_k (rho_k the rotated right-hand side), and
+// with P = I the basis is the Krylov space of plain GMRES.  Modified Gram-Schmidt
+// with mgsro's selective re-orthogonalisation (0.98 test), Givens rotations,
+// stopping rule ||r|| <= rtol ||rhs|| + atol and an explicit residual at every
+// restart, as gmres() above.  Fallback: when the residual estimate has fallen by
+// less than half over the last 10 steps, or P returns a non-finite or zero
+// vector, z_k = v_k (plain GMRES) for the rest of the solve.
 int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
-           double* Z, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
+           double* Z, double* R, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
            FgmresResult* res) {
   hipStream_t s = op->stream;
   auto vec = [&](int j) { return V + (int64_t)j * n; };
@@ -287,6 +292,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
       break;
     }
     if (maxit > 0 && iters >= maxit) break;
+    if (use_pc) LB_TRY(launch_copy(s, n, v0, R));   // r_0
     LB_TRY(launch_scale(s, n, 1.0 / beta, v0));
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
@@ -294,7 +300,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
     while (k < m) {
       double* zk = zvec(k);
       if (use_pc) {
-        LB_TRY((*P)(vec(k), zk));
+        LB_TRY((*P)(R, zk));
         double zz = 0.0;
         LB_TRY(dot(op, n, zk, zk, &zz));
         if (!std::isfinite(zz) || zz == 0.0) {
@@ -339,6 +345,8 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
       g[k + 1] = -sn[k] * g[k];
       g[k] = c[k] * g[k];
       resn = std::fabs(g[k + 1]);
+      if (use_pc && hn > 0.0)   // r_{k+1} = s^2 r_k + c rho_{k+1} v_{k+1}
+        LB_TRY(launch_axpby(s, n, c[k] * g[k + 1], vec(k + 1), sn[k] * sn[k], R, R));
       ++k;
       ++iters;
       hist.push_back(resn);
@@ -423,7 +431,8 @@ int gmres_snapshots_dev(lb_op* op, const double* q, const lb_solver_opts* o, dou
   };
   Apply A = [&](double* v) { return moment_matvec_dev(op, v, 0); };
   FgmresResult r;
-  LB_TRY(fgmres(op, nphi, A, nullptr, rhs, phi, V, Z, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &snap, &r));
+  LB_TRY(fgmres(op, nphi, A, nullptr, rhs, phi, V, Z, nullptr, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &snap,
+                &r));
   LB_HIP(hipStreamSynchronize(op->stream));
   if (nsnaps) *nsnaps = count;
   if (st) {
@@ -552,12 +561,13 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
       return LB_E_NOT_READY;
     }
     const int m = opt.restart;
-    LB_TRY(op->w_krylov.reserve((size_t)(2 * m + 3) * nphi * sizeof(double)));
+    LB_TRY(op->w_krylov.reserve((size_t)(2 * m + 4) * nphi * sizeof(double)));
     LB_TRY(op->w_a2.reserve(n * sizeof(double)));
     double* V = op->w_krylov.as<double>();
     double* Z = V + (int64_t)(m + 1) * nphi;
     double* phi = Z + (int64_t)m * nphi;
     double* rhs = phi + nphi;
+    double* R = rhs + nphi;
     LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
     LB_TRY(launch_copy(str, nphi, rhs, phi));
     LB_TRY(launch_scale(str, nphi, 1.0 / sig1, phi));
@@ -567,7 +577,7 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     };
     Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g0); };
     FgmresResult r;
-    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r));
+    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r));
     LB_TRY(launch_copy(str, n, q, s));
     LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
     LB_TRY(sweep_inplace(op, s, g0));
