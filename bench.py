@@ -311,7 +311,7 @@ def batch_leg(reps=10, rtol=1e-6):
     return res
 
 
-def learned_pc_leg(m, ang, rtol=5e-8, fmt="bf16"):
+def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     """cfg5: the 24-group P3 problem with within-group c = 0.99, solved group by
     group with plain flux-moment GMRES(30) and with flexible GMRES(30) right-
     preconditioned by the per-cell-node network, trained first on the solver's own
@@ -334,7 +334,7 @@ def learned_pc_leg(m, ang, rtol=5e-8, fmt="bf16"):
         phi = torch.zeros_like(qmom)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = op.multigroup_solve_dev(S, phi, qmom=qmom, kind=kind, space="moment", m=30, rtol=rtol)
+        st = op.multigroup_solve_dev(S, phi, qmom=qmom, kind=kind, space="moment", m=30, rtol=rtol, atol=0.0)
         torch.cuda.synchronize()
         res[kind] = {"seconds": time.perf_counter() - t0, "iterates": [s["iterates"] for s in st],
                      "total_iterates": sum(s["iterates"] for s in st), "sweeps": sum(s["sweeps"] for s in st),

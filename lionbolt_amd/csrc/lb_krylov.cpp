@@ -245,9 +245,11 @@ struct FgmresResult {
 // with P = I the basis is the Krylov space of plain GMRES.  Modified Gram-Schmidt
 // with mgsro's selective re-orthogonalisation (0.98 test), Givens rotations,
 // stopping rule ||r|| <= rtol ||rhs|| + atol and an explicit residual at every
-// restart, as gmres() above.  Fallback: when the residual estimate has fallen by
-// less than half over the last 10 steps, or P returns a non-finite or zero
-// vector, z_k = v_k (plain GMRES) for the rest of the solve.
+// restart, as gmres() above.  Fallback, z_k = v_k (plain GMRES) for the rest of
+// the solve: when the first 3 steps reduce the residual by less than a factor 2
+// per step on average (a useful approximate inverse does better than that), when
+// the residual estimate has fallen by less than half over the last 10 steps, or
+// when P returns a non-finite or zero vector.
 int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
            double* Z, double* R, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
            FgmresResult* res) {
@@ -258,6 +260,9 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
   auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
   constexpr int kWindow = 10;
   constexpr double kDrop = 0.5;
+  constexpr int kProbe = 3;
+  constexpr double kProbeRate = 0.5;
+  double beta0 = -1.0;
   double r2 = 0.0;
   LB_TRY(dot(op, n, rhs, rhs, &r2));
   const double eps = rtol * std::sqrt(r2) + atol;
@@ -292,6 +297,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
       break;
     }
     if (maxit > 0 && iters >= maxit) break;
+    if (beta0 < 0.0) beta0 = beta;
     if (use_pc) LB_TRY(launch_copy(s, n, v0, R));   // r_0
     LB_TRY(launch_scale(s, n, 1.0 / beta, v0));
     std::fill(g.begin(), g.end(), 0.0);
@@ -361,7 +367,8 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
         done = true;
         break;
       }
-      if (use_pc && (int)hist.size() > kWindow && resn > kDrop * hist[hist.size() - 1 - kWindow]) {
+      if (use_pc && ((iters == kProbe && resn > std::pow(kProbeRate, kProbe) * beta0) ||
+                     ((int)hist.size() > kWindow && resn > kDrop * hist[hist.size() - 1 - kWindow]))) {
         use_pc = false;
         fallback = iters;
       }
