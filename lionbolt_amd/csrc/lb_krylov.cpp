@@ -246,10 +246,9 @@ struct FgmresResult {
 // with mgsro's selective re-orthogonalisation (0.98 test), Givens rotations,
 // stopping rule ||r|| <= rtol ||rhs|| + atol and an explicit residual at every
 // restart, as gmres() above.  Fallback, z_k = v_k (plain GMRES) for the rest of
-// the solve: when the first 3 steps reduce the residual by less than a factor 2
-// per step on average (a useful approximate inverse does better than that), when
-// the residual estimate has fallen by less than half over the last 10 steps, or
-// when P returns a non-finite or zero vector.
+// the solve: when any 3 consecutive preconditioned steps reduce the residual
+// estimate by less than a factor 2 per step on average (a useful approximate
+// inverse does better than that), or when P returns a non-finite or zero vector.
 int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
            double* Z, double* R, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
            FgmresResult* res) {
@@ -258,10 +257,8 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
   auto zvec = [&](int j) { return Z + (int64_t)j * n; };
   std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hist;
   auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
-  constexpr int kWindow = 10;
-  constexpr double kDrop = 0.5;
-  constexpr int kProbe = 3;
-  constexpr double kProbeRate = 0.5;
+  constexpr int kWindow = 3;
+  constexpr double kDrop = 0.125;   // 0.5 per step over the window
   double beta0 = -1.0;
   double r2 = 0.0;
   LB_TRY(dot(op, n, rhs, rhs, &r2));
@@ -367,8 +364,9 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
         done = true;
         break;
       }
-      if (use_pc && ((iters == kProbe && resn > std::pow(kProbeRate, kProbe) * beta0) ||
-                     ((int)hist.size() > kWindow && resn > kDrop * hist[hist.size() - 1 - kWindow]))) {
+      const double before = (int)hist.size() > kWindow ? hist[hist.size() - 1 - kWindow]
+                            : ((int)hist.size() == kWindow ? beta0 : -1.0);
+      if (use_pc && before > 0.0 && resn > kDrop * before) {
         use_pc = false;
         fallback = iters;
       }
