@@ -235,35 +235,61 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
   solve_cell(A, wd.x, wd.y + c, g);
 }
 
+// Static part of the cells this lane sweeps in one level: up to R rounds of
+// BLOCK / G cells each.
+template <int R>
+__device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
+                                           bool lane_ok, CellPrep (&P)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = s0 + c + r * cpr;
+    if (lane_ok && s < s1) prep_cell(A, i, s, g, P[r]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
+                                             bool lane_ok, const CellPrep (&P)[R]) {
+  if (!lane_ok) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = s0 + c + r * cpr;
+    if (s < s1) finish_cell(A, i, P[r], g);
+  }
+  for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell(A, i, s, g);   // levels beyond R rounds
+}
+
 // Walks the levels of one direction (positions off[0..nl]) with the whole
 // workgroup: BLOCK / G cells per round, a workgroup barrier between levels.  The
-// first cell of each level for this lane is prepared one level ahead, so only
-// the upstream-trace loads sit between two barriers.
-template <int BLOCK>
+// static data of level l+1 (records, c_f k.n, cross section, source) is loaded
+// at the start of level l into the second of two register buffers, so its
+// latency overlaps level l's upstream loads and solves; only the upstream-trace
+// loads of a level wait for the barrier.
+template <int BLOCK, int R>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl) {
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
-  CellPrep P;
-  bool have = false;
-  if (nl > 0 && lane_ok && off[0] + c < off[1]) {
-    prep_cell(A, i, off[0] + c, g, P);
-    have = true;
-  }
-  for (int l = 0; l < nl; ++l) {
-    const int s0 = off[l], s1 = off[l + 1];
-    if (have) finish_cell(A, i, P, g);
-    if (lane_ok)
-      for (int s = s0 + c + cpr; s < s1; s += cpr) solve_cell(A, i, s, g);
-    have = false;
-    if (l + 1 < nl && lane_ok && s1 + c < off[l + 2]) {
-      prep_cell(A, i, s1 + c, g, P);
-      have = true;
-    }
+  CellPrep Pa[R], Pb[R];
+  if (nl > 0) prep_level<R>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
+  for (int l = 0; l < nl; l += 2) {
+    if (l + 1 < nl) prep_level<R>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    finish_level<R>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa);
+    __syncthreads();
+    if (l + 1 >= nl) break;
+    if (l + 2 < nl) prep_level<R>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
+    finish_level<R>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
     __syncthreads();
   }
 }
+
+// Rounds of static data held per lane for a workgroup size (register budget:
+// a 1024-lane workgroup leaves 128 VGPRs per lane).
+template <int BLOCK>
+struct LevelRounds {
+  static constexpr int value = BLOCK >= 1024 ? 1 : 2;
+};
 
 // Direction-resident sweep: one workgroup owns one direction (of one batch of
 // groups) and walks its levels in order.  All producers and consumers of a
@@ -276,7 +302,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
                                                                 const int32_t* __restrict__ nlev, int maxlev,
                                                                 SweepArgs A) {
   const int i = blockIdx.x;
-  walk_levels<BLOCK>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+  walk_levels<BLOCK, LevelRounds<BLOCK>::value>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
@@ -299,7 +325,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_persistent_kernel(const int4* __r
     if (it >= nitems) break;
     const int4 d = items[it];
     if (active[d.x] == 0) continue;
-    walk_levels<BLOCK>(A, d.y, lvl + d.z, d.w);
+    walk_levels<BLOCK, 1>(A, d.y, lvl + d.z, d.w);   // one round: several workgroups share a CU
   }
 }
 
