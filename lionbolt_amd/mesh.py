@@ -155,11 +155,17 @@ def jittered_cube(n, a=10.0, jitter=0.2, seed=1001, slab=None):
 
 
 def read_gmsh41(path):
-    """Read the 4-node tetrahedra of an ASCII GMSH 4.1 file.
+    """Read the 4-node tetrahedra of an ASCII GMSH 4.1 file as the reference does.
 
-    Element blocks are taken in file order and nodes are renumbered densely in
-    the order they appear in ``$Nodes`` (``ReadGMSH.f90:150-189``); the material
-    of an element is the physical tag of its volume entity (1 when untagged).
+    ``ReadGMSH.f90:75-176``: the material of the b-th 3D element block is the
+    physical tag of the b-th volume entity in ``$Entities`` (used as given, no
+    renumbering; a volume with no physical tag means material 1, which then must
+    hold for every volume; more than one tag is an error).  Elements are taken in
+    file order, block by block.  Nodes are numbered in ``$Nodes`` order (the
+    reference writes them 1..NK in that order, ``gmshprocessing.py:65-84``, and
+    indexes them by GMSH node tag; a dense renumbering of the tags is the same
+    map when the tags run 1..NK in order).  Neighbours come from the sorted
+    global node triples of the faces (``gmshprocessing.py:144-288``).
     """
     with open(path) as fh:
         lines = fh.read().split("\n")
@@ -167,18 +173,21 @@ def read_gmsh41(path):
     for i, line in enumerate(lines):
         if line.startswith("$") and not line.startswith("$End"):
             pos[line.strip()] = i
-    # Volume entity -> physical tag
-    vol_phys = {}
+    volinfo = []
     if "$Entities" in pos:
         i = pos["$Entities"] + 1
         npnt, ncur, nsur, nvol = (int(t) for t in lines[i].split()[:4])
         i += 1 + npnt + ncur + nsur
         for _ in range(nvol):
             t = lines[i].split()
-            tag = int(t[0])
             nphys = int(t[7])
-            vol_phys[tag] = abs(int(t[8])) if nphys > 0 else 1
+            if nphys > 1:
+                raise ValueError("ReadGMSH: a volume has more than one physical tag")
+            volinfo.append(int(t[8]) if nphys == 1 else 1)
             i += 1
+        if any(int(lines[pos["$Entities"] + 2 + npnt + ncur + nsur + b].split()[7]) == 0 for b in range(nvol)) \
+                and any(v != 1 for v in volinfo):
+            raise ValueError("ReadGMSH: the mesh does not specify a material for every volume")
     # Nodes
     i = pos["$Nodes"] + 1
     nblocks, nnodes = (int(t) for t in lines[i].split()[:2])
@@ -197,11 +206,12 @@ def read_gmsh41(path):
         c += cnt
     remap = np.full(int(tags.max()) + 1, -1, dtype=np.int64)
     remap[tags] = np.arange(nnodes)
-    # Elements
+    # Elements: 3D blocks in file order, block b takes volinfo[b]
     i = pos["$Elements"] + 1
     nblocks = int(lines[i].split()[0])
     i += 1
     conn, mats = [], []
+    b = 0
     for _ in range(nblocks):
         dim, etag, etype, cnt = (int(t) for t in lines[i].split()[:4])
         i += 1
@@ -210,14 +220,10 @@ def read_gmsh41(path):
                 raise ValueError("only 4-node tetrahedra are supported (ReadGMSH.f90:160-166)")
             blk = np.array([lines[i + j].split()[1:5] for j in range(cnt)], dtype=np.int64)
             conn.append(remap[blk])
-            mats.append(np.full(cnt, vol_phys.get(etag, 1), dtype=np.int64))
+            mats.append(np.full(cnt, volinfo[b] if b < len(volinfo) else 1, dtype=np.int64))
+            b += 1
         i += cnt
-    node = np.concatenate(conn)
-    mat = np.concatenate(mats)
-    # dense material numbering 1..Nm in order of first appearance of the tag value
-    uniq = sorted(set(mat.tolist()))
-    mat = np.searchsorted(np.array(uniq), mat) + 1
-    return TetMesh(xyz, node, mat)
+    return TetMesh(xyz, np.concatenate(conn), np.concatenate(mats))
 
 
 def union(meshes):
