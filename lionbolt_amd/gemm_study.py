@@ -159,3 +159,26 @@ def run(ncols=1536000, groups=24, seed=1003, reps=3, formats=tuple(FORMATS)):
     torch.cuda.synchronize()
     out["fp64_torch_per_group_d_seconds"] = (time.perf_counter() - t0) / reps
     return out
+
+
+def main():
+    """Command-line run of the study (the profile summaries under profiles/ use it)."""
+    import argparse
+    import json
+    p = argparse.ArgumentParser(description="scattering-moment products on MFMA: error and speed per format")
+    p.add_argument("--ncols", type=int, default=1536000, help="cell-nodes per group")
+    p.add_argument("--groups", type=int, default=24)
+    p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--formats", default=",".join(FORMATS))
+    p.add_argument("--out", default=None, help="also write the result JSON here")
+    a = p.parse_args()
+    res = run(ncols=a.ncols, groups=a.groups, reps=a.reps, formats=tuple(a.formats.split(",")))
+    text = json.dumps(res)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text)
+    print(text)
+
+
+if __name__ == "__main__":
+    main()
