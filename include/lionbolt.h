@@ -255,6 +255,16 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
 int lb_multigroup_solve_dev(lb_op* op, const double* d_qmom, const double* transfer, const lb_solver_opts* opts,
                             double* d_phi, lb_solve_stats* stats);
 
+/* Energy groups per coupled solve in lb_multigroup_solve_dev (1..16, default 1).
+ * With B > 1, batches of B consecutive groups are solved as one flux-moment
+ * GMRES system whose operator includes the downscatter inside the batch, so each
+ * Arnoldi step is one B-group sweep; batches still follow each other in energy
+ * (Gauss-Seidel).  Each group's relative residual meets rtol (the system is
+ * scaled per group and solved to rtol / sqrt(B)).  stats[g].sweeps holds the
+ * batch's sweeps in its first group and 0 in the others.  No reference
+ * counterpart (the reference solves group by group). */
+int lb_op_set_group_batch(lb_op* op, int groups);
+
 /* ---- scattering-moment products on MFMA, reduced input precision, FP32
  * accumulation (no reference counterpart: the reference applies the dense
  * FP64 kernel YYS with dgemm, ScatteringAlgebra.f90:30-53).  Formats:

@@ -59,6 +59,7 @@ SIGNATURES = {
     "lb_op_sweep_choice": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]),
     "lb_op_set_workgroups_per_cu": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lb_op_set_group_batch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_op_set_problems": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
     "lb_batch_solve_dev": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(SolverOpts), _vp, ctypes.POINTER(SolveStats)]),
     "lb_op_set_preconditioner": (ctypes.c_int, [_vp, _f, ctypes.c_int, ctypes.c_int]),

@@ -239,6 +239,15 @@ int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads, int* per_cu) {
   return LB_OK;
 }
 
+int lb_op_set_group_batch(lb_op* op, int groups) {
+  if (!op || groups < 1 || groups > kMaxGroupBatch) {
+    set_error("lb_op_set_group_batch: groups must be 1..16");
+    return LB_E_ARG;
+  }
+  op->group_batch = groups;
+  return LB_OK;
+}
+
 int lb_op_set_workgroups_per_cu(lb_op* op, int per_cu) {
   if (!op || per_cu < 1 || per_cu > 8) {
     set_error("lb_op_set_workgroups_per_cu: per_cu must be 1..8");

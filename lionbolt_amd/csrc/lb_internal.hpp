@@ -81,6 +81,9 @@ struct SweepTune {
   int choice = -1;                       // index into the config table once settled
 };
 
+// Most energy groups solved together by the group-batched multigroup solve.
+constexpr int kMaxGroupBatch = 16;
+
 // Contiguous (problem, row) ranges of a [rows][NENK] vector, split into tiles, for
 // the per-problem reductions and updates of batched solves.
 struct SegTable {
@@ -148,6 +151,7 @@ struct lb_op {
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
   int dir_block = 1024;             // workgroup size of the direction-resident and persistent sweeps
   int per_cu = 2;                   // persistent sweep: resident workgroups per compute unit
+  int group_batch = 1;              // energy groups per coupled solve in the multigroup driver
   lb::BatchTable batch;             // independent problems (set by lb_op_set_problems)
   std::map<int, lb::SweepTune> btune;  // autotuner state of batched sweeps, per group-batch size
   std::vector<int32_t> h_nbr;       // int32 [NE][4] neighbours (-1 boundary)
@@ -191,6 +195,14 @@ int launch_scale(hipStream_t s, int64_t n, double a, double* x);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int launch_fluence(lb_op* op, const double* v, double* phi);
+
+// lb_groups.hip (group-batched multigroup solve)
+int launch_batch_source(lb_op* op, const double* phi_all, const double* S, const double* qmom, int g0, int B,
+                        double* acc);
+int launch_batch_transfer(lb_op* op, const double* S, int g0, int B, const double* v, double* out);
+int launch_batch_store(lb_op* op, const double* Phi, int g0, int B, double* phi_all);
+int launch_group_scale(lb_op* op, int64_t n, int B, const double* w, double* x);
+int group_dot(lb_op* op, int64_t n, int B, const double* x, const double* y, double* out);
 
 // lb_mlp.hip
 int mlp_set(lb_op* op, const float* params, int nparams, int fmt);

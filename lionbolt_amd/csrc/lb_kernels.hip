@@ -739,8 +739,8 @@ int launch_copy(hipStream_t s, int64_t n, const double* x, double* y) {
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   LB_TRY(op->w_red.reserve((kDotBlocks + 8) * sizeof(double)));
   if (!op->w_host.p) {
-    LB_HIP(hipHostMalloc(&op->w_host.p, 64, hipHostMallocDefault));
-    op->w_host.bytes = 64;
+    LB_HIP(hipHostMalloc(&op->w_host.p, kMaxGroupBatch * sizeof(double), hipHostMallocDefault));
+    op->w_host.bytes = kMaxGroupBatch * sizeof(double);
   }
   double* part = op->w_red.as<double>();
   double* res = part + kDotBlocks;

@@ -109,6 +109,10 @@ class Operator:
         name = {v: k for k, v in self._SWEEP_MODES.items()}[mode.value]
         return name, threads.value, per_cu.value
 
+    def set_group_batch(self, groups):
+        """Energy groups solved together per coupled system in ``multigroup_solve_dev``."""
+        _lib.check(_lib.lib().lb_op_set_group_batch(self.h, int(groups)))
+
     def set_workgroups_per_cu(self, per_cu):
         _lib.check(_lib.lib().lb_op_set_workgroups_per_cu(self.h, int(per_cu)))
 
