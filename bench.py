@@ -547,8 +547,8 @@ def main():
     p.add_argument("--no-solve", dest="solve", action="store_false")
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
-    p.add_argument("--group-batches", type=lambda t: [int(x) for x in t.split(",")], default=[1, 4, 8],
-                   help="energy groups per coupled solve to try in the solve leg")
+    p.add_argument("--group-batches", type=lambda t: [int(x) for x in t.split(",")], default=[1],
+                   help="energy groups per coupled solve to try in the solve leg (e.g. 1,2,4)")
     p.add_argument("--no-ref-method", dest="ref_method", action="store_false",
                    help="skip the reference-method (psi-space GMRES(10)) solve on the GPU")
     p.add_argument("--no-cfg1", dest="cfg1", action="store_false")
