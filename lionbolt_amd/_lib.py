@@ -115,6 +115,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                               "(make -C lionbolt_amd/csrc). There is no CPU fallback.")
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+        # Loaded first, it is the one the library binds to, so torch tensors and the
+        # library share one runtime; loaded after the library it would find the
+        # system runtime already in place and fail to initialise.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
