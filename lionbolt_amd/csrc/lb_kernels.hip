@@ -46,7 +46,6 @@ struct SweepArgs {
   int64_t NE;
   const double* q;   // may alias psi (in-place sweep)
   double* psi;
-  int stream;        // read records, c_f k.n and source as streaming (non-temporal) data
 };
 
 // With many groups per cell the records and sources are read once per sweep and
@@ -57,8 +56,9 @@ constexpr int kStreamGroups = 8;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ TaskRec load_rec(const TaskRec* p, bool stream) {
-  if (!stream) return *p;
+template <bool NT>
+__device__ __forceinline__ TaskRec load_rec(const TaskRec* p) {
+  if (!NT) return *p;
   const u32x4* q = reinterpret_cast<const u32x4*>(p);
   const u32x4 a = __builtin_nontemporal_load(q);
   const u32x4 b = __builtin_nontemporal_load(q + 1);
@@ -71,8 +71,9 @@ __device__ __forceinline__ TaskRec load_rec(const TaskRec* p, bool stream) {
   return r;
 }
 
-__device__ __forceinline__ double4 load_ck(const double4* p, bool stream) {
-  if (!stream) return *p;
+template <bool NT>
+__device__ __forceinline__ double4 load_ck(const double4* p) {
+  if (!NT) return *p;
   const double* d = reinterpret_cast<const double*>(p);
   return make_double4(__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1),
                       __builtin_nontemporal_load(d + 2), __builtin_nontemporal_load(d + 3));
@@ -89,11 +90,11 @@ struct CellPrep {
   uint32_t pm;
 };
 
+template <bool NT>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   const int64_t k = (int64_t)i * A.NE + s;
-  const bool st = A.stream != 0;
-  const TaskRec r = load_rec(A.rec + k, st);
-  const double4 c = load_ck(A.ck + k, st);
+  const TaskRec r = load_rec<NT>(A.rec + k);
+  const double4 c = load_ck<NT>(A.ck + k);
   P.w = r.w;
   P.ck[0] = c.x;
   P.ck[1] = c.y;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int 
   const int e = int(r.w & 0xFFFFFFu);
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) P.b[a] = st ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
+  for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
 }
 
 // The dependent part: upstream traces, local matrix, LU solve, store.
@@ -222,9 +223,10 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
 }
 
+template <bool NT>
 __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g) {
   CellPrep P;
-  prep_cell(A, i, s, g, P);
+  prep_cell<NT>(A, i, s, g, P);
   finish_cell(A, i, P, g);
 }
 
@@ -258,27 +260,28 @@ __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t N
   ck[k] = make_double4(v[0], v[1], v[2], v[3]);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restrict__ work, SweepArgs A) {
   const int4 wd = work[blockIdx.x];
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   if (c >= wd.z) return;
-  solve_cell(A, wd.x, wd.y + c, g);
+  solve_cell<NT>(A, wd.x, wd.y + c, g);
 }
 
 // Static part of the cells this lane sweeps in one level: up to R rounds of
 // BLOCK / G cells each.
-template <int R>
+template <int R, bool NT>
 __device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                            bool lane_ok, CellPrep (&P)[R]) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
-    if (lane_ok && s < s1) prep_cell(A, i, s, g, P[r]);
+    if (lane_ok && s < s1) prep_cell<NT>(A, i, s, g, P[r]);
   }
 }
 
-template <int R>
+template <int R, bool NT>
 __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                              bool lane_ok, const CellPrep (&P)[R]) {
   if (!lane_ok) return;
@@ -287,7 +290,7 @@ __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, 
     const int s = s0 + c + r * cpr;
     if (s < s1) finish_cell(A, i, P[r], g);
   }
-  for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell(A, i, s, g);   // levels beyond R rounds
+  for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT>(A, i, s, g);   // levels beyond R rounds
 }
 
 // Walks the levels of one direction (positions off[0..nl]) with the whole
@@ -296,21 +299,21 @@ __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, 
 // at the start of level l into the second of two register buffers, so its
 // latency overlaps level l's upstream loads and solves; only the upstream-trace
 // loads of a level wait for the barrier.
-template <int BLOCK, int R>
+template <int BLOCK, int R, bool NT>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl) {
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
   CellPrep Pa[R], Pb[R];
-  if (nl > 0) prep_level<R>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
+  if (nl > 0) prep_level<R, NT>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
   for (int l = 0; l < nl; l += 2) {
-    if (l + 1 < nl) prep_level<R>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
-    finish_level<R>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa);
+    if (l + 1 < nl) prep_level<R, NT>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    finish_level<R, NT>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa);
     __syncthreads();
     if (l + 1 >= nl) break;
-    if (l + 2 < nl) prep_level<R>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
-    finish_level<R>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    if (l + 2 < nl) prep_level<R, NT>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
+    finish_level<R, NT>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
     __syncthreads();
   }
 }
@@ -328,12 +331,12 @@ struct LevelRounds {
 // enough: no inter-workgroup flags, no agent-scope cache write-back or
 // invalidation, and the traces of the last few levels are re-read from the
 // XCD's L2.  One launch sweeps every direction.
-template <int BLOCK>
+template <int BLOCK, bool NT>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* __restrict__ lvl_off,
                                                                 const int32_t* __restrict__ nlev, int maxlev,
                                                                 SweepArgs A) {
   const int i = blockIdx.x;
-  walk_levels<BLOCK, LevelRounds<BLOCK>::value>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+  walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
 // levels like the direction-resident kernel.  Items of inactive problems
 // (converged in a batched solve) are skipped.  Every workgroup leaves the loop
 // once the counter passes the item count.
-template <int BLOCK>
+template <int BLOCK, bool NT>
 __global__ __launch_bounds__(BLOCK) void sweep_persistent_kernel(const int4* __restrict__ items, int nitems,
                                                                  const int32_t* __restrict__ lvl,
                                                                  const int32_t* __restrict__ active,
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_persistent_kernel(const int4* __r
     if (it >= nitems) break;
     const int4 d = items[it];
     if (active[d.x] == 0) continue;
-    walk_levels<BLOCK, 1>(A, d.y, lvl + d.z, d.w);   // one round: several workgroups share a CU
+    walk_levels<BLOCK, 1, NT>(A, d.y, lvl + d.z, d.w);   // one round: several workgroups share a CU
   }
 }
 
@@ -520,6 +523,44 @@ inline unsigned grid_stride(int64_t n) {
 
 }  // namespace
 
+template <bool NT>
+static void launch_direction_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
+                                int maxlev, const SweepArgs& A) {
+  if (block == 256)
+    hipLaunchKernelGGL((sweep_direction_kernel<256, NT>), dim3(ndir), dim3(256), 0, st, lo, nl, maxlev, A);
+  else if (block == 512)
+    hipLaunchKernelGGL((sweep_direction_kernel<512, NT>), dim3(ndir), dim3(512), 0, st, lo, nl, maxlev, A);
+  else
+    hipLaunchKernelGGL((sweep_direction_kernel<1024, NT>), dim3(ndir), dim3(1024), 0, st, lo, nl, maxlev, A);
+}
+
+static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
+                             int maxlev, const SweepArgs& A) {
+  if (nt)
+    launch_direction_nt<true>(block, ndir, st, lo, nl, maxlev, A);
+  else
+    launch_direction_nt<false>(block, ndir, st, lo, nl, maxlev, A);
+}
+
+template <bool NT>
+static void launch_persistent_nt(int block, dim3 grid, hipStream_t st, const int4* items, int nitems,
+                                 const int32_t* lvl, const int32_t* act, unsigned int* ctr, const SweepArgs& A) {
+  if (block == 256)
+    hipLaunchKernelGGL((sweep_persistent_kernel<256, NT>), grid, dim3(256), 0, st, items, nitems, lvl, act, ctr, A);
+  else if (block == 512)
+    hipLaunchKernelGGL((sweep_persistent_kernel<512, NT>), grid, dim3(512), 0, st, items, nitems, lvl, act, ctr, A);
+  else
+    hipLaunchKernelGGL((sweep_persistent_kernel<1024, NT>), grid, dim3(1024), 0, st, items, nitems, lvl, act, ctr, A);
+}
+
+static void launch_persistent(int block, bool nt, dim3 grid, hipStream_t st, const int4* items, int nitems,
+                              const int32_t* lvl, const int32_t* act, unsigned int* ctr, const SweepArgs& A) {
+  if (nt)
+    launch_persistent_nt<true>(block, grid, st, items, nitems, lvl, act, ctr, A);
+  else
+    launch_persistent_nt<false>(block, grid, st, items, nitems, lvl, act, ctr, A);
+}
+
 // Records and c_f (k . n_f) of every (direction, sweep position) for a task order.
 static int ensure_records(lb_op* op, DevBuf& rec, DevBuf& ck, const DevBuf& tasks) {
   if (rec.p) return LB_OK;
@@ -594,7 +635,8 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     BatchTable& B = op->batch;
     LB_TRY(ensure_records(op, B.d_rec, B.d_ck, B.d_tasks));
     SweepArgs A{B.d_rec.as<const TaskRec>(), B.d_ck.as<const double4>(), op->d_sigt.as<const double>(),
-                op->G, g0, G, op->NE, q, psi, G >= kStreamGroups ? 1 : 0};
+                op->G, g0, G, op->NE, q, psi};
+    const bool nt = G >= kStreamGroups;
     const int4* items = B.d_items.as<const int4>();
     const int nitems = (int)(B.items.size() / 4);
     const int32_t* lvl = B.d_lvl.as<const int32_t>();
@@ -603,19 +645,14 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     const dim3 grid((unsigned)(B.ncu * cfg.per_cu));
     return replay_graph(op, q, psi, g0, G, cfg, [&]() {
       hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, op->stream, ctr);
-      if (block == 256)
-        hipLaunchKernelGGL(sweep_persistent_kernel<256>, grid, dim3(256), 0, op->stream, items, nitems, lvl, act, ctr, A);
-      else if (block == 512)
-        hipLaunchKernelGGL(sweep_persistent_kernel<512>, grid, dim3(512), 0, op->stream, items, nitems, lvl, act, ctr, A);
-      else
-        hipLaunchKernelGGL(sweep_persistent_kernel<1024>, grid, dim3(1024), 0, op->stream, items, nitems, lvl, act, ctr,
-                           A);
+      launch_persistent(block, nt, grid, op->stream, items, nitems, lvl, act, ctr, A);
     });
   }
 
   LB_TRY(ensure_records(op, op->d_rec, op->d_ck, op->d_tasks));
   SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
-              op->G, g0, G, op->NE, q, psi, G >= kStreamGroups ? 1 : 0};
+              op->G, g0, G, op->NE, q, psi};
+  const bool nt = G >= kStreamGroups;
   if (mode == LB_SWEEP_DIRECTION) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
@@ -627,12 +664,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     LB_HIP(hipEventRecord(op->ev0, op->stream));
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
-    if (block == 256)
-      hipLaunchKernelGGL(sweep_direction_kernel<256>, dim3(op->NI), dim3(256), 0, op->stream, lo, nl, op->maxlev, A);
-    else if (block == 512)
-      hipLaunchKernelGGL(sweep_direction_kernel<512>, dim3(op->NI), dim3(512), 0, op->stream, lo, nl, op->maxlev, A);
-    else
-      hipLaunchKernelGGL(sweep_direction_kernel<1024>, dim3(op->NI), dim3(1024), 0, op->stream, lo, nl, op->maxlev, A);
+    launch_direction(block, nt, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A);
     LB_HIP(hipGetLastError());
     LB_HIP(hipEventRecord(op->ev1, op->stream));
     return LB_OK;
@@ -646,7 +678,10 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     for (int l = 0; l < op->maxlev; ++l) {
       const int nb = t->level_off[l + 1] - t->level_off[l];
       if (nb == 0) continue;
-      hipLaunchKernelGGL(sweep_level_kernel, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
+      if (nt)
+        hipLaunchKernelGGL(sweep_level_kernel<true>, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
+      else
+        hipLaunchKernelGGL(sweep_level_kernel<false>, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
     }
   };
   if (mode == LB_SWEEP_GRAPH) return replay_graph(op, q, psi, g0, G, cfg, enqueue);
