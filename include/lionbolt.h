@@ -61,12 +61,16 @@ typedef struct {
   int index_base;        /* 1: Fortran indices, boundary neighbour 0; 0: C indices, boundary -1 */
 } lb_mesh_desc;
 
+/* Highest scattering order: PNSCATTERING 15, the highest the reference's
+ * input decks use (validation/electrons_incident/Au_1MeV_0deg/lb.in and its siblings). */
+#define LB_MAX_ORDER 15
+
 /* AngularClass fields the operators read (AngularSpace.f90:5-20). */
 typedef struct {
   int64_t NI;            /* directions */
   const double* k;       /* k(3,NI) unit propagation directions */
   const double* w;       /* w(NI) weights (sum 4 pi) */
-  int L;                 /* scattering order: YY(:,:,0:L) */
+  int L;                 /* scattering order: YY(:,:,0:L), 0 <= L <= LB_MAX_ORDER */
 } lb_quad_desc;
 
 typedef struct {

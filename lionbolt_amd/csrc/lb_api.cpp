@@ -120,9 +120,9 @@ int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out
     set_error("lb_op_create: null argument");
     return LB_E_ARG;
   }
-  if (mesh->NE <= 0 || mesh->NK <= 0 || quad->NI <= 0 || quad->L < 0 || quad->L > 7 ||
+  if (mesh->NE <= 0 || mesh->NK <= 0 || quad->NI <= 0 || quad->L < 0 || quad->L > LB_MAX_ORDER ||
       (mesh->index_base != 0 && mesh->index_base != 1)) {
-    set_error("lb_op_create: invalid sizes (need NE, NK, NI > 0, 0 <= L <= 7, index_base 0 or 1)");
+    set_error("lb_op_create: invalid sizes (need NE, NK, NI > 0, 0 <= L <= 15, index_base 0 or 1)");
     return LB_E_ARG;
   }
   int ndev = 0;

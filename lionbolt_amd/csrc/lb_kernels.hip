@@ -415,6 +415,57 @@ __global__ __launch_bounds__(kBlock) void moments_to_discrete_kernel(
   }
 }
 
+// Scattering orders above 7 ((L+1)^2 > 64 moments) do not fit one lane's
+// accumulators: the moments are split into chunks of kMomChunk, one grid row
+// of workgroups per chunk for D psi, and one accumulating pass per chunk for
+// M phi.
+constexpr int kMomChunk = 32;
+
+// phi[m0 + m][col] = sum_i D[m0 + m][i] psi[i][col], m < kMomChunk, m0 = blockIdx.y kMomChunk.
+__global__ __launch_bounds__(kBlock) void moments_chunk_kernel(const double* __restrict__ D, int NI, int nmom,
+                                                               int64_t ncols, const double* __restrict__ psi,
+                                                               double* __restrict__ phi) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= ncols) return;
+  const int m0 = (int)blockIdx.y * kMomChunk;
+  double acc[kMomChunk];
+#pragma unroll
+  for (int m = 0; m < kMomChunk; ++m) acc[m] = 0.0;
+  for (int i = 0; i < NI; ++i) {
+    const double v = psi[(int64_t)i * ncols + col];
+#pragma unroll
+    for (int m = 0; m < kMomChunk; ++m) acc[m] += D[(int64_t)min(m0 + m, nmom - 1) * NI + i] * v;
+  }
+#pragma unroll
+  for (int m = 0; m < kMomChunk; ++m)
+    if (m0 + m < nmom) phi[(int64_t)(m0 + m) * ncols + col] = acc[m];
+}
+
+// psi[i][col] (+)= sum_{m0 <= lm < m0 + kMomChunk} M[i][lm] sigma_{l(lm)}(mat, g) phi[lm][col].
+__global__ __launch_bounds__(kBlock) void moments_to_discrete_chunk_kernel(
+    const double* __restrict__ M, const int32_t* __restrict__ deg, int NI, int nmom, int m0, int64_t ncols, int G,
+    int Gtot, int g0, int L, const int32_t* __restrict__ mat, const double* __restrict__ sigs,
+    const double* __restrict__ phi, double* __restrict__ psi, int accumulate) {
+  const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (col >= ncols) return;
+  const int64_t sd = col / G;
+  const int g = int(col - sd * G);
+  const double* sl = sigs ? sigs + ((int64_t)mat[sd >> 2] * Gtot + g0 + g) * (L + 1) : nullptr;
+  double sp[kMomChunk];
+#pragma unroll
+  for (int m = 0; m < kMomChunk; ++m) {
+    const int lm = m0 + m;
+    sp[m] = lm < nmom ? phi[(int64_t)lm * ncols + col] * (sl ? sl[deg[lm]] : 1.0) : 0.0;
+  }
+  for (int i = 0; i < NI; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < kMomChunk; ++m) acc += M[(int64_t)i * nmom + min(m0 + m, nmom - 1)] * sp[m];
+    double* dst = psi + (int64_t)i * ncols + col;
+    *dst = accumulate ? *dst + acc : acc;
+  }
+}
+
 // Dense kernel of ScatteringAlgebra.f90:30-53: y[i][col] = sum_ip x[ip][col] YYS_m[ip][i],
 // m = material of the column; 16 outputs per pass over the column.
 __global__ __launch_bounds__(kBlock) void scatter_dense_kernel(const double* __restrict__ yys, int NI, int64_t ncols,
@@ -737,12 +788,22 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
     case 49: CALL(49); break;     \
     case 64: CALL(64); break;     \
     default:                      \
-      set_error("moments: scattering order above 7 is not supported"); \
+      set_error("moments: moment count is not (L+1)^2"); \
       return LB_E_ARG;            \
   }
 
+// Moment counts above this go through the chunked kernels.
+constexpr int kMaxRegisterMoments = 64;
+
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols) {
   (void)G;
+  if (op->nmom > kMaxRegisterMoments) {
+    const unsigned chunks = (unsigned)((op->nmom + kMomChunk - 1) / kMomChunk);
+    hipLaunchKernelGGL(moments_chunk_kernel, dim3(grid_for(ncols), chunks), dim3(kBlock), 0, op->stream,
+                       op->d_D.as<const double>(), (int)op->NI, (int)op->nmom, ncols, psi, phi);
+    LB_HIP(hipGetLastError());
+    return LB_OK;
+  }
 #define CALL(NMV)                                                                                                  \
   hipLaunchKernelGGL(moments_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream, op->d_D.as<double>(), \
                      (int)op->NI, ncols, psi, phi)
@@ -755,6 +816,15 @@ int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t nco
 int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate, bool scaled) {
   const int64_t ncols = op->NENK * G;
   const double* sigs = scaled ? op->d_sigs.as<const double>() : nullptr;
+  if (op->nmom > kMaxRegisterMoments) {
+    for (int m0 = 0; m0 < op->nmom; m0 += kMomChunk)
+      hipLaunchKernelGGL(moments_to_discrete_chunk_kernel, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream,
+                         op->d_M.as<const double>(), op->d_deg.as<const int32_t>(), (int)op->NI, (int)op->nmom, m0,
+                         ncols, G, op->G, g0, op->L, op->d_mat.as<const int32_t>(), sigs, phi, psi,
+                         (m0 > 0 || accumulate) ? 1 : 0);
+    LB_HIP(hipGetLastError());
+    return LB_OK;
+  }
 #define CALL(NMV)                                                                                                 \
   hipLaunchKernelGGL(moments_to_discrete_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream,       \
                      op->d_M.as<double>(), op->d_deg.as<const int32_t>(), (int)op->NI, ncols, G, op->G, g0, op->L,             \
