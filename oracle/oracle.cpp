@@ -691,6 +691,50 @@ void orc_scatter_matvec(void* h, double* x) {
   std::memcpy(x, xw, sizeof(double) * NENK * NI);  // xwp = xw
 }
 
+// K x in moment form, for the CPU baseline only (no reference routine: the
+// reference always applies the dense kernel above).  phi[lm][sd] =
+// sum_i D[lm][i] x[i][sd]; x[i][sd] <- sum_lm M[i][lm] sigma_{deg[lm]}(mat(sd))
+// phi[lm][sd].  With D = w Y and M = Y^T over real orthonormal harmonics this is
+// the dense kernel by the addition theorem; timing it keeps the O(NI^2) ->
+// O(NI nmom) change out of the GPU/CPU ratio.
+void orc_scatter_moment_matvec(void* h, const double* D /*[nmom][NI]*/, const double* M /*[NI][nmom]*/,
+                               const int32_t* deg, int nmom, double* x) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  const i64 NI = op->NI, NENK = op->NENK;
+  const int L1 = op->L + 1;
+  const i64 CH = 64;  // spatial dofs per block
+#pragma omp parallel
+  {
+    std::vector<double> phi((size_t)nmom * CH), y((size_t)NI * CH);
+#pragma omp for schedule(static)
+    for (i64 s0 = 0; s0 < NENK; s0 += CH) {
+      const i64 nc = std::min(CH, NENK - s0);
+      std::fill(phi.begin(), phi.end(), 0.0);
+      for (i64 i = 0; i < NI; ++i) {
+        const double* xi = x + i * NENK + s0;
+        for (int lm = 0; lm < nmom; ++lm) {
+          const double d = D[lm * NI + i];
+          double* p = &phi[(size_t)lm * CH];
+          for (i64 c = 0; c < nc; ++c) p[c] += d * xi[c];
+        }
+      }
+      for (int lm = 0; lm < nmom; ++lm)
+        for (i64 c = 0; c < nc; ++c)
+          phi[(size_t)lm * CH + c] *= op->sigs[(op->mat[(s0 + c) >> 2] - 1) * L1 + deg[lm]];
+      for (i64 i = 0; i < NI; ++i) {
+        double* yi = &y[(size_t)i * CH];
+        std::fill(yi, yi + CH, 0.0);
+        for (int lm = 0; lm < nmom; ++lm) {
+          const double mm = M[i * nmom + lm];
+          const double* p = &phi[(size_t)lm * CH];
+          for (i64 c = 0; c < nc; ++c) yi[c] += mm * p[c];
+        }
+      }
+      for (i64 i = 0; i < NI; ++i) std::memcpy(x + i * NENK + s0, &y[(size_t)i * CH], sizeof(double) * nc);
+    }
+  }
+}
+
 // BoltzmannOp%MatVec (BoltzmannOperator.f90:59-104), Tprecond: x <- x - T^{-1} K x.
 void orc_boltzmann_matvec(void* h, double* x) {
   OrcOp* op = static_cast<OrcOp*>(h);

@@ -52,6 +52,7 @@ def lib():
         L.orc_boltzmann_matinv.restype = ctypes.c_int
         L.orc_boltzmann_matinv.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_int, _d, ctypes.c_int, _ci, _ci]
+        L.orc_scatter_moment_matvec.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int, _d]
         L.orc_fluence.argtypes = [ctypes.c_void_p, _d, _d]
         L.orc_norm2.restype = ctypes.c_double
         L.orc_norm2.argtypes = [_d, ctypes.c_int64]
@@ -130,6 +131,16 @@ class OracleOp:
     def scatter_matvec(self, s):
         x = self._vec(s)
         lib().orc_scatter_matvec(self.h, _p(x))
+        return x
+
+    def scatter_moment_matvec(self, s, D, M, deg):
+        """K x through the flux moments (the CPU baseline's moment form)."""
+        x = self._vec(s)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        deg = np.ascontiguousarray(deg, dtype=np.int32)
+        assert D.shape == (deg.size, self.NI) and M.shape == (self.NI, deg.size)
+        lib().orc_scatter_moment_matvec(self.h, _p(D), _p(M), deg.ctypes.data_as(_i32), deg.size, _p(x))
         return x
 
     def boltzmann_matvec(self, s):

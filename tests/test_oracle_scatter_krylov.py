@@ -63,6 +63,22 @@ def test_two_material_scattering(small_mesh):
     assert np.abs(out - expect).max() <= 1e-13 * np.abs(expect).max()
 
 
+def test_moment_form_equals_dense_kernel(small_mesh):
+    """The CPU baseline's moment-form K against the dense kernel and the einsum
+    restatement (two materials, P3)."""
+    m = M.TetMesh(small_mesh.rg, small_mesh.node, np.where(small_mesh.centroids()[:, 2] > 1.0, 2, 1))
+    ang = angular.snprep(3)
+    op = OracleOp(m, ang)
+    sigs = np.array([[0.5, 0.2, 0.1, 0.05], [2.0, 0.7, 0.3, 0.1]])
+    op.set_physics([1.0, 3.0], sigs)
+    D, Mm = angular.moment_matrices(ang)
+    deg = angular.moment_degree(ang.L)
+    psi = np.random.default_rng(3).standard_normal(op.n)
+    dense = op.scatter_matvec(psi)
+    mom = op.scatter_moment_matvec(psi, D, Mm, deg)
+    assert np.abs(mom - dense).max() <= 1e-13 * np.abs(dense).max()
+
+
 def _scatter_moment(l):
     if l >= 14:
         return 0.0
