@@ -50,9 +50,13 @@ def test_create_without_device_fails_loudly():
     assert rc == -4 and b"device" in _lib.lib().lb_last_error()
 
 
-@pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cfg1_s8"])
+@pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cfg1_s8", "grazing_s8"])
 def test_host_schedule_and_normals_bit_identical_to_oracle(case, small_mesh, cfg1_mesh):
-    if case == "small_s8":
+    if case == "grazing_s8":
+        # unjittered Kuhn cube: S8's mu = eta directions are exactly parallel to
+        # the (1,-1,0)/sqrt2 faces (k.n == 0 on 6,144 interior face-directions)
+        m, ang = M.jittered_cube(4, a=2.0, jitter=0.0), angular.level_symmetric_s8()
+    elif case == "small_s8":
         m, ang = small_mesh, angular.level_symmetric_s8()
     elif case == "small_snprep3":
         m, ang = small_mesh, angular.snprep(3)

@@ -62,6 +62,39 @@ def test_sweep_parity(ops_module, n, angname):
     assert rel(a, b) <= 1e-12
 
 
+def test_grazing_faces_sweep_and_solve(ops_module):
+    """Unjittered cube with S8: k.n == 0 exactly on many interior faces, which
+    drop out of both the schedule and the diagonal (SweepOrder.f90:126,
+    TransportAlgebra.f90:150)."""
+    m = M.jittered_cube(4, a=2.0, jitter=0.0)
+    ang = angular.level_symmetric_s8(1)
+    lib_op, orc = make_pair(ops_module, m, ang, [1.5], [[0.8, 0.3]])
+    rng = np.random.default_rng(21)
+    q = rng.standard_normal(lib_op.n)
+    assert rel(lib_op.transport_matinv(q), orc.transport_matinv(q)) <= 1e-12
+    q = rng.random(lib_op.n)
+    a, st = lib_op.boltzmann_matinv(q, kind="gmres", space="psi", m=10)
+    b, ost = orc.boltzmann_matinv(q, kind="gmres", m=10)
+    assert st["status"] == 0 and st["iterates"] == ost["iters"]
+    assert rel(a, b) <= 1e-7
+
+
+def test_single_tet(ops_module):
+    """One element: every face a boundary face, one level per direction."""
+    rg = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 1.0, 0.1], [0.1, 0.3, 1.2]])
+    node = np.array([[0, 1, 2, 3]])
+    m = M.TetMesh(rg, node)
+    ang = angular.snprep(2)
+    lib_op, orc = make_pair(ops_module, m, ang, [2.0], [[1.0, 0.4, 0.1]])
+    sl, nlev = lib_op.schedule()
+    assert np.all(nlev == 1) and np.all(sl == 0)
+    q = np.random.default_rng(5).random(lib_op.n)
+    assert rel(lib_op.transport_matinv(q), orc.transport_matinv(q)) <= 1e-12
+    a, st = lib_op.boltzmann_matinv(q, kind="si")
+    b, ost = orc.boltzmann_matinv(q, kind="si")
+    assert st["status"] == 0 and st["iterates"] == ost["iters"] and rel(a, b) <= 1e-7
+
+
 def test_sweep_cfg1_source(ops_module, cfg1_mesh):
     m = cfg1_mesh
     ang = angular.level_symmetric_s8()
