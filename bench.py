@@ -381,16 +381,17 @@ def single_group_leg(m, ang, reps=10):
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
     res = {}
-    for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "auto"):
+    for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "direction_lds-256",
+                 "direction_lds-512", "auto"):
         if mode == "auto":
             op.set_sweep_mode("auto")
-            for _ in range(16):
+            for _ in range(32):
                 if op.sweep_choice(1)[0] != "auto":
                     break
                 op.transport_matinv_dev(q, psi, 0, 1)
         elif mode.startswith("direction"):
             op.set_direction_block(int(mode.split("-")[1]) if "-" in mode else 1024)
-            op.set_sweep_mode("direction")
+            op.set_sweep_mode(mode.split("-")[0])
         else:
             op.set_sweep_mode(mode)
         op.transport_matinv_dev(q, psi, 0, 1)
@@ -432,7 +433,7 @@ def run(args):
     mode, block = args.sweep_mode, 0
     if args.sweep_mode == "auto":
         # the autotuner times each launch configuration on its first sweeps (untimed here)
-        for _ in range(16):
+        for _ in range(32):
             if op.sweep_choice(G)[0] != "auto":
                 break
             op.transport_matinv_dev(q, psi, 0, G)
@@ -472,8 +473,9 @@ def run(args):
             op.synchronize()
             other_modes[other] = {"ms_per_sweep": 1000.0 * (time.perf_counter() - t1) / 3}
     levels = op.maxlev
-    launches = 1 if mode == "direction" else levels
-    kernel = f"sweep_direction_kernel<{block}>" if mode == "direction" else "sweep_level_kernel"
+    launches = 1 if mode.startswith("direction") else levels
+    kernel = {"direction": f"sweep_direction_kernel<{block}>",
+              "direction_lds": f"sweep_direction_lds_kernel<{block}>"}.get(mode, "sweep_level_kernel")
     op.close()
     del q, psi
     torch.cuda.empty_cache()

@@ -45,7 +45,14 @@ enum {
 enum { LB_SOLVER_GMRES = 0, LB_SOLVER_SI = 1, LB_SOLVER_FGMRES = 2 };
 enum { LB_SPACE_PSI = 0, LB_SPACE_MOMENT = 1 };
 enum { LB_SCATTER_MOMENT = 0, LB_SCATTER_DENSE = 1 };
-enum { LB_SWEEP_LEVELS = 0, LB_SWEEP_GRAPH = 1, LB_SWEEP_DIRECTION = 2, LB_SWEEP_AUTO = 3, LB_SWEEP_PERSISTENT = 4 };
+enum {
+  LB_SWEEP_LEVELS = 0,
+  LB_SWEEP_GRAPH = 1,
+  LB_SWEEP_DIRECTION = 2,
+  LB_SWEEP_AUTO = 3,
+  LB_SWEEP_PERSISTENT = 4,
+  LB_SWEEP_DIRECTION_LDS = 5
+};
 
 typedef struct lb_op lb_op;
 
@@ -117,16 +124,20 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
  * on the operator's stream; LB_SWEEP_GRAPH captures those launches once per
  * (source, flux, group range) into a HIP graph and replays it;
  * LB_SWEEP_DIRECTION runs one kernel in which each workgroup owns one
- * direction and walks its levels with workgroup barriers; LB_SWEEP_AUTO
- * (default) times the graph mode and the direction-resident mode at 256, 512
- * and 1024 threads on the first sweeps of each group-batch size (two sweeps
+ * direction and walks its levels with workgroup barriers;
+ * LB_SWEEP_DIRECTION_LDS does the same and also keeps the flux of the
+ * direction's most recent sweep positions in LDS, reading upstream traces from
+ * there when they are still held (global memory otherwise); LB_SWEEP_AUTO
+ * (default) times the graph mode, the direction-resident mode at 256, 512 and
+ * 1024 threads and the LDS one at 256 and 512 threads on the first sweeps of
+ * each group-batch size (two sweeps
  * each, synchronising after each) and then keeps the fastest.  All modes give
  * bit-identical results.  No reference counterpart (the reference sweeps one
  * direction per OpenMP thread). */
 int lb_op_set_sweep_mode(lb_op* op, int mode);
 
 /* The launch configuration LB_SWEEP_AUTO settled on for a group batch of G:
- * *mode is LB_SWEEP_GRAPH, LB_SWEEP_DIRECTION or LB_SWEEP_PERSISTENT (with
+ * *mode is LB_SWEEP_GRAPH, LB_SWEEP_DIRECTION, LB_SWEEP_DIRECTION_LDS or LB_SWEEP_PERSISTENT (with
  * *threads its workgroup size and *per_cu the persistent grid's workgroups per
  * compute unit), or LB_SWEEP_AUTO while that batch size is still being timed. */
 int lb_op_sweep_choice(lb_op* op, int G, int* mode, int* threads, int* per_cu);

@@ -15,7 +15,7 @@ ERRORS = {-1: "LB_E_ARG", -2: "LB_E_STAGNANT", -3: "LB_E_NOT_READY", -4: "LB_E_H
 SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
-SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT = 0, 1, 2, 3, 4
+SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT, SWEEP_DIRECTION_LDS = 0, 1, 2, 3, 4, 5
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)

@@ -208,7 +208,7 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 }
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
-  if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_PERSISTENT) {
+  if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_DIRECTION_LDS) {
     set_error("lb_op_set_sweep_mode: invalid mode");
     return LB_E_ARG;
   }
@@ -523,6 +523,7 @@ void lb_op_destroy(lb_op* op) {
   op->batch.h_io = nullptr;
   op->d_lvl_off.release();
   op->d_nlev.release();
+  op->d_upos.release();
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);
   if (op->stream) (void)hipStreamDestroy(op->stream);

@@ -66,9 +66,11 @@ struct SweepGraph {
 struct SweepConfig {
   int mode, block, per_cu;
 };
-constexpr int kTuneConfigs = 4;
+// (The LDS sweep at 1024 threads spills registers and is not timed.)
+constexpr int kTuneConfigs = 6;
 constexpr SweepConfig kTuneConfig[kTuneConfigs] = {
-    {LB_SWEEP_GRAPH, 0, 0}, {LB_SWEEP_DIRECTION, 256, 0}, {LB_SWEEP_DIRECTION, 512, 0}, {LB_SWEEP_DIRECTION, 1024, 0}};
+    {LB_SWEEP_GRAPH, 0, 0},        {LB_SWEEP_DIRECTION, 256, 0},     {LB_SWEEP_DIRECTION, 512, 0},
+    {LB_SWEEP_DIRECTION, 1024, 0}, {LB_SWEEP_DIRECTION_LDS, 256, 0}, {LB_SWEEP_DIRECTION_LDS, 512, 0}};
 constexpr int kBatchTuneConfigs = 8;
 constexpr SweepConfig kBatchTuneConfig[kBatchTuneConfigs] = {
     {LB_SWEEP_GRAPH, 0, 0},         {LB_SWEEP_PERSISTENT, 256, 4},  {LB_SWEEP_PERSISTENT, 256, 8},
@@ -149,6 +151,9 @@ struct lb_op {
   lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
   lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
+  lb::DevBuf d_upos;                // int32 [NI][NE][4]: sweep position of the upstream neighbour across
+                                    // each face in that direction's order, -1 where the face is not upstream
+  int lds_max = 0;                  // LDS bytes a workgroup may allocate (queried once)
   int dir_block = 1024;             // workgroup size of the direction-resident and persistent sweeps
   int per_cu = 2;                   // persistent sweep: resident workgroups per compute unit
   int group_batch = 1;              // energy groups per coupled solve in the multigroup driver

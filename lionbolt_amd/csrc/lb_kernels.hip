@@ -46,6 +46,18 @@ struct SweepArgs {
   int64_t NE;
   const double* q;   // may alias psi (in-place sweep)
   double* psi;
+  const int4* __restrict__ upos = nullptr;   // LDS sweep: upstream neighbour positions per face
+};
+
+// Trace ring of the direction-resident LDS sweep: the flux of the direction's
+// last `mask + 1` sweep positions, laid out [slot][node][group], slot = position
+// & mask.  While level [s0, s1) is solved it writes the slots of positions
+// [s0 - slots, s1 - slots), so an upstream position p (< s0) is still held iff
+// p >= s1 - slots; older ones are read from global memory.
+struct TraceRing {
+  double* buf;
+  int mask;
+  int lim;
 };
 
 // With many groups per cell the records and sources are read once per sweep and
@@ -88,12 +100,18 @@ struct CellPrep {
   double b[4];
   int4 nb;
   uint32_t pm;
+  int s;      // LDS sweep only: sweep position and upstream positions
+  int4 us;
 };
 
-template <bool NT>
+template <bool NT, bool LDS = false>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   const int64_t k = (int64_t)i * A.NE + s;
   const TaskRec r = load_rec<NT>(A.rec + k);
+  if constexpr (LDS) {
+    P.s = s;
+    P.us = A.upos[k];
+  }
   const double4 c = load_ck<NT>(A.ck + k);
   P.w = r.w;
   P.ck[0] = c.x;
@@ -109,8 +127,50 @@ __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int 
   for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
 }
 
-// The dependent part: upstream traces, local matrix, LU solve, store.
-__device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const CellPrep& P, int g) {
+// Where a cell reads its upstream traces and what it writes besides the flux.
+enum TraceSrc { kTraceGlobal = 0, kTraceGlobalRing = 1, kTraceRing = 2 };
+
+// Upstream face traces of a cell: u[f][j] = the neighbour's flux at the j-th node
+// of local face f, for the upstream faces only (from the ring or global memory).
+template <int SRC>
+__device__ __forceinline__ void load_upstream(const SweepArgs& A, int i, const CellPrep& P, int g,
+                                              const TraceRing& ring, double (&u)[4][3]) {
+  const unsigned up = (P.w >> 24) & 0xFu;
+  if (up) {
+    const int nbv[4] = {P.nb.x, P.nb.y, P.nb.z, P.nb.w};
+    int usv[4] = {0, 0, 0, 0};
+    if constexpr (SRC == kTraceRing) {
+      usv[0] = P.us.x;
+      usv[1] = P.us.y;
+      usv[2] = P.us.z;
+      usv[3] = P.us.w;
+    }
+    const uint32_t pm = P.pm;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (up >> f & 1u) {
+        const unsigned n0 = (pm >> (6 * f)) & 3u, n1 = (pm >> (6 * f + 2)) & 3u, n2 = (pm >> (6 * f + 4)) & 3u;
+        if constexpr (SRC == kTraceRing) {
+          const double* rb = ring.buf + (int64_t)(usv[f] & ring.mask) * 4 * A.G + g;
+          u[f][0] = rb[n0 * A.G];
+          u[f][1] = rb[n1 * A.G];
+          u[f][2] = rb[n2 * A.G];
+        } else {
+          const int64_t ub = ((int64_t)i * A.NE + nbv[f]) * 4 * A.G + g;
+          u[f][0] = A.psi[ub + n0 * A.G];
+          u[f][1] = A.psi[ub + n1 * A.G];
+          u[f][2] = A.psi[ub + n2 * A.G];
+        }
+      }
+    }
+  }
+}
+
+// The rest of a cell's solve once its traces are loaded: upstream coupling,
+// local matrix, LU solve, store.
+template <int SRC>
+__device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, const CellPrep& P, int g,
+                                                    const TraceRing& ring, const double (&u)[4][3]) {
   const uint32_t w = P.w;
   const int e = int(w & 0xFFFFFFu);
   const unsigned up = (w >> 24) & 0xFu;
@@ -123,23 +183,16 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
   // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
   // node, u = neighbour values at the shared nodes, U their sum.
-  if (up) {
-    const int nbv[4] = {P.nb.x, P.nb.y, P.nb.z, P.nb.w};
-    const uint32_t pm = P.pm;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      if (up >> f & 1u) {
-        const int64_t ub = ((int64_t)i * A.NE + nbv[f]) * 4 * A.G + g;
-        const double u0 = A.psi[ub + ((pm >> (6 * f)) & 3u) * A.G];
-        const double u1 = A.psi[ub + ((pm >> (6 * f + 2)) & 3u) * A.G];
-        const double u2 = A.psi[ub + ((pm >> (6 * f + 4)) & 3u) * A.G];
-        const double U = u0 + u1 + u2;
-        const double coef = -P.ck[f];
-        b[kFaceNodes[f][0]] += coef * (u0 + 0.2 * U);
-        b[kFaceNodes[f][1]] += coef * (u1 + 0.2 * U);
-        b[kFaceNodes[f][2]] += coef * (u2 + 0.2 * U);
-        b[3 - f] += coef * (-0.8 * U);
-      }
+  for (int f = 0; f < 4; ++f) {
+    if (up >> f & 1u) {
+      const double u0 = u[f][0], u1 = u[f][1], u2 = u[f][2];
+      const double U = u0 + u1 + u2;
+      const double coef = -P.ck[f];
+      b[kFaceNodes[f][0]] += coef * (u0 + 0.2 * U);
+      b[kFaceNodes[f][1]] += coef * (u1 + 0.2 * U);
+      b[kFaceNodes[f][2]] += coef * (u2 + 0.2 * U);
+      b[3 - f] += coef * (-0.8 * U);
     }
   }
 
@@ -221,13 +274,52 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
+  if constexpr (SRC != kTraceGlobal) {
+    double* rb = ring.buf + (int64_t)(P.s & ring.mask) * 4 * A.G + g;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rb[a * A.G] = x[a];
+  }
 }
 
-template <bool NT>
-__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g) {
+template <int SRC>
+__device__ __forceinline__ void finish_cell_body(const SweepArgs& A, int i, const CellPrep& P, int g,
+                                                 const TraceRing& ring) {
+  double u[4][3];
+  load_upstream<SRC>(A, i, P, g, ring, u);
+  solve_with_upstream<SRC>(A, i, P, g, ring, u);
+}
+
+__device__ __forceinline__ bool ring_hit(const CellPrep& P, const TraceRing& ring) {
+  const unsigned up = (P.w >> 24) & 0xFu;
+  return (!(up & 1u) || P.us.x >= ring.lim) && (!(up & 2u) || P.us.y >= ring.lim) &&
+         (!(up & 4u) || P.us.z >= ring.lim) && (!(up & 8u) || P.us.w >= ring.lim);
+}
+
+// With the trace ring, a cell whose upstream traces are all still held reads
+// them from LDS only; any other reads them all from global memory.  The two
+// paths are separate to the end of the cell, so a wave whose cells all hit
+// never waits on the vector-memory counter, which would also wait for the
+// look-ahead loads issued before it.  The device sweep order puts the hitting
+// cells of each level first, so most waves take one path.
+template <bool LDS = false>
+__device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const CellPrep& P, int g,
+                                            const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
+  if constexpr (LDS) {
+    if (ring_hit(P, ring))
+      finish_cell_body<kTraceRing>(A, i, P, g, ring);
+    else
+      finish_cell_body<kTraceGlobalRing>(A, i, P, g, ring);
+  } else {
+    finish_cell_body<kTraceGlobal>(A, i, P, g, ring);
+  }
+}
+
+template <bool NT, bool LDS = false>
+__device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g,
+                                           const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
   CellPrep P;
-  prep_cell<NT>(A, i, s, g, P);
-  finish_cell(A, i, P, g);
+  prep_cell<NT, LDS>(A, i, s, g, P);
+  finish_cell<LDS>(A, i, P, g, ring);
 }
 
 // Records and c_f (k . n_f) for every (direction, sweep position).
@@ -271,26 +363,27 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
 
 // Static part of the cells this lane sweeps in one level: up to R rounds of
 // BLOCK / G cells each.
-template <int R, bool NT>
+template <int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                            bool lane_ok, CellPrep (&P)[R]) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
-    if (lane_ok && s < s1) prep_cell<NT>(A, i, s, g, P[r]);
+    if (lane_ok && s < s1) prep_cell<NT, LDS>(A, i, s, g, P[r]);
   }
 }
 
-template <int R, bool NT>
+template <int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
-                                             bool lane_ok, const CellPrep (&P)[R]) {
+                                             bool lane_ok, const CellPrep (&P)[R],
+                                             const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
   if (!lane_ok) return;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
-    if (s < s1) finish_cell(A, i, P[r], g);
+    if (s < s1) finish_cell<LDS>(A, i, P[r], g, ring);
   }
-  for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT>(A, i, s, g);   // levels beyond R rounds
+  for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
 }
 
 // Walks the levels of one direction (positions off[0..nl]) with the whole
@@ -299,21 +392,85 @@ __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, 
 // at the start of level l into the second of two register buffers, so its
 // latency overlaps level l's upstream loads and solves; only the upstream-trace
 // loads of a level wait for the barrier.
-template <int BLOCK, int R, bool NT>
-__device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl) {
+template <int BLOCK, int R, bool NT, bool LDS = false>
+__device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
+                                            TraceRing ring = TraceRing{nullptr, 0, 0}) {
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
+  const int slots = ring.mask + 1;
   CellPrep Pa[R], Pb[R];
-  if (nl > 0) prep_level<R, NT>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
+  if (nl > 0) prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
   for (int l = 0; l < nl; l += 2) {
-    if (l + 1 < nl) prep_level<R, NT>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
-    finish_level<R, NT>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa);
+    if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    ring.lim = off[l + 1] - slots;
+    finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
     __syncthreads();
     if (l + 1 >= nl) break;
-    if (l + 2 < nl) prep_level<R, NT>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
-    finish_level<R, NT>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
+    ring.lim = off[l + 2] - slots;
+    finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
+    __syncthreads();
+  }
+}
+
+// One level in upstream-first order: the trace loads of the level's cells are
+// issued before the look-ahead loads of the next level, so that waiting for the
+// traces (the vector-memory counter retires in order) does not also wait for
+// the look-ahead, which then completes while the cells are solved.  The
+// look-ahead issues the same loads on every path (lanes past the next level's
+// end re-read its last cell), which keeps the compiler's counter waits exact.
+template <int R, bool NT, bool LDS>
+__device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, int s0, int s1, int n0, int n1,
+                                                     int cpr, int c, int g, bool lane_ok, const CellPrep (&P)[R],
+                                                     CellPrep (&Pn)[R], const TraceRing& ring) {
+  constexpr int kSolve = LDS ? kTraceGlobalRing : kTraceGlobal;
+  double u[R][4][3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = s0 + c + r * cpr;
+    if (lane_ok && s < s1) {
+      if (LDS && ring_hit(P[r], ring))
+        load_upstream<kTraceRing>(A, i, P[r], g, ring, u[r]);
+      else
+        load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = s0 + c + r * cpr;
+    if (lane_ok && s < s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+  }
+  if (lane_ok)
+    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
+}
+
+template <int BLOCK, int R, bool NT, bool LDS = false>
+__device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
+                                                           int nl, TraceRing ring = TraceRing{nullptr, 0, 0}) {
+  const int cpr = BLOCK / A.G;
+  const int c = threadIdx.x / A.G;
+  const int g = threadIdx.x - c * A.G;
+  const bool lane_ok = c < cpr;
+  const int slots = ring.mask + 1;
+  CellPrep Pa[R], Pb[R];
+  if (nl <= 0) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(off[0] + c + r * cpr, off[1] - 1), g, Pa[r]);
+  for (int l = 0; l < nl; l += 2) {
+    const int l1 = min(l + 1, nl - 1);
+    ring.lim = off[l + 1] - slots;
+    level_upstream_first<R, NT, LDS>(A, i, off[l], off[l + 1], off[l1], off[l1 + 1], cpr, c, g, lane_ok, Pa, Pb,
+                                     ring);
+    __syncthreads();
+    if (l + 1 >= nl) break;
+    const int l2 = min(l + 2, nl - 1);
+    ring.lim = off[l + 2] - slots;
+    level_upstream_first<R, NT, LDS>(A, i, off[l + 1], off[l + 2], off[l2], off[l2 + 1], cpr, c, g, lane_ok, Pb, Pa,
+                                     ring);
     __syncthreads();
   }
 }
@@ -336,7 +493,59 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
                                                                 const int32_t* __restrict__ nlev, int maxlev,
                                                                 SweepArgs A) {
   const int i = blockIdx.x;
-  walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+  // upstream-first order pays with few groups per cell (measured: 2 groups 10.5 -> 9.7 ms on 384k tets x 256
+  // directions) and costs with many (24 groups at 1024 threads 52.4 -> 55.3 ms)
+  if constexpr (NT)
+    walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+  else
+    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1),
+                                                                     nlev[i]);
+}
+
+// Direction-resident sweep with the trace ring in LDS (dynamic shared memory of
+// (mask + 1) * 4 * G doubles): upstream traces of the direction's most recent
+// positions come from LDS, older ones from global memory.  The flux is written
+// to both; the workgroup barrier between levels orders the LDS writes and reads.
+template <int BLOCK, bool NT>
+__global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
+                                                                    const int32_t* __restrict__ nlev, int maxlev,
+                                                                    SweepArgs A, int mask) {
+  extern __shared__ double ring_buf[];
+  const int i = blockIdx.x;
+  if constexpr (NT)
+    walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
+                                                            TraceRing{ring_buf, mask, 0});
+  else
+    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true>(
+        A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf, mask, 0});
+}
+
+// pos[i][e] = sweep position of element e in direction i.
+__global__ __launch_bounds__(kBlock) void sweep_positions_kernel(int NI, int64_t NE,
+                                                                 const uint32_t* __restrict__ tasks,
+                                                                 int32_t* __restrict__ pos) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (int64_t)NI * NE) return;
+  const int64_t i = k / NE;
+  pos[i * NE + (tasks[k] & 0xFFFFFFu)] = int32_t(k - i * NE);
+}
+
+// upos[i][s][f] = sweep position (direction i) of the upstream neighbour across
+// face f of the cell at position s, -1 where f is not upstream.
+__global__ __launch_bounds__(kBlock) void upstream_positions_kernel(int NI, int64_t NE,
+                                                                    const uint32_t* __restrict__ tasks,
+                                                                    const int4* __restrict__ nbr,
+                                                                    const int32_t* __restrict__ pos,
+                                                                    int4* __restrict__ upos) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (int64_t)NI * NE) return;
+  const int64_t i = k / NE;
+  const uint32_t w = tasks[k];
+  const unsigned up = (w >> 24) & 0xFu;
+  const int4 n = nbr[w & 0xFFFFFFu];
+  const int32_t* p = pos + i * NE;
+  upos[k] = make_int4((up & 1u) ? p[n.x] : -1, (up & 2u) ? p[n.y] : -1, (up & 4u) ? p[n.z] : -1,
+                      (up & 8u) ? p[n.w] : -1);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
@@ -594,6 +803,65 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
 }
 
 template <bool NT>
+static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
+                                   int maxlev, const SweepArgs& A, int mask, size_t bytes) {
+  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
+                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
+                                  : (const void*)&sweep_direction_lds_kernel<1024, NT>;
+  LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  if (block == 256)
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A,
+                       mask);
+  else if (block == 512)
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A,
+                       mask);
+  else
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A,
+                       mask);
+  return LB_OK;
+}
+
+// Ring slots for a group batch: the largest power of two whose [slot][4][G]
+// doubles fit the LDS a workgroup may allocate.
+static int trace_ring_size(lb_op* op, int G, int* mask, size_t* bytes) {
+  if (op->lds_max == 0) {
+    int dev = 0, v = 0;
+    LB_HIP(hipGetDevice(&dev));
+    LB_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    op->lds_max = v;
+  }
+  const size_t per_slot = (size_t)4 * G * sizeof(double);
+  if (per_slot > (size_t)op->lds_max) {
+    set_error("sweep: the LDS trace ring cannot hold one position of this group batch");
+    return LB_E_ARG;
+  }
+  size_t slots = 1;
+  while (slots * 2 * per_slot <= (size_t)op->lds_max) slots *= 2;
+  *mask = (int)slots - 1;
+  *bytes = slots * per_slot;
+  return LB_OK;
+}
+
+// Upstream neighbour positions of every (direction, sweep position), for the
+// LDS sweep (built once; a temporary inverse permutation [NI][NE]).
+static int ensure_upstream_positions(lb_op* op) {
+  if (op->d_upos.p) return LB_OK;
+  const int64_t n = op->NI * op->NE;
+  DevBuf pos;
+  LB_TRY(pos.reserve(n * sizeof(int32_t)));
+  LB_TRY(op->d_upos.reserve(n * sizeof(int4)));
+  hipLaunchKernelGGL(sweep_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                     op->d_tasks.as<const uint32_t>(), pos.as<int32_t>());
+  hipLaunchKernelGGL(upstream_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                     op->d_tasks.as<const uint32_t>(), op->d_nbr.as<const int4>(), pos.as<const int32_t>(),
+                     op->d_upos.as<int4>());
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipStreamSynchronize(op->stream));
+  pos.release();
+  return LB_OK;
+}
+
+template <bool NT>
 static void launch_persistent_nt(int block, dim3 grid, hipStream_t st, const int4* items, int nitems,
                                  const int32_t* lvl, const int32_t* act, unsigned int* ctr, const SweepArgs& A) {
   if (block == 256)
@@ -669,7 +937,7 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
 // One sweep issued in a fixed launch configuration.
 static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
   const int mode = cfg.mode, block = cfg.block;
-  if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_PERSISTENT) &&
+  if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
       !((block == 256 || block == 512 || block == 1024) && block / G >= 1)) {
     set_error("sweep: the workgroup size must be 256, 512 or 1024 and hold the group batch");
     return LB_E_ARG;
@@ -704,7 +972,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
   SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
               op->G, g0, G, op->NE, q, psi};
   const bool nt = G >= kStreamGroups;
-  if (mode == LB_SWEEP_DIRECTION) {
+  if (mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
       LB_HIP(hipMemcpy(op->d_lvl_off.p, op->h_lvl_off.data(), op->h_lvl_off.size() * sizeof(int32_t),
@@ -712,9 +980,24 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       LB_TRY(op->d_nlev.reserve(op->h_nlev.size() * sizeof(int32_t)));
       LB_HIP(hipMemcpy(op->d_nlev.p, op->h_nlev.data(), op->h_nlev.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
-    LB_HIP(hipEventRecord(op->ev0, op->stream));
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
+    if (mode == LB_SWEEP_DIRECTION_LDS) {
+      int mask = 0;
+      size_t bytes = 0;
+      LB_TRY(trace_ring_size(op, G, &mask, &bytes));
+      LB_TRY(ensure_upstream_positions(op));
+      A.upos = op->d_upos.as<const int4>();
+      LB_HIP(hipEventRecord(op->ev0, op->stream));
+      if (nt)
+        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, mask, bytes));
+      else
+        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, mask, bytes));
+      LB_HIP(hipGetLastError());
+      LB_HIP(hipEventRecord(op->ev1, op->stream));
+      return LB_OK;
+    }
+    LB_HIP(hipEventRecord(op->ev0, op->stream));
     launch_direction(block, nt, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A);
     LB_HIP(hipGetLastError());
     LB_HIP(hipEventRecord(op->ev1, op->stream));

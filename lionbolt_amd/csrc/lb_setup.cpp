@@ -260,7 +260,55 @@ int compute_schedule(lb_op* op, const int64_t* nb) {
     int32_t* o = &op->h_lvl_off[i * (op->maxlev + 1)];
     for (int l = 0; l <= op->maxlev; ++l) o[l] = l < (int)lvl[i].size() ? lvl[i][l] : (int32_t)NE;
   }
-  LB_TRY(upload(op, op->d_tasks, op->h_tasks.data(), op->h_tasks.size() * sizeof(uint32_t)));
+  if (op->host_only) return LB_OK;
+  // The device sweeps each level in its own order: the cells whose upstream
+  // neighbours all sit in the last `slots` positions before the level's end
+  // (still in the LDS trace ring of a one-group direction-resident sweep) come
+  // first, the others after them, each part in the reference's ascending
+  // element order.  Cells of a level are independent, so every order gives the
+  // same flux; h_tasks keeps the reference order for the schedule queries.
+  int dev = 0, lds = 0;
+  LB_HIP(hipGetDevice(&dev));
+  LB_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  op->lds_max = lds;
+  int64_t slots = 1;
+  while (slots * 2 * 4 * (int64_t)sizeof(double) <= lds) slots *= 2;
+  std::vector<uint32_t> dev_tasks(NI * NE);
+#pragma omp parallel
+  {
+    std::vector<int32_t> fpos(NE);
+    std::vector<uint32_t> miss;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t i = 0; i < NI; ++i) {
+      const uint32_t* tk = &op->h_tasks[i * NE];
+      uint32_t* dk = &dev_tasks[i * NE];
+      const int32_t* off = &op->h_lvl_off[i * (op->maxlev + 1)];
+      for (int l = 0; l < op->h_nlev[i]; ++l) {
+        const int64_t s0 = off[l], s1 = off[l + 1], lim = s1 - slots;
+        int64_t o = s0;
+        miss.clear();
+        for (int64_t sp = s0; sp < s1; ++sp) {
+          const uint32_t w = tk[sp];
+          const int64_t e = w & 0xFFFFFFu;
+          const unsigned up = (w >> 24) & 0xFu;
+          bool hit = true;
+          for (int f = 0; f < 4; ++f)
+            if ((up >> f & 1u) && fpos[nb[e * 4 + f]] < lim) hit = false;
+          if (hit) {
+            dk[o] = w;
+            fpos[e] = (int32_t)o++;
+          } else {
+            miss.push_back(w);
+          }
+        }
+        for (uint32_t w : miss) {
+          dk[o] = w;
+          fpos[w & 0xFFFFFFu] = (int32_t)o++;
+        }
+      }
+    }
+  }
+  LB_TRY(upload(op, op->d_tasks, dev_tasks.data(), dev_tasks.size() * sizeof(uint32_t)));
   return LB_OK;
 }
 

@@ -95,7 +95,8 @@ class Operator:
         _lib.check(_lib.lib().lb_op_set_scatter_mode(self.h, m))
 
     _SWEEP_MODES = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION,
-                    "auto": _lib.SWEEP_AUTO, "persistent": _lib.SWEEP_PERSISTENT}
+                    "auto": _lib.SWEEP_AUTO, "persistent": _lib.SWEEP_PERSISTENT,
+                    "direction_lds": _lib.SWEEP_DIRECTION_LDS}
 
     def set_sweep_mode(self, mode):
         _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, self._SWEEP_MODES[mode]))

@@ -1,7 +1,8 @@
 """Every way of issuing a sweep produces the same flux, bit for bit: one kernel
 per global level (streamed or replayed from a captured graph) and the
-direction-resident kernel in which a workgroup walks all levels of one
-direction with workgroup barriers."""
+direction-resident kernels in which a workgroup walks all levels of one
+direction with workgroup barriers, reading upstream traces from global memory
+or from the LDS ring of the direction's recent positions."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ from lionbolt_amd import angular, mesh as M
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("G,block", [(1, 256), (1, 1024), (5, 512), (24, 1024), (24, 256)])
+@pytest.mark.parametrize("G,block", [(1, 256), (1, 512), (1, 1024), (2, 512), (5, 512), (24, 1024), (24, 256)])
 def test_direction_resident_equals_level_launches(G, block):
     import torch
     from lionbolt_amd.ops import Operator
@@ -22,15 +23,40 @@ def test_direction_resident_equals_level_launches(G, block):
     op.set_direction_block(block)
     q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
     res = {}
-    for mode in ("levels", "direction", "direction", "graph"):
+    for mode in ("levels", "direction", "direction", "graph", "direction_lds", "direction_lds"):
         op.set_sweep_mode(mode)
         psi = torch.full_like(q, float("nan"))
         op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         res.setdefault(mode, []).append(psi.cpu().numpy())
-    for mode in ("direction", "graph"):
+    for mode in ("direction", "graph", "direction_lds"):
         for v in res[mode]:
             assert np.array_equal(v, res["levels"][0]), mode
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_lds_ring_with_global_fallback(G):
+    """A direction longer than the LDS ring (16,464 tets): upstream traces of
+    positions that left the ring come from global memory."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(14, a=6.0, seed=1013, slab=(2.0, 3.5))
+    ang = angular.product(2, 4, 0)
+    op = Operator(m, ang)
+    sigt = np.stack([np.linspace(0.3, 0.9, G), np.linspace(0.8, 2.5, G)], axis=1)
+    op.set_physics_groups(sigt, np.zeros((G, 2, 1)))
+    q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
+    out = {}
+    for mode in ("levels", "direction_lds"):
+        op.set_sweep_mode(mode)
+        for block in (256, 512):
+            op.set_direction_block(block)
+            psi = torch.full_like(q, float("nan"))
+            op.transport_matinv_dev(q, psi, 0, G)
+            op.synchronize()
+            out[(mode, block)] = psi.cpu().numpy()
+    for block in (256, 512):
+        assert np.array_equal(out[("direction_lds", block)], out[("levels", 256)])
 
 
 def test_direction_resident_in_place_and_solver():
@@ -43,12 +69,13 @@ def test_direction_resident_in_place_and_solver():
     op.set_sweep_mode("levels")
     a, sa = op.boltzmann_matinv(q, space="moment", m=30)
     t1 = op.transport_matinv(q)
-    op.set_sweep_mode("direction")
-    b, sb = op.boltzmann_matinv(q, space="moment", m=30)
-    t2 = op.transport_matinv(q)             # in place on the staging buffer
-    assert sa["iterates"] == sb["iterates"]
-    assert np.array_equal(a, b)
-    assert np.array_equal(t1, t2)
+    for mode in ("direction", "direction_lds"):
+        op.set_sweep_mode(mode)
+        b, sb = op.boltzmann_matinv(q, space="moment", m=30)
+        t2 = op.transport_matinv(q)             # in place on the staging buffer
+        assert sa["iterates"] == sb["iterates"]
+        assert np.array_equal(a, b)
+        assert np.array_equal(t1, t2)
 
 
 @pytest.mark.parametrize("G", [1, 24])
@@ -70,10 +97,11 @@ def test_autotuned_sweeps_equal_level_launches(G):
     ref = ref.cpu().numpy()
     op.set_sweep_mode("auto")
     assert op.sweep_choice(G) == ("auto", 0, 0)
-    for _ in range(10):
+    for _ in range(16):     # two timed sweeps for each of the six configurations, then settled ones
         psi = torch.full_like(q, float("nan"))
         op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         assert np.array_equal(psi.cpu().numpy(), ref)
     mode, threads, _ = op.sweep_choice(G)
-    assert (mode, threads) in {("graph", 0), ("direction", 256), ("direction", 512), ("direction", 1024)}
+    assert (mode, threads) in {("graph", 0), ("direction", 256), ("direction", 512), ("direction", 1024),
+                               ("direction_lds", 256), ("direction_lds", 512)}
