@@ -222,8 +222,20 @@ int lb_op_set_direction_block(lb_op* op, int threads);
 /* TransportOp%MatInv (TransportOperator.f90:72-151): s <- T^-1 s in place. */
 int lb_transport_matinv(lb_op* op, double* s, int64_t n);
 
-/* ScatteringOp%MatVec (ScatteringOperator.f90:39-70): x <- K x in place. */
+/* ScatteringOp%MatVec (ScatteringOperator.f90:39-70): x <- K x in place.
+ * After lb_op_build_delta_down, K is the delta-down kernel. */
 int lb_scatter_matvec(lb_op* op, double* x, int64_t n);
+
+/* ScatteringOp%BuildDeltaDown (ScatteringOperator.f90:291-319): K becomes
+ * XSdd(mat) times the identity in angle (ApplyDeltaDown,
+ * ScatteringAlgebra.f90:62-90), xsdd[m] for each of the Nm mesh materials
+ * (Nm >= the mesh's material count).  The reference's energy-coupling loop
+ * applies it to a source group's angular flux when the cross sections carry a
+ * delta-down row (Main.f90:610-624).  Only lb_scatter_matvec applies it: the
+ * Boltzmann and transport entry points of a delta-down handle return
+ * LB_E_ARG until lb_op_set_physics, which restores the Legendre kernel as
+ * ScatteringOp%Physics does (ScatteringOperator.f90:231). */
+int lb_op_build_delta_down(lb_op* op, int Nm, const double* xsdd);
 
 /* BoltzmannOp%MatVec (BoltzmannOperator.f90:59-104): x <- x - T^-1 K x. */
 int lb_boltzmann_matvec(lb_op* op, double* x, int64_t n);

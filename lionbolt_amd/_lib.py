@@ -56,6 +56,7 @@ SIGNATURES = {
     "lb_op_set_physics_groups": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d]),
     "lb_op_set_scatter_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_op_set_sweep_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lb_op_build_delta_down": (ctypes.c_int, [_vp, ctypes.c_int, _d]),
     "lb_op_sweep_choice": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]),
     "lb_op_set_workgroups_per_cu": (ctypes.c_int, [_vp, ctypes.c_int]),

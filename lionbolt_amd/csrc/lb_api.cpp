@@ -58,6 +58,11 @@ int check_ready(lb_op* op, const char* who) {
     set_error(std::string(who) + ": null operator");
     return LB_E_ARG;
   }
+  if (op->delta_down) {
+    set_error(std::string(who) + " was called on a delta-down scattering kernel, which only "
+              "ScatteringOp%MatVec applies (set the physics first).");
+    return LB_E_ARG;
+  }
   if (!op->phys) {
     set_error(std::string(who) + " was called, but the operator was not ready.");
     return LB_E_NOT_READY;
@@ -106,6 +111,7 @@ int upload_physics(lb_op* op) {
   LB_TRY(op->d_yys.reserve(yys.size() * sizeof(double)));
   LB_HIP(hipMemcpy(op->d_yys.p, yys.data(), yys.size() * sizeof(double), hipMemcpyHostToDevice));
   op->phys = true;
+  op->delta_down = false;   // Physics builds the Legendre kernel (ScatteringOperator.f90:231)
   return LB_OK;
 }
 
@@ -364,7 +370,25 @@ int lb_transport_matinv(lb_op* op, double* s, int64_t n) {
   return to_host(op, s, n);
 }
 
+int lb_op_build_delta_down(lb_op* op, int Nm, const double* xsdd) {
+  if (!op || !xsdd || Nm < op->Nm) {
+    set_error("ScatteringOp%BuildDeltaDown: need one cross section per mesh material (Nm >= " +
+              std::to_string(op ? op->Nm : 0) + ")");
+    return LB_E_ARG;
+  }
+  LB_TRY(op->d_xsdd.reserve((size_t)op->Nm * sizeof(double)));
+  LB_HIP(hipMemcpy(op->d_xsdd.p, xsdd, (size_t)op->Nm * sizeof(double), hipMemcpyHostToDevice));
+  op->delta_down = true;
+  return LB_OK;
+}
+
 int lb_scatter_matvec(lb_op* op, double* x, int64_t n) {
+  if (op && op->delta_down) {   // ApplyDeltaDown (ScatteringAlgebra.f90:62-90)
+    LB_TRY(check_n(op, n, "ScatteringOp%MatVec"));
+    LB_TRY(to_device(op, x, n));
+    LB_TRY(launch_delta_down(op, n, op->w_a.as<double>()));
+    return to_host(op, x, n);
+  }
   LB_TRY(check_ready(op, "ScatteringOp%MatVec"));
   LB_TRY(check_n(op, n, "ScatteringOp%MatVec"));
   LB_TRY(to_device(op, x, n));
@@ -524,6 +548,7 @@ void lb_op_destroy(lb_op* op) {
   op->d_lvl_off.release();
   op->d_nlev.release();
   op->d_upos.release();
+  op->d_xsdd.release();
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);
   if (op->stream) (void)hipStreamDestroy(op->stream);

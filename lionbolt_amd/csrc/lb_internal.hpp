@@ -142,6 +142,8 @@ struct lb_op {
   // physics
   int G = 0;                        // groups in the physics table
   bool phys = false;
+  bool delta_down = false;          // K = XSdd(mat) I (ScatteringOp%BuildDeltaDown)
+  lb::DevBuf d_xsdd;                // double [Nm]
   std::vector<double> h_sigt;       // [Nm][G]
   std::vector<double> h_sigs;       // [Nm][G][L+1]
   lb::DevBuf d_sigt, d_sigs;
@@ -197,6 +199,7 @@ int launch_downscatter(lb_op* op, const double* phi_all, const double* S, int g,
 int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G);
 int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* z);
 int launch_scale(hipStream_t s, int64_t n, double a, double* x);
+int launch_delta_down(lb_op* op, int64_t n, double* x);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int launch_fluence(lb_op* op, const double* v, double* phi);

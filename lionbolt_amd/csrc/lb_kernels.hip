@@ -775,6 +775,14 @@ __global__ __launch_bounds__(kBlock) void fluence_kernel(const double* __restric
   phi[j] = acc;
 }
 
+// x[sd + i NENK] *= xsdd[mat(sd)]: the delta-down kernel, identity in angle
+// (ApplyDeltaDown, ScatteringAlgebra.f90:62-90).
+__global__ __launch_bounds__(kBlock) void delta_down_kernel(int64_t n, int64_t NENK, const int32_t* __restrict__ mat,
+                                                            const double* __restrict__ xsdd, double* __restrict__ x) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock)
+    x[j] *= xsdd[mat[(j % NENK) >> 2]];
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 inline unsigned grid_stride(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
@@ -1023,6 +1031,13 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
   enqueue();
   LB_HIP(hipGetLastError());
   LB_HIP(hipEventRecord(op->ev1, op->stream));
+  return LB_OK;
+}
+
+int launch_delta_down(lb_op* op, int64_t n, double* x) {
+  hipLaunchKernelGGL(delta_down_kernel, dim3(grid_stride(n)), dim3(kBlock), 0, op->stream, n, op->NENK,
+                     op->d_mat.as<const int32_t>(), op->d_xsdd.as<const double>(), x);
+  LB_HIP(hipGetLastError());
   return LB_OK;
 }
 

@@ -54,6 +54,12 @@ Module LionboltHIP
             Integer (c_int), Value       :: Nm, L
             Real (c_double), Intent (In) :: sigt (*), sigs (*)
         End Function
+        Integer (c_int) Function lb_op_build_delta_down (op, Nm, xsdd) Bind(C, name='lb_op_build_delta_down')
+            import :: c_int, c_ptr, c_double
+            Type (c_ptr),    Value       :: op
+            Integer (c_int), Value       :: Nm
+            Real (c_double), Intent (In) :: xsdd (*)
+        End Function
         Integer (c_int) Function lb_transport_matinv (op, s, n) Bind(C, name='lb_transport_matinv')
             import :: c_int, c_ptr, c_double, c_int64_t
             Type (c_ptr),        Value          :: op
@@ -107,6 +113,7 @@ Module LionboltHIP
     Contains
         Procedure :: BuildSpaceAngle
         Procedure :: BuildPhysics
+        Procedure :: BuildDeltaDown
         Procedure :: TransportMatInv
         Procedure :: ScatterMatVec
         Procedure :: BoltzmannMatVec
@@ -171,6 +178,16 @@ Contains
         Integer (c_int) :: rc
         rc = lb_op_set_physics(Op%h, INT(SIZE(sigt), c_int), sigt, INT(SIZE(sigs, DIM=1) - 1, c_int), sigs)
         if (rc /= 0) call lb_stop('Op%Build(angular, XS)', rc)
+        Op%PHYSREADY = .TRUE.
+    End Subroutine
+
+    ! ScatteringOp%BuildDeltaDown(XSdd): K = XSdd(mat) I for the next MatVec calls
+    Subroutine BuildDeltaDown (Op, XSdd)
+        Class (HIPOperator), Intent (InOut) :: Op
+        Real (c_double),     Intent (In)    :: XSdd (:)
+        Integer (c_int) :: rc
+        rc = lb_op_build_delta_down(Op%h, INT(SIZE(XSdd), c_int), XSdd)
+        if (rc /= 0) call lb_stop('ScatteringOp%BuildDeltaDown', rc)
         Op%PHYSREADY = .TRUE.
     End Subroutine
 

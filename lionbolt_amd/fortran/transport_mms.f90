@@ -6,15 +6,17 @@
 !  (real64); node(4,NE), neighbors(4,NE), mat(NE) (int64, 1-based, boundary 0);
 !  sigma (real64), c (real64: scattering ratio of the Boltzmann solve).
 !  output: T^-1 s for the MMS source (NENK*NI real64), then the scalar flux of
-!  BoltzmannOp%MatInv of a unit source (NENK real64).  Prints the RMS MMS error.
+!  BoltzmannOp%MatInv of a unit source (NENK real64).  Prints the RMS MMS error
+!  and the error of the delta-down kernel in the shape of
+!  validation/operators/deltadownMMS.f90 (XSdd = 1, 10 on two materials, s = 1).
 Program transport_mms
     use iso_c_binding
     use LionboltHIP
     Implicit None
-    Type (HIPOperator)               :: OpT, OpL
+    Type (HIPOperator)               :: OpT, OpL, OpK
     Integer (c_int64_t)              :: NE, NK, NI, L, e, i, kk, sdof, n
     Real (c_double),     Allocatable :: rg (:,:), kv (:,:), w (:), s (:), strue (:), phi (:), sigs (:,:)
-    Integer (c_int64_t), Allocatable :: node (:,:), nbr (:,:), mat (:)
+    Integer (c_int64_t), Allocatable :: node (:,:), nbr (:,:), mat (:), mat2 (:)
     Real (c_double)                  :: sigma, c, r (3), err
     Real (c_double), Parameter       :: A = 10.0_c_double
     Character (LEN=512)              :: fin, fout
@@ -60,6 +62,27 @@ Program transport_mms
     phi = OpL%Fluence(strue)
     write (*, '(A,I0,A,ES12.4)') 'BoltzmannOp%MatInv iterates: ', OpL%stats%iterates, &
         '  true residual: ', OpL%stats%true_residual
+
+    ! deltadownMMS: two materials split at z = 0, K = XSdd(mat) I, s = 1
+    ALLOCATE(mat2(NE))
+    do e = 1, NE
+        mat2(e) = MERGE(2_c_int64_t, 1_c_int64_t, SUM(rg(3, node(:, e))) > 0.0_c_double)
+    end do
+    call OpK%BuildSpaceAngle(rg, node, nbr, mat2, kv, w, INT(L), 'GMRES')
+    call OpK%BuildDeltaDown([1.0_c_double, 10.0_c_double])
+    strue = 1.0_c_double
+    call OpK%ScatterMatVec(strue)
+    err = 0.0_c_double
+    do i = 1, NI
+        do e = 1, NE
+            do kk = 1, 4
+                sdof = 4 * (e - 1) + kk
+                err = MAX(err, ABS(strue(sdof + (i - 1) * 4 * NE) - MERGE(10.0_c_double, 1.0_c_double, mat2(e) == 2)))
+            end do
+        end do
+    end do
+    write (*, '(A,ES12.4)') 'deltadownMMS max error: ', err
+    call OpK%Destroy()
 
     open (newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
     write (u) s, phi

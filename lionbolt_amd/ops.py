@@ -80,6 +80,11 @@ class Operator:
         self.G = 1
         self.sigt = sigt.reshape(1, -1)
 
+    def build_delta_down(self, xsdd):
+        """ScatteringOp%BuildDeltaDown: K = XSdd(mat) I until the next set_physics."""
+        xsdd = np.ascontiguousarray(np.atleast_1d(xsdd), dtype=np.float64)
+        _lib.check(_lib.lib().lb_op_build_delta_down(self.h, xsdd.size, _lib.dptr(xsdd)))
+
     def set_physics_groups(self, sigt, sigs):
         """``sigt`` (G, Nm), ``sigs`` (G, Nm, L+1)."""
         sigt = np.ascontiguousarray(sigt, dtype=np.float64)

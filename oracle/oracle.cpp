@@ -181,6 +181,8 @@ struct OrcOp {
   std::vector<double> sigt /*[m]*/, sigs /*[m][l]*/;
   std::vector<std::vector<i64>> mat2sd;
   bool phys = false;
+  bool deltadown = false;           // ScatteringOp%BuildDeltaDown
+  std::vector<double> xsdd /*[m]*/;
   // workspaces
   std::vector<double> xw, sxw;
 };
@@ -628,6 +630,7 @@ void orc_set_physics(void* h, const double* sigt /*[Nm]*/, const double* sigs /*
     for (int l = 0; l <= op->L; ++l)
       for (i64 j = 0; j < NI * NI; ++j) op->YYS[m * NI * NI + j] += op->YY[l * NI * NI + j] * op->sigs[m * (op->L + 1) + l];
   op->phys = true;
+  op->deltadown = false;
 }
 
 void orc_get_geometry(void* h, double* IC, double* IFM, double* IIM, double* nrm, double* area, double* detJ) {
@@ -664,11 +667,27 @@ void orc_transport_matinv(void* h, double* s) {
 // order (for timing the sweep alone).
 void orc_sweep_only(void* h, double* s) { sweep(static_cast<OrcOp*>(h), s); }
 
+// ScatteringOp%BuildDeltaDown (ScatteringOperator.f90:291-319): the next
+// scattering products apply XSdd(mat) times the identity in angle, until
+// orc_set_physics builds the Legendre kernel again (ScatteringOperator.f90:231).
+void orc_build_delta_down(void* h, const double* xsdd /*[Nm]*/) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  op->xsdd.assign(xsdd, xsdd + op->Nm);
+  op->deltadown = true;
+}
+
 // SNMGXS (ScatteringAlgebra.f90:13-55): x(NENK,NI) <- x . YYS; dgemm for one
-// material, per-dof dgemv('T') over mat2sd otherwise.
+// material, per-dof dgemv('T') over mat2sd otherwise.  With a delta-down kernel,
+// ApplyDeltaDown (ScatteringAlgebra.f90:62-90): x(sd + i NENK) *= XSdd(mat).
 void orc_scatter_matvec(void* h, double* x) {
   OrcOp* op = static_cast<OrcOp*>(h);
   const i64 NI = op->NI, NENK = op->NENK;
+  if (op->deltadown) {
+    for (int m = 0; m < op->Nm; ++m)
+      for (i64 sd : op->mat2sd[m])
+        for (i64 i = 0; i < NI; ++i) x[sd + i * NENK] *= op->xsdd[m];
+    return;
+  }
   double* xw = op->xw.data();
   std::vector<i64> sd2m(NENK);
   for (int m = 0; m < op->Nm; ++m)

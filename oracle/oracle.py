@@ -52,6 +52,7 @@ def lib():
         L.orc_boltzmann_matinv.restype = ctypes.c_int
         L.orc_boltzmann_matinv.argtypes = [ctypes.c_void_p, _d, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_int, _d, ctypes.c_int, _ci, _ci]
+        L.orc_build_delta_down.argtypes = [ctypes.c_void_p, _d]
         L.orc_scatter_moment_matvec.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int, _d]
         L.orc_fluence.argtypes = [ctypes.c_void_p, _d, _d]
         L.orc_norm2.restype = ctypes.c_double
@@ -132,6 +133,12 @@ class OracleOp:
         x = self._vec(s)
         lib().orc_scatter_matvec(self.h, _p(x))
         return x
+
+    def build_delta_down(self, xsdd):
+        """ScatteringOp%BuildDeltaDown: K = XSdd(mat) I until the next set_physics."""
+        self._xsdd = np.ascontiguousarray(xsdd, dtype=np.float64)
+        assert self._xsdd.size >= self.Nm
+        lib().orc_build_delta_down(self.h, _p(self._xsdd))
 
     def scatter_moment_matvec(self, s, D, M, deg):
         """K x through the flux moments (the CPU baseline's moment form)."""

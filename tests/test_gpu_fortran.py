@@ -1,7 +1,8 @@
 """The Fortran iso_c_binding shim drives the HIP path like the reference's own
 TransportMMS driver (validation/operators/TransportMMS.f90:47-89): a Fortran
 program builds the operators, applies T^-1 to the MMS source and solves a
-scattering problem with BoltzmannOp%MatInv; its outputs match the oracle."""
+scattering problem with BoltzmannOp%MatInv; its outputs match the oracle.  It
+also applies a delta-down kernel as validation/operators/deltadownMMS.f90 does."""
 import os
 import subprocess
 
@@ -38,6 +39,9 @@ def test_fortran_driver_matches_oracle(tmp_path):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     assert "Successfully terminated" in r.stdout
+    # deltadownMMS through the shim's BuildDeltaDown: K s = XSdd(mat) exactly
+    dd = [ln for ln in r.stdout.splitlines() if ln.startswith("deltadownMMS max error:")]
+    assert len(dd) == 1 and float(dd[0].split(":")[1]) == 0.0
     out = np.fromfile(fout, dtype=np.float64)
     n = 4 * m.NE * ang.NI
     psi_f, phi_f = out[:n], out[n:]

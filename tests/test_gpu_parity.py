@@ -139,6 +139,27 @@ def test_high_scattering_orders_match_oracle(ops_module, L):
     assert rel(lib_op.fluence(a), orc.fluence(b)) <= 1e-7
 
 
+def test_delta_down_kernel_matches_oracle(ops_module):
+    """ScatteringOp%BuildDeltaDown: K = XSdd(mat) I, bit for bit; Boltzmann calls
+    are refused on such a handle; set_physics restores the Legendre kernel."""
+    from lionbolt_amd import _lib
+    m = two_material(M.jittered_cube(3, a=2.0))
+    ang = angular.snprep(2)
+    sigs = np.array([[0.6, 0.3, 0.1], [1.5, 0.9, 0.4]])
+    lib_op, orc = make_pair(ops_module, m, ang, [1.0, 2.0], sigs)
+    x = np.random.default_rng(12).standard_normal(lib_op.n)
+    xs = np.array([0.37, 2.9])
+    lib_op.build_delta_down(xs)
+    orc.build_delta_down(xs)
+    assert np.array_equal(lib_op.scatter_matvec(x), orc.scatter_matvec(x))
+    with pytest.raises(_lib.LionboltError, match="delta-down"):
+        lib_op.boltzmann_matvec(x)
+    lib_op.set_physics([1.0, 2.0], sigs)
+    orc.set_physics([1.0, 2.0], sigs)
+    assert rel(lib_op.scatter_matvec(x), orc.scatter_matvec(x)) <= 1e-12
+    assert rel(lib_op.boltzmann_matvec(x), orc.boltzmann_matvec(x)) <= 1e-12
+
+
 def test_gmres_psi_space_matches_oracle(ops_module):
     m = M.jittered_cube(4, a=4.0)
     ang = angular.level_symmetric_s8(1)

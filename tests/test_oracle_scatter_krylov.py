@@ -79,6 +79,31 @@ def test_moment_form_equals_dense_kernel(small_mesh):
     assert np.abs(mom - dense).max() <= 1e-13 * np.abs(dense).max()
 
 
+def test_delta_down_kernel(small_mesh):
+    """deltadownMMS (validation/operators/deltadownMMS.f90): XSdd = (1, 10) on two
+    materials, s = 1 -> K s = XSdd(mat) in every direction; then any vector is
+    scaled pointwise by its dof's material, and set_physics restores the
+    Legendre kernel."""
+    m = M.TetMesh(small_mesh.rg, small_mesh.node, np.where(small_mesh.centroids()[:, 2] > 1.0, 2, 1))
+    ang = angular.snprep(1)
+    op = OracleOp(m, ang)
+    sigs = np.array([[0.5, 0.2], [2.0, 0.7]])
+    op.set_physics([1.0, 3.0], sigs)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(op.n)
+    legendre = op.scatter_matvec(x)
+    op.build_delta_down([1.0, 10.0])
+    ones = op.scatter_matvec(np.ones(op.n)).reshape(ang.NI, m.NE, 4)
+    expect = np.where(m.mat == 2, 10.0, 1.0)[None, :, None] * np.ones((ang.NI, 1, 4))
+    assert np.array_equal(ones, expect)
+    xs = np.array([0.37, 2.9])
+    op.build_delta_down(xs)
+    out = op.scatter_matvec(x).reshape(ang.NI, m.NE, 4)
+    assert np.array_equal(out, x.reshape(ang.NI, m.NE, 4) * xs[m.mat - 1][None, :, None])
+    op.set_physics([1.0, 3.0], sigs)
+    assert np.array_equal(op.scatter_matvec(x), legendre)
+
+
 def _scatter_moment(l):
     if l >= 14:
         return 0.0
