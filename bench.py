@@ -439,7 +439,7 @@ def run(args):
             op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         mode, block, _ = op.sweep_choice(G)
-    elif args.sweep_mode == "direction":
+    elif args.sweep_mode in ("direction", "direction_lds"):
         block = 1024
     for _ in range(args.warmup):
         op.transport_matinv_dev(q, psi, 0, G)
@@ -461,7 +461,7 @@ def run(args):
         raise RuntimeError("non-finite flux")
     other_modes = {}
     if args.compare_modes:
-        for other in ("levels", "graph", "direction"):
+        for other in ("levels", "graph", "direction", "direction_lds"):
             if other == args.sweep_mode:
                 continue
             op.set_sweep_mode(other)
@@ -574,7 +574,7 @@ def main():
     p.add_argument("--no-learned-pc", dest="learned_pc", action="store_false",
                    help="skip the learned-preconditioner solve (cfg5)")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
-    p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction"])
+    p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction", "direction_lds"])
     p.add_argument("--compare-modes", action="store_true")
     p.add_argument("--gemm-study", action="store_true", help="scattering-moment products on MFMA, all formats")
     args = p.parse_args()
