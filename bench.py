@@ -154,6 +154,25 @@ def copy_calibration(gib=4):
             "copy_GBs": 2 * gib * (1 << 30) / dt / 1e9}
 
 
+def settle_autotuner(op, m, ang, G=1):
+    """Run the sweep autotuner to its choice on scratch vectors (part of setup,
+    like the schedule build) and return the seconds it took."""
+    import torch
+    q = torch.rand(m.NE * ang.NI * 4 * G, dtype=torch.float64, device="cuda")
+    psi = torch.empty_like(q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(32):
+        if op.sweep_choice(G)[0] != "auto":
+            break
+        op.transport_matinv_dev(q, psi, 0, G)
+    op.synchronize()
+    dt = time.perf_counter() - t0
+    del q, psi
+    torch.cuda.empty_cache()
+    return dt
+
+
 def solve_leg(m, ang, rtol, group_batch=1):
     """cfg2 multigroup downscatter solve with the +z beam first-collision source;
     ``group_batch`` consecutive groups per coupled solve (1 = group by group)."""
@@ -165,6 +184,7 @@ def solve_leg(m, ang, rtol, group_batch=1):
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_group_batch(group_batch)
+    tune_s = settle_autotuner(op, m, ang, group_batch)
     qmom = _beam_qmom(m, ang, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
@@ -172,8 +192,9 @@ def solve_leg(m, ang, rtol, group_batch=1):
     stats = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    choice = list(op.sweep_choice(group_batch))
     op.close()
-    return {"seconds": wall, "groups": G, "rtol": rtol,
+    return {"seconds": wall, "groups": G, "rtol": rtol, "autotune_s": tune_s, "sweep_choice": choice,
             "max_true_residual": max(s["true_residual"] for s in stats),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
             "group_batch": group_batch,
@@ -194,6 +215,7 @@ def reference_method_leg(m, ang, rtol):
     sigt, S = problems.cfg2_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
+    settle_autotuner(op, m, ang)
     qmom = _beam_qmom(m, ang, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
@@ -345,6 +367,7 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_preconditioner(params, fmt)
+    settle_autotuner(op, m, ang)
     qmom = _beam_qmom(m, ang, sigt, S)
     res = {"training": {"rows": data["rows"], "solves": len(data["solves"]),
                         "data_seconds": data["seconds"], **tinfo}}
@@ -440,7 +463,8 @@ def run(args):
         op.synchronize()
         mode, block, _ = op.sweep_choice(G)
     elif args.sweep_mode in ("direction", "direction_lds"):
-        block = 1024
+        block = args.block
+        op.set_direction_block(block)
     for _ in range(args.warmup):
         op.transport_matinv_dev(q, psi, 0, G)
     op.synchronize()
@@ -575,6 +599,8 @@ def main():
                    help="skip the learned-preconditioner solve (cfg5)")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction", "direction_lds"])
+    p.add_argument("--block", type=int, default=1024, choices=[256, 512, 1024],
+                   help="workgroup size of the direction-resident sweep modes")
     p.add_argument("--compare-modes", action="store_true")
     p.add_argument("--gemm-study", action="store_true", help="scattering-moment products on MFMA, all formats")
     args = p.parse_args()
