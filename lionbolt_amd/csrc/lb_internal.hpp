@@ -156,6 +156,8 @@ struct lb_op {
   lb::DevBuf d_upos;                // int32 [NI][NE][4]: sweep position of the upstream neighbour across
                                     // each face in that direction's order, -1 where the face is not upstream
   int lds_max = 0;                  // LDS bytes a workgroup may allocate (queried once)
+  void* blas = nullptr;             // rocBLAS handle of the FP64 moment GEMMs (created on first use)
+  lb::DevBuf w_phis;                // sigma-scaled flux moments, input of the M GEMM
   int dir_block = 1024;             // workgroup size of the direction-resident and persistent sweeps
   int per_cu = 2;                   // persistent sweep: resident workgroups per compute unit
   int group_batch = 1;              // energy groups per coupled solve in the multigroup driver
@@ -200,6 +202,7 @@ int launch_scatter_dense(lb_op* op, const double* x, double* y, int g0, int G);
 int launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* z);
 int launch_scale(hipStream_t s, int64_t n, double a, double* x);
 int launch_delta_down(lb_op* op, int64_t n, double* x);
+void release_blas(lb_op* op);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int launch_fluence(lb_op* op, const double* v, double* phi);

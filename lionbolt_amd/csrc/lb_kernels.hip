@@ -9,6 +9,7 @@
 // LU.  Vector layout: v[((i*NE + e)*4 + k)*G + g] (groups fastest) so the lanes of
 // one cell read consecutive doubles.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 
@@ -675,6 +676,23 @@ __global__ __launch_bounds__(kBlock) void moments_to_discrete_chunk_kernel(
   }
 }
 
+// out[lm][col] = sigma_{l(lm)}(mat, g) phi[lm][col]: the input of the M GEMM.
+__global__ __launch_bounds__(kBlock) void scale_moments_kernel(const double* __restrict__ phi, int nmom,
+                                                               int64_t ncols, int G, int Gtot, int g0, int L,
+                                                               const int32_t* __restrict__ deg,
+                                                               const int32_t* __restrict__ mat,
+                                                               const double* __restrict__ sigs,
+                                                               double* __restrict__ out) {
+  const int64_t n = (int64_t)nmom * ncols;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    const int lm = int(j / ncols);
+    const int64_t col = j - (int64_t)lm * ncols;
+    const int64_t sd = col / G;
+    const int g = int(col - sd * G);
+    out[j] = phi[j] * sigs[((int64_t)mat[sd >> 2] * Gtot + g0 + g) * (L + 1) + deg[lm]];
+  }
+}
+
 // Dense kernel of ScatteringAlgebra.f90:30-53: y[i][col] = sum_ip x[ip][col] YYS_m[ip][i],
 // m = material of the column; 16 outputs per pass over the column.
 __global__ __launch_bounds__(kBlock) void scatter_dense_kernel(const double* __restrict__ yys, int NI, int64_t ncols,
@@ -1090,11 +1108,74 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
       return LB_E_ARG;            \
   }
 
-// Moment counts above this go through the chunked kernels.
+// Moment counts above this go through the chunked kernels when rocBLAS is
+// not used.
 constexpr int kMaxRegisterMoments = 64;
+
+// Moment counts from which the D and M products are rocBLAS FP64 GEMMs.  At
+// these sizes the products are compute-bound (2 nmom NI flops per column
+// against 8 (NI + nmom) bytes), and a library GEMM on the matrix cores beats
+// one lane per column.
+constexpr int kBlasMinMoments = 25;
+
+static rocblas_handle blas_handle(lb_op* op, int* rc) {
+  if (!op->blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) {
+      set_error("moments: rocblas_create_handle failed");
+      *rc = LB_E_HIP;
+      return nullptr;
+    }
+    op->blas = h;
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(op->blas);
+  if (rocblas_set_stream(h, op->stream) != rocblas_status_success) {
+    set_error("moments: rocblas_set_stream failed");
+    *rc = LB_E_HIP;
+    return nullptr;
+  }
+  *rc = LB_OK;
+  return h;
+}
+
+void release_blas(lb_op* op) {
+  if (op->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(op->blas));
+  op->blas = nullptr;
+}
+
+// phi (ncols x nmom, column-major, = phi[lm][col]) = psi (ncols x NI, = psi[i][col])
+// times D^T (NI x nmom, column-major with leading dimension NI, = D[lm][i]).
+static int blas_moments(lb_op* op, const double* psi, double* phi, int64_t ncols) {
+  int rc = LB_OK;
+  rocblas_handle h = blas_handle(op, &rc);
+  if (!h) return rc;
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, ncols, op->nmom, op->NI, &one, psi, ncols,
+                       op->d_D.as<const double>(), op->NI, &zero, phi, ncols) != rocblas_status_success) {
+    set_error("moments: rocblas_dgemm_64 (D psi) failed");
+    return LB_E_HIP;
+  }
+  return LB_OK;
+}
+
+// psi (ncols x NI) (+)= phis (ncols x nmom) times M^T (nmom x NI, column-major with
+// leading dimension nmom, = M[i][lm]).
+static int blas_moments_to_discrete(lb_op* op, const double* phis, double* psi, int64_t ncols, bool accumulate) {
+  int rc = LB_OK;
+  rocblas_handle h = blas_handle(op, &rc);
+  if (!h) return rc;
+  const double one = 1.0, beta = accumulate ? 1.0 : 0.0;
+  if (rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, ncols, op->NI, op->nmom, &one, phis, ncols,
+                       op->d_M.as<const double>(), op->nmom, &beta, psi, ncols) != rocblas_status_success) {
+    set_error("moments: rocblas_dgemm_64 (M phi) failed");
+    return LB_E_HIP;
+  }
+  return LB_OK;
+}
 
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols) {
   (void)G;
+  if (op->nmom >= kBlasMinMoments) return blas_moments(op, psi, phi, ncols);
   if (op->nmom > kMaxRegisterMoments) {
     const unsigned chunks = (unsigned)((op->nmom + kMomChunk - 1) / kMomChunk);
     hipLaunchKernelGGL(moments_chunk_kernel, dim3(grid_for(ncols), chunks), dim3(kBlock), 0, op->stream,
@@ -1114,6 +1195,18 @@ int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t nco
 int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate, bool scaled) {
   const int64_t ncols = op->NENK * G;
   const double* sigs = scaled ? op->d_sigs.as<const double>() : nullptr;
+  if (op->nmom >= kBlasMinMoments) {
+    const double* in = phi;
+    if (sigs) {
+      LB_TRY(op->w_phis.reserve((size_t)op->nmom * ncols * sizeof(double)));
+      hipLaunchKernelGGL(scale_moments_kernel, dim3(grid_stride((int64_t)op->nmom * ncols)), dim3(kBlock), 0,
+                         op->stream, phi, (int)op->nmom, ncols, G, op->G, g0, op->L, op->d_deg.as<const int32_t>(),
+                         op->d_mat.as<const int32_t>(), sigs, op->w_phis.as<double>());
+      LB_HIP(hipGetLastError());
+      in = op->w_phis.as<const double>();
+    }
+    return blas_moments_to_discrete(op, in, psi, ncols, accumulate != 0);
+  }
   if (op->nmom > kMaxRegisterMoments) {
     for (int m0 = 0; m0 < op->nmom; m0 += kMomChunk)
       hipLaunchKernelGGL(moments_to_discrete_chunk_kernel, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream,
