@@ -28,9 +28,13 @@ The line also carries:
 * batch_cfg4: 64 uneven single-group S8 beam problems (2k-56k tets) swept as
   one batch (persistent work-queue kernel, autotuned) and solved with
   per-problem GMRES in lockstep;
-* cpu_baseline also carries the oracle's cfg1 solve time, a projected
-  (lower-bound) CPU time of the 24-group solve by the reference's method and
-  the oracle's sweep rate on the batch problems.
+* high_order_operator: one sweep and the two moment products at the
+  scattering orders of the reference's input decks (7, 11, 15) on an
+  82,944-tet cube;
+* cpu_baseline also carries the oracle's cfg1 solve time, projected
+  (lower-bound) CPU times of the 24-group solve by the reference's method and
+  by the GPU solve's own algorithm, and the oracle's sweep rate on the batch
+  problems.
 """
 import argparse
 import json
@@ -395,6 +399,47 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     return res
 
 
+def high_order_leg(n=24, orders=(3, 7, 11, 15), reps=5):
+    """The scattering orders of the reference's input decks (PNSCATTERING 7, 11,
+    15; SNPrep sets of 128, 288, 512 directions) on an 82,944-tet cube, one
+    group: one sweep and the two moment products of a flux-moment operator
+    application, each timed alone after the autotuner settles."""
+    import torch
+    from lionbolt_amd import angular, mesh as M
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(n, a=30.0)
+    out = {"tets": m.NE}
+    for L in orders:
+        ang = angular.snprep(L)
+        op = Operator(m, ang)
+        op.set_physics([0.5], [0.45 * 0.6 ** np.arange(L + 1)])
+        q = torch.rand(m.NE * 4 * ang.NI, dtype=torch.float64, device="cuda")
+        psi = torch.empty_like(q)
+        phi = torch.empty(ang.n_moments * m.NENK, dtype=torch.float64, device="cuda")
+        settle_autotuner(op, m, ang)
+
+        def timed(f):
+            f()
+            op.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f()
+            op.synchronize()
+            return 1000.0 * (time.perf_counter() - t0) / reps
+
+        ts = timed(lambda: op.transport_matinv_dev(q, psi, 0, 1))
+        td = timed(lambda: op.moments_dev(psi, phi))
+        tm = timed(lambda: op.moments_to_discrete_dev(phi, q))
+        flops = 2.0 * ang.n_moments * ang.NI * m.NENK
+        out[f"L{L}"] = {"directions": ang.NI, "moments": ang.n_moments, "sweep_ms": ts, "d_psi_ms": td,
+                        "m_phi_ms": tm, "d_psi_tflops": flops / td / 1e9, "m_phi_tflops": flops / tm / 1e9,
+                        "sweep_choice": list(op.sweep_choice(1))}
+        op.close()
+        del q, psi, phi
+        torch.cuda.empty_cache()
+    return out
+
+
 def single_group_leg(m, ang, reps=10):
     """One-group sweeps of the same mesh and directions (what a within-group solve
     issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s."""
@@ -545,6 +590,8 @@ def run(args):
         out["other_sweep_modes"] = other_modes
     if args.single:
         out["single_group_sweep"] = single_group_leg(m, ang)
+    if args.high_order:
+        out["high_order_operator"] = high_order_leg()
     if args.solve:
         runs = [solve_leg(m, ang, args.solve_rtol, b) for b in args.group_batches]
         out["solve"] = min(runs, key=lambda r: r["seconds"])
@@ -591,6 +638,8 @@ def main():
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--no-solve", dest="solve", action="store_false")
     p.add_argument("--no-single", dest="single", action="store_false")
+    p.add_argument("--no-high-order", dest="high_order", action="store_false",
+                   help="skip the scattering orders 7, 11 and 15 of the reference's decks")
     p.add_argument("--solve-rtol", type=float, default=1e-7)
     p.add_argument("--group-batches", type=lambda t: [int(x) for x in t.split(",")], default=[1],
                    help="energy groups per coupled solve to try in the solve leg (e.g. 1,2,4)")
