@@ -1112,11 +1112,12 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
 // not used.
 constexpr int kMaxRegisterMoments = 64;
 
-// Moment counts from which the D and M products are rocBLAS FP64 GEMMs.  At
-// these sizes the products are compute-bound (2 nmom NI flops per column
-// against 8 (NI + nmom) bytes), and a library GEMM on the matrix cores beats
-// one lane per column.
-constexpr int kBlasMinMoments = 25;
+// Moment counts from which the D and M products are rocBLAS FP64 GEMMs.  From
+// L = 4 on the products are compute-bound (2 nmom NI flops per column against
+// 8 (NI + nmom) bytes) and the matrix cores win; at L = 3 they are HBM-bound
+// and the GEMM still streams closer to the copy rate (measured on 384k tets x
+// 256 directions: D psi 1.03 -> 0.69 ms, M phi 1.03 -> 0.84 ms).
+constexpr int kBlasMinMoments = 16;
 
 static rocblas_handle blas_handle(lb_op* op, int* rc) {
   if (!op->blas) {

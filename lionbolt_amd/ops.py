@@ -13,6 +13,7 @@ every failure raises ``LionboltError`` where the reference calls ``stophere``.
 the device-resident entry points.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -68,7 +69,10 @@ class Operator:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the ctypes binding and the HIP runtime may already be
+        # gone; the process releases the device memory itself
+        if not sys.is_finalizing():
+            self.close()
 
     # -- physics ---------------------------------------------------------------
     def set_physics(self, sigt, sigs):
