@@ -1112,12 +1112,13 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
 // not used.
 constexpr int kMaxRegisterMoments = 64;
 
-// Moment counts from which the D and M products are rocBLAS FP64 GEMMs.  From
-// L = 4 on the products are compute-bound (2 nmom NI flops per column against
-// 8 (NI + nmom) bytes) and the matrix cores win; at L = 3 they are HBM-bound
-// and the GEMM still streams closer to the copy rate (measured on 384k tets x
-// 256 directions: D psi 1.03 -> 0.69 ms, M phi 1.03 -> 0.84 ms).
-constexpr int kBlasMinMoments = 16;
+// When the D and M products are rocBLAS FP64 GEMMs.  From L = 4 on they are
+// compute-bound (2 nmom NI flops per column against 8 (NI + nmom) bytes) and
+// the matrix cores win.  At L = 3 they are HBM-bound: with 256 directions the
+// GEMM still streams closer to the copy rate (384k tets: D psi 1.03 -> 0.69
+// ms, M phi 1.03 -> 0.84 ms), with 32 directions one lane per column is
+// faster (83k tets: 0.05 against 0.12 ms).
+static bool use_blas(const lb_op* op) { return op->nmom >= 25 || (op->nmom >= 16 && op->NI >= 128); }
 
 static rocblas_handle blas_handle(lb_op* op, int* rc) {
   if (!op->blas) {
@@ -1176,7 +1177,7 @@ static int blas_moments_to_discrete(lb_op* op, const double* phis, double* psi, 
 
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols) {
   (void)G;
-  if (op->nmom >= kBlasMinMoments) return blas_moments(op, psi, phi, ncols);
+  if (use_blas(op)) return blas_moments(op, psi, phi, ncols);
   if (op->nmom > kMaxRegisterMoments) {
     const unsigned chunks = (unsigned)((op->nmom + kMomChunk - 1) / kMomChunk);
     hipLaunchKernelGGL(moments_chunk_kernel, dim3(grid_for(ncols), chunks), dim3(kBlock), 0, op->stream,
@@ -1196,7 +1197,7 @@ int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t nco
 int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate, bool scaled) {
   const int64_t ncols = op->NENK * G;
   const double* sigs = scaled ? op->d_sigs.as<const double>() : nullptr;
-  if (op->nmom >= kBlasMinMoments) {
+  if (use_blas(op)) {
     const double* in = phi;
     if (sigs) {
       LB_TRY(op->w_phis.reserve((size_t)op->nmom * ncols * sizeof(double)));
