@@ -68,6 +68,9 @@ typedef struct {
   int index_base;        /* 1: Fortran indices, boundary neighbour 0; 0: C indices, boundary -1 */
 } lb_mesh_desc;
 
+/* Most elements of one operator: sweep records keep a cell index in 24 bits. */
+#define LB_MAX_ELEMENTS 16777215
+
 /* Highest scattering order: PNSCATTERING 15, the highest the reference's
  * input decks use (validation/electrons_incident/Au_1MeV_0deg/lb.in and its siblings). */
 #define LB_MAX_ORDER 15

@@ -131,6 +131,11 @@ int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out
     set_error("lb_op_create: invalid sizes (need NE, NK, NI > 0, 0 <= L <= 15, index_base 0 or 1)");
     return LB_E_ARG;
   }
+  if (mesh->NE > LB_MAX_ELEMENTS) {
+    set_error("lb_op_create: at most 16,777,215 elements per operator (the sweep records hold 24-bit cell "
+              "indices); split larger meshes into problems of one batch or several operators");
+    return LB_E_ARG;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     (void)hipGetLastError();
@@ -163,7 +168,8 @@ int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out
 
 int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t* sl, int32_t* nlev, double* normals) {
   if (!mesh || !quad || !mesh->rg || !mesh->node || !mesh->nbr || !mesh->mat || !quad->k || !quad->w ||
-      mesh->NE <= 0 || quad->NI <= 0 || (mesh->index_base != 0 && mesh->index_base != 1)) {
+      mesh->NE <= 0 || mesh->NE > LB_MAX_ELEMENTS || quad->NI <= 0 ||
+      (mesh->index_base != 0 && mesh->index_base != 1)) {
     set_error("lb_schedule_host: invalid argument");
     return LB_E_ARG;
   }

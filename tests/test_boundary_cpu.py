@@ -50,6 +50,29 @@ def test_create_without_device_fails_loudly():
     assert rc == -4 and b"device" in _lib.lib().lb_last_error()
 
 
+def test_size_limits_are_rejected_before_any_work():
+    """Argument checks run before the device is touched: more elements than the
+    24-bit cell index of the sweep records, a scattering order above 15, an
+    empty mesh."""
+    import ctypes
+    m = M.jittered_cube(1)
+    ang = angular.snprep(1)
+    keep = [np.ascontiguousarray(a) for a in (m.rg, m.node, m.nbr, m.mat, ang.k, ang.w)]
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    for NE, NI, order, msg in ((1 << 24, ang.NI, 1, b"16,777,215"), (m.NE, ang.NI, 16, b"L <= 15"),
+                               (0, ang.NI, 1, b"invalid sizes"), (m.NE, 0, 1, b"invalid sizes")):
+        md = _lib.MeshDesc(NE, m.NK, _lib.dptr(keep[0]), _lib.i64ptr(keep[1]), _lib.i64ptr(keep[2]),
+                           _lib.i64ptr(keep[3]), 0)
+        qd = _lib.QuadDesc(NI, _lib.dptr(keep[4]), _lib.dptr(keep[5]), order)
+        assert L.lb_op_create(ctypes.byref(md), ctypes.byref(qd), ctypes.byref(h)) == -1
+        assert msg in L.lb_last_error()
+    md = _lib.MeshDesc(1 << 24, m.NK, _lib.dptr(keep[0]), _lib.i64ptr(keep[1]), _lib.i64ptr(keep[2]),
+                       _lib.i64ptr(keep[3]), 0)
+    qd = _lib.QuadDesc(ang.NI, _lib.dptr(keep[4]), _lib.dptr(keep[5]), 1)
+    assert L.lb_schedule_host(ctypes.byref(md), ctypes.byref(qd), None, None, None) == -1
+
+
 @pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cfg1_s8", "grazing_s8"])
 def test_host_schedule_and_normals_bit_identical_to_oracle(case, small_mesh, cfg1_mesh):
     if case == "grazing_s8":
