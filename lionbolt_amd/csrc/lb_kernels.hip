@@ -873,17 +873,25 @@ static int trace_ring_size(lb_op* op, int G, int* mask, size_t* bytes) {
 static int ensure_upstream_positions(lb_op* op) {
   if (op->d_upos.p) return LB_OK;
   const int64_t n = op->NI * op->NE;
-  DevBuf pos;
-  LB_TRY(pos.reserve(n * sizeof(int32_t)));
-  LB_TRY(op->d_upos.reserve(n * sizeof(int4)));
+  struct Scratch {
+    DevBuf b;
+    ~Scratch() { b.release(); }   // also on the error returns below
+  } pos;
+  LB_TRY(pos.b.reserve(n * sizeof(int32_t)));
+  int rc = op->d_upos.reserve(n * sizeof(int4));
+  if (rc != LB_OK) return rc;
   hipLaunchKernelGGL(sweep_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_tasks.as<const uint32_t>(), pos.as<int32_t>());
+                     op->d_tasks.as<const uint32_t>(), pos.b.as<int32_t>());
   hipLaunchKernelGGL(upstream_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_tasks.as<const uint32_t>(), op->d_nbr.as<const int4>(), pos.as<const int32_t>(),
+                     op->d_tasks.as<const uint32_t>(), op->d_nbr.as<const int4>(), pos.b.as<const int32_t>(),
                      op->d_upos.as<int4>());
-  LB_HIP(hipGetLastError());
-  LB_HIP(hipStreamSynchronize(op->stream));
-  pos.release();
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(op->stream);   // the scratch is freed on return
+  if (e != hipSuccess) {
+    op->d_upos.release();
+    set_error(std::string("upstream positions: ") + hipGetErrorString(e));
+    return LB_E_HIP;
+  }
   return LB_OK;
 }
 
