@@ -166,9 +166,9 @@ int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off);
  * root-mean-square s), log(sigma_t h) and sigma_s0/sigma_t of the dof's element
  * (h = (6 V)^(1/3)); outputs the error's 16 moments there (times s).  Used as
  * the right preconditioner z = P(v) of flexible GMRES(m) on flux moments
- * (LB_SOLVER_FGMRES); when the residual estimate falls by less than half over
- * 10 Arnoldi steps, or P returns a non-finite vector, the solve continues as
- * plain GMRES.  Parameters: W1[18][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+ * (LB_SOLVER_FGMRES); when the residual estimate falls by less than a factor
+ * 8 over the last three Arnoldi steps, or P returns a non-finite or zero
+ * vector, the solve continues as plain GMRES.  Parameters: W1[18][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
  * W4[64][16] b4[16], FP32, W[i][j] from input i to output j. */
 #define LB_MLP_PARAMS 10576
 
