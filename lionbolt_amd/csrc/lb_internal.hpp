@@ -179,6 +179,7 @@ struct lb_op {
   float last_sweep_ms = 0.f;
   // workspaces
   lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host, w_mg, w_acc, w_S;
+  lb::DevBuf w_tq;                  // T^-1 q of the current solve, for its true residual
 };
 
 namespace lb {

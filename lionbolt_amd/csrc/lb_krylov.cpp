@@ -37,8 +37,10 @@ void print_row(int verbose, int iter, double v) {
 }
 
 // Restarted GMRES(m) on device vectors of length n.  V holds m+1 vectors.
+// With zero_x0 the solve starts from x0 = 0 (sol is cleared), so the first
+// residual is rhs itself and costs no operator application.
 int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* sol, double* V, int m, double rtol,
-          double atol, int maxit, int verbose, GmresResult* res) {
+          double atol, int maxit, int verbose, GmresResult* res, bool zero_x0 = false) {
   hipStream_t s = op->stream;
   std::vector<double> H((size_t)(m + 1) * (m + 2) / 2 + 2, 0.0), c(m + 2), sn(m + 2), rs(m + 3);
   auto hcol = [&](int k) { return &H[(size_t)k * (k - 1) / 2]; };
@@ -59,12 +61,19 @@ int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* s
   }
   ret(0.0);
   int status = 0;
+  if (zero_x0) LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
+  bool initial = zero_x0;
   while (true) {
     double* v0 = vec(0);
-    LB_TRY(launch_copy(s, n, sol, v0));
-    LB_TRY(apply(v0));
-    ++nmv;
-    LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    if (initial) {
+      LB_TRY(launch_copy(s, n, rhs, v0));   // r0 = rhs - A 0
+      initial = false;
+    } else {
+      LB_TRY(launch_copy(s, n, sol, v0));
+      LB_TRY(apply(v0));
+      ++nmv;
+      LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    }
     double a2;
     LB_TRY(dot(op, n, v0, v0, &a2));
     double alpha = std::sqrt(a2);
@@ -252,7 +261,7 @@ struct FgmresResult {
 // inverse does better than that), or when P returns a non-finite or zero vector.
 int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
            double* Z, double* R, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
-           FgmresResult* res) {
+           FgmresResult* res, bool zero_x0 = false) {
   hipStream_t s = op->stream;
   auto vec = [&](int j) { return V + (int64_t)j * n; };
   auto zvec = [&](int j) { return Z + (int64_t)j * n; };
@@ -276,12 +285,19 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
     }
   };
   bool done = false;
+  if (zero_x0) LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
+  bool initial = zero_x0;   // x0 = 0: the first residual is rhs, no operator application
   while (!done) {
     double* v0 = vec(0);
-    LB_TRY(launch_copy(s, n, sol, v0));
-    LB_TRY(apply(v0));
-    ++nmv;
-    LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    if (initial) {
+      LB_TRY(launch_copy(s, n, rhs, v0));
+      initial = false;
+    } else {
+      LB_TRY(launch_copy(s, n, sol, v0));
+      LB_TRY(apply(v0));
+      ++nmv;
+      LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
+    }
     double b2 = 0.0;
     LB_TRY(dot(op, n, v0, v0, &b2));
     const double beta = std::sqrt(b2);
@@ -495,10 +511,13 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
   hipStream_t str = op->stream;
   // keep q for the moment-space reconstruction, then b = T^-1 q
   LB_TRY(op->w_b.reserve(n * sizeof(double)));
+  LB_TRY(op->w_tq.reserve(n * sizeof(double)));
   double* q = op->w_b.as<double>();
   LB_TRY(launch_copy(str, n, s, q));
   LB_TRY(sweep_inplace(op, s, g0));
   ++sweeps;
+  double* tq = op->w_tq.as<double>();   // T^-1 q, kept for the true residual
+  LB_TRY(launch_copy(str, n, s, tq));
   const double sig1 = op->h_sigt[(size_t)0 * op->G + g0];  // XS%t(1) of this group
 
   if (opt.kind == LB_SOLVER_SI) {
@@ -575,15 +594,14 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     double* rhs = phi + nphi;
     double* R = rhs + nphi;
     LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
-    LB_TRY(launch_copy(str, nphi, rhs, phi));
-    LB_TRY(launch_scale(str, nphi, 1.0 / sig1, phi));
     Apply A = [&](double* v) -> int {
       ++sweeps;
       return moment_matvec_dev(op, v, g0);
     };
     Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g0); };
     FgmresResult r;
-    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r));
+    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r,
+                  true));
     LB_TRY(launch_copy(str, n, q, s));
     LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
     LB_TRY(sweep_inplace(op, s, g0));
@@ -605,8 +623,6 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     double* rhs = phi + nphi;
     double* psi_t = op->w_a2.as<double>();
     LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
-    LB_TRY(launch_copy(str, nphi, rhs, phi));
-    LB_TRY(launch_scale(str, nphi, 1.0 / sig1, phi));
     GmresResult r;
     Apply A = [&](double* v) -> int {
       ++sweeps;
@@ -617,7 +633,8 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
       LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
       return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
     };
-    LB_TRY(gmres(op, nphi, A, rhs, phi, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r));
+    // x0 = 0 (the psi-space solver keeps the reference's q / sigma_t(1))
+    LB_TRY(gmres(op, nphi, A, rhs, phi, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r, true));
     // psi = T^-1 (M S phi + q)
     LB_TRY(launch_copy(str, n, q, s));
     LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
@@ -629,15 +646,12 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     stats.final_norm = r.final_norm;
   }
   // true transport residual ||T^-1 q - (I - T^-1 K) psi|| / ||T^-1 q||, with
-  // T^-1 q recomputed from the saved source
+  // the T^-1 q of the first sweep
   {
-    LB_TRY(op->w_a2.reserve(n * sizeof(double)));
-    double* b = op->w_a2.as<double>();
-    LB_TRY(launch_copy(str, n, q, b));
-    LB_TRY(sweep_inplace(op, b, g0));
+    const double* b = tq;
     LB_TRY(launch_copy(str, n, s, q));  // q no longer needed: reuse as A psi
     LB_TRY(boltzmann_matvec_dev(op, q, g0));
-    sweeps += 2;
+    sweeps += 1;
     double bb;
     LB_TRY(dot(op, n, b, b, &bb));
     LB_TRY(launch_axpby(str, n, 1.0, b, -1.0, q, q));
