@@ -557,6 +557,7 @@ void lb_op_destroy(lb_op* op) {
   op->d_xsdd.release();
   op->w_phis.release();
   op->w_tq.release();
+  op->w_tail.release();
   release_blas(op);
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);

@@ -77,6 +77,19 @@ __global__ __launch_bounds__(kBlock) void batch_store_kernel(const double* __res
   phi_all[((int64_t)(g0 + g) * nmom + lm) * NENK + sd] = Phi[j];
 }
 
+// Phi[lm][sd*B+g] = phi_all[g0+g][lm][sd] (the inverse of batch_store_kernel)
+__global__ __launch_bounds__(kBlock) void batch_gather_kernel(const double* __restrict__ phi_all, int g0, int B,
+                                                              int nmom, int64_t NENK, double* __restrict__ Phi) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= (int64_t)nmom * NENK * B) return;
+  const int64_t row = NENK * B;
+  const int lm = int(j / row);
+  const int64_t rem = j - (int64_t)lm * row;
+  const int64_t sd = rem / B;
+  const int g = int(rem - sd * B);
+  Phi[j] = phi_all[((int64_t)(g0 + g) * nmom + lm) * NENK + sd];
+}
+
 struct GroupWeights {
   double w[kMaxGroupBatch];
 };
@@ -154,6 +167,14 @@ int launch_batch_store(lb_op* op, const double* Phi, int g0, int B, double* phi_
   const int64_t tot = (int64_t)op->nmom * op->NENK * B;
   hipLaunchKernelGGL(batch_store_kernel, dim3(grid_for(tot)), dim3(kBlock), 0, op->stream, Phi, g0, B, (int)op->nmom,
                      op->NENK, phi_all);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_batch_gather(lb_op* op, const double* phi_all, int g0, int B, double* Phi) {
+  const int64_t tot = (int64_t)op->nmom * op->NENK * B;
+  hipLaunchKernelGGL(batch_gather_kernel, dim3(grid_for(tot)), dim3(kBlock), 0, op->stream, phi_all, g0, B,
+                     (int)op->nmom, op->NENK, Phi);
   LB_HIP(hipGetLastError());
   return LB_OK;
 }

@@ -180,6 +180,7 @@ struct lb_op {
   // workspaces
   lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host, w_mg, w_acc, w_S;
   lb::DevBuf w_tq;                  // T^-1 q of the current solve, for its true residual
+  lb::DevBuf w_tail;                // batched residual checks of the multigroup solve (freed after it)
 };
 
 namespace lb {
@@ -194,6 +195,8 @@ LevelTable* level_table(lb_op* op, int G, int* rc);
 
 // lb_kernels.hip (launchers; all enqueue on op->stream)
 int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G);
+// One sweep with the direction-resident kernel at a fixed workgroup size (no autotuning).
+int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int G, int block);
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols);
 int launch_moments_to_discrete(lb_op* op, const double* phi, double* psi, int g0, int G, int accumulate,
                                bool scaled = true);
@@ -213,6 +216,7 @@ int launch_batch_source(lb_op* op, const double* phi_all, const double* S, const
                         double* acc);
 int launch_batch_transfer(lb_op* op, const double* S, int g0, int B, const double* v, double* out);
 int launch_batch_store(lb_op* op, const double* Phi, int g0, int B, double* phi_all);
+int launch_batch_gather(lb_op* op, const double* phi_all, int g0, int B, double* Phi);
 int launch_group_scale(lb_op* op, int64_t n, int B, const double* w, double* x);
 int group_dot(lb_op* op, int64_t n, int B, const double* x, const double* y, double* out);
 

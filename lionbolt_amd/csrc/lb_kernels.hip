@@ -1067,6 +1067,14 @@ int launch_delta_down(lb_op* op, int64_t n, double* x) {
   return LB_OK;
 }
 
+int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int G, int block) {
+  if (G < 1 || G > block) {
+    set_error("sweep: the group batch does not fit the workgroup");
+    return LB_E_ARG;
+  }
+  return launch_sweep_as(op, q, psi, g0, G, SweepConfig{LB_SWEEP_DIRECTION, block, 0});
+}
+
 int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   if (G < 1 || G > kBlock) {
     set_error("sweep: the group batch must be between 1 and 256");

@@ -795,6 +795,132 @@ static int multigroup_batched(lb_op* op, const double* qmom, const double* S, co
   return rc_all;
 }
 
+// Group-by-group flux-moment GMRES (or flexible GMRES with the learned
+// preconditioner) with the residual checks batched over groups.
+// Each group g solves (I - D T^-1 M S_g) phi_g = D T^-1 q_g from phi_g = 0, its
+// source q_g = M (qmom_g + sum_{g' < g} S_{g'->g} phi_g') built from the groups
+// already solved; phi_g is the result.  The true transport residual of
+// psi_g = T^-1 (M S_g phi_g + q_g) is
+//   T^-1 q_g - (I - T^-1 K_g) psi_g = T^-1 M S_g (D psi_g - phi_g),
+// so after all groups are solved, psi and that residual take two sweeps for
+// kTailGroups groups at a time instead of two single-group sweeps per group.
+static constexpr int kTailGroups = 8;
+
+static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* S, const lb_solver_opts& opt,
+                                   double* phi_out, lb_solve_stats* stats) {
+  const int G = op->G;
+  const int64_t n = op->NENK * op->NI;
+  const int64_t nphi = (int64_t)op->nmom * op->NENK;
+  hipStream_t str = op->stream;
+  const int m = opt.restart > 0 ? opt.restart : 30;
+  const bool flexible = opt.kind == LB_SOLVER_FGMRES;
+  if (flexible && !op->mlp) {
+    set_error("FGMRES : no preconditioner set (lb_op_set_preconditioner)");
+    return LB_E_NOT_READY;
+  }
+  LB_TRY(op->w_mg.reserve(n * sizeof(double)));
+  LB_TRY(op->w_acc.reserve(nphi * sizeof(double)));
+  LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+  // V (m+1), rhs, and for the flexible solver Z (m) and the carried residual R
+  LB_TRY(op->w_krylov.reserve((size_t)(flexible ? 2 * m + 3 : m + 2) * nphi * sizeof(double)));
+  double* q = op->w_mg.as<double>();
+  double* acc = op->w_acc.as<double>();
+  double* psi_t = op->w_a2.as<double>();
+  double* V = op->w_krylov.as<double>();
+  double* rhs = V + (int64_t)(m + 1) * nphi;
+  double* Z = rhs + nphi;
+  double* R = Z + (int64_t)m * nphi;
+  std::vector<double> tq_norm(G, 0.0);
+  std::vector<lb_solve_stats> st(G);
+  int rc_all = LB_OK;
+  for (int g = 0; g < G; ++g) {
+    auto t0 = std::chrono::steady_clock::now();
+    LB_TRY(launch_downscatter(op, phi_out, S, g, G, qmom ? qmom + (int64_t)g * nphi : nullptr, acc));
+    LB_TRY(launch_moments_to_discrete(op, acc, q, g, 1, 0, false));
+    LB_TRY(sweep_inplace(op, q, g));          // T^-1 q_g
+    double tq2;
+    LB_TRY(dot(op, n, q, q, &tq2));
+    tq_norm[g] = std::sqrt(tq2);
+    LB_TRY(launch_moments(op, q, rhs, 1, op->NENK));
+    int sweeps = 1;
+    Apply A = [&](double* v) -> int {
+      ++sweeps;
+      LB_TRY(launch_moments_to_discrete(op, v, psi_t, g, 1, 0));
+      LB_TRY(sweep_inplace(op, psi_t, g));
+      LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
+      double* tphi = op->w_phi.as<double>();
+      LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+      return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
+    };
+    double* phi_g = phi_out + (int64_t)g * nphi;
+    if (flexible) {
+      Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g); };
+      FgmresResult r;
+      LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi_g, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr,
+                    &r, true));
+      st[g].status = r.status;
+      st[g].iterates = r.iterates;
+      st[g].matvecs = r.matvecs;
+      st[g].final_norm = r.final_norm;
+      st[g].fallback = r.fallback;
+    } else {
+      GmresResult r;
+      LB_TRY(gmres(op, nphi, A, rhs, phi_g, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r, true));
+      st[g].status = r.status;
+      st[g].iterates = r.iterates;
+      st[g].matvecs = r.matvecs;
+      st[g].final_norm = r.final_norm;
+    }
+    st[g].sweeps = sweeps;
+    st[g].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st[g].status != 0) rc_all = st[g].status;
+  }
+  // psi and the true residual, kTailGroups groups per sweep
+  const int B0 = std::min(kTailGroups, G);
+  const int64_t nb = n * B0, pb = nphi * B0;
+  LB_TRY(op->w_tail.reserve((size_t)(2 * nb + 3 * pb) * sizeof(double)));
+  double* psi_b = op->w_tail.as<double>();
+  double* r_b = psi_b + nb;
+  double* Phi = r_b + nb;
+  double* T = Phi + pb;
+  double* tmp = T + pb;
+  for (int c0 = 0; c0 < G; c0 += B0) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int B = std::min(B0, G - c0);
+    const int64_t nB = n * B, pB = nphi * B;
+    LB_TRY(launch_batch_gather(op, phi_out, c0, B, Phi));
+    LB_TRY(launch_batch_source(op, phi_out, S, qmom, c0, B, T));      // qmom + downscatter from g' < c0
+    LB_TRY(launch_batch_transfer(op, S, c0, B, Phi, tmp));             // in-chunk g' <= g, within-group included
+    LB_TRY(launch_axpby(str, pB, 1.0, tmp, 1.0, T, T));
+    LB_TRY(launch_moments_to_discrete(op, T, psi_b, c0, B, 0, false));
+    LB_TRY(launch_sweep_direction(op, psi_b, psi_b, c0, B, 1024));    // psi = T^-1 (M S phi + q)
+    LB_TRY(launch_moments(op, psi_b, tmp, B, op->NENK * B));           // D psi
+    LB_TRY(launch_axpby(str, pB, 1.0, tmp, -1.0, Phi, tmp));            // D psi - phi
+    LB_TRY(launch_moments_to_discrete(op, tmp, r_b, c0, B, 0, true));  // M S_g (D psi - phi)
+    LB_TRY(launch_sweep_direction(op, r_b, r_b, c0, B, 1024));
+    double rr[kTailGroups];
+    LB_TRY(group_dot(op, nB, B, r_b, r_b, rr));
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int b = 0; b < B; ++b) {
+      lb_solve_stats& sg = st[c0 + b];
+      sg.true_residual = tq_norm[c0 + b] > 0.0 ? std::sqrt(rr[b]) / tq_norm[c0 + b] : std::sqrt(rr[b]);
+      sg.sweeps += 2;
+      sg.seconds += dt / B;
+    }
+  }
+  op->w_tail.release();
+  LB_HIP(hipStreamSynchronize(str));
+  if (stats)
+    for (int g = 0; g < G; ++g) stats[g] = st[g];
+  if (rc_all != LB_OK) {
+    set_error(std::string(flexible ? "FGMRES : " : "GMRES : ") +
+              (rc_all == LB_E_MAXIT ? "did not converge in the iteration limit"
+                                         : rc_all == LB_E_BREAKDOWN ? "anticipated breakdown / division by zero"
+                                                                    : "RESIDUAL NORM is NaN"));
+  }
+  return rc_all;
+}
+
 int multigroup_solve_dev(lb_op* op, const double* qmom, const double* transfer, const lb_solver_opts* opts,
                          double* phi_out, lb_solve_stats* stats) {
   const int G = op->G;
@@ -804,6 +930,8 @@ int multigroup_solve_dev(lb_op* op, const double* qmom, const double* transfer, 
   LB_TRY(op->w_S.reserve(nS * sizeof(double)));
   LB_HIP(hipMemcpyAsync(op->w_S.p, transfer, nS * sizeof(double), hipMemcpyHostToDevice, op->stream));
   if (op->group_batch > 1) return multigroup_batched(op, qmom, op->w_S.as<double>(), opts, phi_out, stats);
+  if (opts && opts->space == LB_SPACE_MOMENT && (opts->kind == LB_SOLVER_GMRES || opts->kind == LB_SOLVER_FGMRES))
+    return multigroup_moment_gmres(op, qmom, op->w_S.as<double>(), *opts, phi_out, stats);
   LB_TRY(op->w_mg.reserve(n * sizeof(double)));
   LB_TRY(op->w_acc.reserve(nphi * sizeof(double)));
   double* psi = op->w_mg.as<double>();
