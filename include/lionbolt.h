@@ -281,7 +281,12 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
  * each a within-group BoltzmannOp%MatInv with source
  * M (qmom_g + sum_{g'<g} sigma_l(g'->g) phi_g'), flux moments kept in HBM.
  * d_qmom [G][nmom][NENK] or NULL; transfer (host) [G][G][Nm][L+1] indexed
- * [g'][g]; d_phi [G][nmom][NENK] out; stats[G] out.  Synchronous. */
+ * [g'][g]; d_phi [G][nmom][NENK] out; stats[G] out.  Synchronous.  With the
+ * flux-moment solvers (LB_SPACE_MOMENT, GMRES or FGMRES) d_phi holds each
+ * group's solution phi_g, and psi_g = T^-1 (M S_g phi_g + q_g) and its true
+ * transport residual are evaluated after all groups, 8 groups per sweep
+ * (each group's stats count those two sweeps).  Otherwise d_phi = D psi_g of
+ * each group's BoltzmannOp%MatInv. */
 int lb_multigroup_solve_dev(lb_op* op, const double* d_qmom, const double* transfer, const lb_solver_opts* opts,
                             double* d_phi, lb_solve_stats* stats);
 
