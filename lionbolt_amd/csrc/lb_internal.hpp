@@ -209,6 +209,11 @@ int launch_delta_down(lb_op* op, int64_t n, double* x);
 void release_blas(lb_op* op);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
+// out[i] = <w, V_i> for i < c (V_i = V + i ld) and, with self, out[c] = <w, w>;
+// one host synchronisation.
+int multi_dot(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* w, bool self, double* out);
+// w <- (w - sum_{i<c} h_i V_i) * scale
+int multi_axpy(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* h, double scale, double* w);
 int launch_fluence(lb_op* op, const double* v, double* phi);
 
 // lb_groups.hip (group-batched multigroup solve)

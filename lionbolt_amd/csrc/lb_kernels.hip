@@ -787,6 +787,80 @@ __global__ __launch_bounds__(kBlock) void dot_final_kernel(int nparts, const dou
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// Fused dots and updates against up to kOrthoChunk basis vectors at a time
+// (V_i = V + i ld): one read of w serves every dot of a chunk, and one
+// read-write of w every subtraction.  Partial sums per block reduce inside each
+// wave by a fixed butterfly and across the block's waves in wave order, then
+// over blocks in block order, so results do not vary from run to run.
+constexpr int kOrthoChunk = 8;
+constexpr int kOrthoBlocks = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// part[(slot0 + i) kOrthoBlocks + block] = partial <w, V_i> for i < c; with
+// self_slot >= 0 also part[self_slot kOrthoBlocks + block] = partial <w, w>.
+__global__ __launch_bounds__(kBlock) void multi_dot_partial_kernel(int64_t n, int c, const double* __restrict__ V,
+                                                                   int64_t ld, const double* __restrict__ w,
+                                                                   int slot0, int self_slot,
+                                                                   double* __restrict__ part) {
+  __shared__ double sh[kOrthoChunk + 1][kBlock / 64];
+  double acc[kOrthoChunk + 1];
+#pragma unroll
+  for (int i = 0; i <= kOrthoChunk; ++i) acc[i] = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    const double wj = w[j];
+#pragma unroll
+    for (int i = 0; i < kOrthoChunk; ++i)
+      if (i < c) acc[i] += wj * V[i * ld + j];
+    acc[kOrthoChunk] += wj * wj;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i <= kOrthoChunk; ++i) {
+    const double s = wave_sum(acc[i]);
+    if (lane == 0) sh[i][wave] = s;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < c || (t == c && self_slot >= 0)) {
+    const int i = t < c ? t : kOrthoChunk;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) s += sh[i][q];
+    part[(int64_t)(t < c ? slot0 + t : self_slot) * kOrthoBlocks + blockIdx.x] = s;
+  }
+}
+
+// out[slot] = sum over blocks of part[slot kOrthoBlocks + block]; one block per slot.
+__global__ __launch_bounds__(kBlock) void multi_dot_final_kernel(const double* __restrict__ part,
+                                                                 double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < kOrthoBlocks; j += kBlock) acc += part[(int64_t)blockIdx.x * kOrthoBlocks + j];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+struct OrthoCoef {
+  double h[kOrthoChunk];
+};
+
+// w <- (w - sum_{i<c} h_i V_i) * scale
+__global__ __launch_bounds__(kBlock) void multi_axpy_kernel(int64_t n, int c, const double* __restrict__ V, int64_t ld,
+                                                            OrthoCoef h, double scale, double* __restrict__ w) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    double a = w[j];
+#pragma unroll
+    for (int i = 0; i < kOrthoChunk; ++i)
+      if (i < c) a -= h.h[i] * V[i * ld + j];
+    w[j] = a * scale;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void fluence_kernel(const double* __restrict__ w, int NI, int64_t NENK,
                                                          const double* __restrict__ v, double* __restrict__ phi) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1298,6 +1372,48 @@ int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
   *out = *static_cast<double*>(op->w_host.p);
+  return LB_OK;
+}
+
+int multi_dot(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* w, bool self, double* out) {
+  const int slots = c + (self ? 1 : 0);
+  if (c < 0 || slots < 1) {
+    set_error("multi_dot: no vectors");
+    return LB_E_ARG;
+  }
+  LB_TRY(op->w_red.reserve((size_t)slots * (kOrthoBlocks + 1) * sizeof(double)));
+  if (!op->w_host.p || op->w_host.bytes < slots * sizeof(double)) {
+    if (op->w_host.p) (void)hipHostFree(op->w_host.p);
+    op->w_host.p = nullptr;
+    const size_t want = std::max<size_t>(slots, kMaxGroupBatch) * sizeof(double);
+    LB_HIP(hipHostMalloc(&op->w_host.p, want, hipHostMallocDefault));
+    op->w_host.bytes = want;
+  }
+  double* part = op->w_red.as<double>();
+  double* res = part + (size_t)slots * kOrthoBlocks;
+  for (int i0 = 0; i0 < c || (i0 == 0 && self); i0 += kOrthoChunk) {
+    const int cc = std::min(kOrthoChunk, c - i0);
+    hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(kOrthoBlocks), dim3(kBlock), 0, op->stream, n, cc,
+                       V + (int64_t)i0 * ld, ld, w, i0, (self && i0 == 0) ? c : -1, part);
+  }
+  hipLaunchKernelGGL(multi_dot_final_kernel, dim3(slots), dim3(kBlock), 0, op->stream, part, res);
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipMemcpyAsync(op->w_host.p, res, slots * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  LB_HIP(hipStreamSynchronize(op->stream));
+  std::copy(static_cast<double*>(op->w_host.p), static_cast<double*>(op->w_host.p) + slots, out);
+  return LB_OK;
+}
+
+int multi_axpy(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* h, double scale, double* w) {
+  if (c == 0) return scale == 1.0 ? LB_OK : launch_scale(op->stream, n, scale, w);
+  for (int i0 = 0; i0 < c; i0 += kOrthoChunk) {
+    const int cc = std::min(kOrthoChunk, c - i0);
+    OrthoCoef k{};
+    for (int i = 0; i < cc; ++i) k.h[i] = h[i0 + i];
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_stride(n)), dim3(kBlock), 0, op->stream, n, cc,
+                       V + (int64_t)i0 * ld, ld, k, i0 + cc == c ? scale : 1.0, w);
+  }
+  LB_HIP(hipGetLastError());
   return LB_OK;
 }
 

@@ -36,11 +36,10 @@ void print_row(int verbose, int iter, double v) {
   if (verbose) std::printf("             %3d       %24.16E\n", iter, v);
 }
 
-// Restarted GMRES(m) on device vectors of length n.  V holds m+1 vectors.
-// With zero_x0 the solve starts from x0 = 0 (sol is cleared), so the first
-// residual is rhs itself and costs no operator application.
+// Restarted GMRES(m) on device vectors of length n from the iterate in sol.
+// V holds m+1 vectors.
 int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* sol, double* V, int m, double rtol,
-          double atol, int maxit, int verbose, GmresResult* res, bool zero_x0 = false) {
+          double atol, int maxit, int verbose, GmresResult* res) {
   hipStream_t s = op->stream;
   std::vector<double> H((size_t)(m + 1) * (m + 2) / 2 + 2, 0.0), c(m + 2), sn(m + 2), rs(m + 3);
   auto hcol = [&](int k) { return &H[(size_t)k * (k - 1) / 2]; };
@@ -61,19 +60,12 @@ int gmres(lb_op* op, int64_t n, const Apply& apply, const double* rhs, double* s
   }
   ret(0.0);
   int status = 0;
-  if (zero_x0) LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
-  bool initial = zero_x0;
   while (true) {
     double* v0 = vec(0);
-    if (initial) {
-      LB_TRY(launch_copy(s, n, rhs, v0));   // r0 = rhs - A 0
-      initial = false;
-    } else {
-      LB_TRY(launch_copy(s, n, sol, v0));
-      LB_TRY(apply(v0));
-      ++nmv;
-      LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
-    }
+    LB_TRY(launch_copy(s, n, sol, v0));
+    LB_TRY(apply(v0));
+    ++nmv;
+    LB_TRY(launch_axpby(s, n, 1.0, rhs, -1.0, v0, v0));
     double a2;
     LB_TRY(dot(op, n, v0, v0, &a2));
     double alpha = std::sqrt(a2);
@@ -237,6 +229,191 @@ void givens(double x, double y, double* c, double* s) {
   }
 }
 
+// Orthogonalises w against the orthonormal vectors V_0 .. V_{c-1} (stride n) by
+// classical Gram-Schmidt, repeated once when the first pass removes more than
+// half of ||w||^2 (the criterion of Daniel, Gragg, Kaufman & Stewart, Math.
+// Comp. 30 (1976) 772-795), and scales w to unit length.  h[i] receives the
+// projection on V_i, *nu the norm of the orthogonalised w.  When the second pass
+// again removes more than half, w lies in span V to working precision: *nu = 0
+// and w is not scaled.  A non-finite dot gives *nu = NaN.  The norm is
+// ||w - V h||^2 = ||w||^2 - ||h||^2, accurate to a few ulps whenever at most
+// half of ||w||^2 is removed, so one pass is one read of V and w for the dots
+// (one host synchronisation) and one read-write of w for the update and scaling.
+int orthonormalize(lb_op* op, int64_t n, const double* V, int c, double* w, double* h, double* nu) {
+  std::vector<double> d(c + 1);
+  for (int pass = 0; pass < 2; ++pass) {
+    LB_TRY(multi_dot(op, n, V, n, c, w, true, d.data()));
+    double hh = 0.0;
+    for (int i = 0; i < c; ++i) hh += d[i] * d[i];
+    const double ww = d[c];
+    if (!std::isfinite(ww) || !std::isfinite(hh)) {
+      *nu = std::nan("");
+      return LB_OK;
+    }
+    for (int i = 0; i < c; ++i) h[i] = pass == 0 ? d[i] : h[i] + d[i];
+    const double rem = ww - hh;
+    if (rem > 0.5 * ww) {
+      *nu = std::sqrt(rem);
+      return multi_axpy(op, n, V, n, c, d.data(), 1.0 / *nu, w);
+    }
+    if (pass == 0) LB_TRY(multi_axpy(op, n, V, n, c, d.data(), 1.0, w));
+  }
+  *nu = 0.0;
+  return LB_OK;
+}
+
+using ApplyK = std::function<int(const double* in, double* out)>;
+
+// Restarted GMRES(m) for (I - K) x = rhs from x0 = 0, with K applied out of
+// place.  I - K and K have the same Krylov spaces, so the Arnoldi process runs
+// on w = K v_j, whose orthogonalisation does not cancel the identity part:
+// if K v_j = sum_{i<=j} h_ij v_i + nu v_{j+1}, then
+// (I - K) v_j = sum_i (delta_ij - h_ij) v_i - nu v_{j+1}, the Hessenberg matrix
+// of I - K.  Orthogonalisation by orthonormalize() above.  Iterate counting,
+// the stopping rule ||r|| <= rtol ||rhs|| + atol, Givens rotations and the
+// explicit residual at every restart as gmres() above.  When the basis stops
+// growing (nu = 0), the solution is updated and the explicit residual decides:
+// converged, another cycle, or LB_E_BREAKDOWN if a cycle made no progress.
+int gmres_shifted(lb_op* op, int64_t n, const ApplyK& applyK, const double* rhs, double* sol, double* V, int m,
+                  double rtol, double atol, int maxit, int verbose, GmresResult* res) {
+  hipStream_t s = op->stream;
+  auto vec = [&](int j) { return V + (int64_t)j * n; };
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hk(m + 1);
+  auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
+  int iter = 0, nmv = 0, status = 0;
+  auto ret = [&](double v) {
+    ++iter;
+    print_row(verbose, iter, v);
+    return !std::isnan(v);
+  };
+  if (verbose) {
+    std::printf("          -------------------------------------\n");
+    std::printf("           ITERATE           RESIDUAL NORM     \n");
+    std::printf("          ---------     -----------------------\n");
+  }
+  ret(0.0);
+  LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
+  double r2 = 0.0;
+  LB_TRY(dot(op, n, rhs, rhs, &r2));
+  const double eps = rtol * std::sqrt(r2) + atol;
+  double resn = 0.0, last_beta = -1.0;
+  bool initial = true;
+  while (true) {
+    double* v0 = vec(0);
+    if (initial) {
+      LB_TRY(launch_copy(s, n, rhs, v0));   // r0 = rhs - (I - K) 0
+      initial = false;
+    } else {                                // r = rhs - x + K x
+      LB_TRY(applyK(sol, v0));
+      ++nmv;
+      LB_TRY(launch_axpby(s, n, 1.0, rhs, 1.0, v0, v0));
+      LB_TRY(launch_axpby(s, n, -1.0, sol, 1.0, v0, v0));
+    }
+    double b2;
+    LB_TRY(dot(op, n, v0, v0, &b2));
+    const double beta = std::sqrt(b2);
+    resn = beta;
+    if (std::isnan(beta)) {
+      ret(beta);
+      status = LB_E_NAN;
+      break;
+    }
+    if (beta <= eps) {
+      ret(beta);
+      break;
+    }
+    if (last_beta >= 0.0 && !(beta < last_beta)) {   // a cycle that ended early made no progress
+      ret(beta);
+      status = LB_E_BREAKDOWN;
+      break;
+    }
+    if (maxit > 0 && nmv >= maxit) {
+      ret(beta);
+      status = LB_E_MAXIT;
+      break;
+    }
+    LB_TRY(launch_scale(s, n, 1.0 / beta, v0));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    bool invariant = false;
+    while (k < m) {
+      if (!ret(resn)) {
+        status = LB_E_NAN;
+        break;
+      }
+      LB_TRY(applyK(vec(k), vec(k + 1)));
+      ++nmv;
+      double nu;
+      LB_TRY(orthonormalize(op, n, V, k + 1, vec(k + 1), hk.data(), &nu));
+      if (std::isnan(nu)) {
+        status = LB_E_NAN;
+        break;
+      }
+      for (int i = 0; i <= k; ++i) h(i, k) = (i == k ? 1.0 : 0.0) - hk[i];
+      h(k + 1, k) = -nu;
+      for (int i = 0; i < k; ++i) {
+        const double a = h(i, k), b = h(i + 1, k);
+        h(i, k) = c[i] * a + sn[i] * b;
+        h(i + 1, k) = c[i] * b - sn[i] * a;
+      }
+      givens(h(k, k), h(k + 1, k), &c[k], &sn[k]);
+      h(k, k) = c[k] * h(k, k) + sn[k] * h(k + 1, k);
+      h(k + 1, k) = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = c[k] * g[k];
+      resn = std::fabs(g[k + 1]);
+      ++k;
+      if (std::isnan(resn)) {
+        status = LB_E_NAN;
+        break;
+      }
+      if (nu == 0.0) {
+        invariant = true;
+        break;
+      }
+      if (resn <= eps || (maxit > 0 && nmv >= maxit)) break;
+    }
+    if (status != 0) break;
+    // y = R^-1 g over the k columns, x += V y (one fused pass per 8 vectors)
+    std::vector<double> y(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double a = g[i];
+      for (int j = i + 1; j < k; ++j) a -= h(i, j) * y[j];
+      y[i] = h(i, i) != 0.0 ? a / h(i, i) : 0.0;
+    }
+    for (double& t : y) t = -t;
+    LB_TRY(multi_axpy(op, n, V, n, k, y.data(), 1.0, sol));
+    if (invariant) {
+      last_beta = beta;                // the explicit residual decides
+      continue;
+    }
+    last_beta = -1.0;
+    if (resn <= eps) {
+      ret(resn);
+      break;
+    }
+    if (maxit > 0 && nmv >= maxit) {
+      ret(resn);
+      status = LB_E_MAXIT;
+      break;
+    }
+    if (!ret(resn)) {
+      status = LB_E_NAN;
+      break;
+    }
+  }
+  res->status = status;
+  res->iterates = iter;
+  res->matvecs = nmv;
+  res->final_norm = resn;
+  if (verbose && status == 0) {
+    std::printf("\n          CONVERGED.\n          Total number of iterates : %3d\n", iter);
+    std::printf("          Final Residual Norm      : %24.16E\n\n", resn);
+  }
+  return LB_OK;
+}
+
 using Precond = std::function<int(const double*, double*)>;
 using Snapshot = std::function<int(const double* sol, const double* Z, int k, const std::vector<double>& y)>;
 
@@ -252,8 +429,8 @@ struct FgmresResult {
 // GMRESR / flexible GCR choice of van der Vorst & Vuik, 1994): z_k = P(r_k).
 // r_k is carried along at the cost of one update per step,
 // r_k = s_k^2 r_{k-1} + c_k rho_k v_k (rho_k the rotated right-hand side), and
-// with P = I the basis is the Krylov space of plain GMRES.  Modified Gram-Schmidt
-// with mgsro's selective re-orthogonalisation (0.98 test), Givens rotations,
+// with P = I the basis is the Krylov space of plain GMRES.  Orthogonalisation by
+// orthonormalize() above (classical Gram-Schmidt, once more when needed), Givens rotations,
 // stopping rule ||r|| <= rtol ||rhs|| + atol and an explicit residual at every
 // restart, as gmres() above.  Fallback, z_k = v_k (plain GMRES) for the rest of
 // the solve: when any 3 consecutive preconditioned steps reduce the residual
@@ -265,7 +442,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
   hipStream_t s = op->stream;
   auto vec = [&](int j) { return V + (int64_t)j * n; };
   auto zvec = [&](int j) { return Z + (int64_t)j * n; };
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hist;
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hk(m + 1), hist;
   auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
   constexpr int kWindow = 3;
   constexpr double kDrop = 0.125;   // 0.5 per step over the window
@@ -285,6 +462,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
     }
   };
   bool done = false;
+  double hn_last = 1.0;
   if (zero_x0) LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
   bool initial = zero_x0;   // x0 = 0: the first residual is rhs, no operator application
   while (!done) {
@@ -333,27 +511,16 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
       LB_TRY(launch_copy(s, n, zk, w));
       LB_TRY(apply(w));
       ++nmv;
-      double nrm0 = 0.0;
-      LB_TRY(dot(op, n, w, w, &nrm0));
-      double thr = nrm0 * 0.98;
-      for (int i = 0; i <= k; ++i) {
-        double f = 0.0;
-        LB_TRY(dot(op, n, w, vec(i), &f));
-        h(i, k) = f;
-        LB_TRY(launch_axpby(s, n, -f, vec(i), 1.0, w, w));
-        if (f * f > thr) {
-          LB_TRY(dot(op, n, w, vec(i), &f));
-          h(i, k) += f;
-          LB_TRY(launch_axpby(s, n, -f, vec(i), 1.0, w, w));
-        }
-        nrm0 = std::max(0.0, nrm0 - h(i, k) * h(i, k));
-        thr = nrm0 * 0.98;
+      double hn = 0.0;
+      LB_TRY(orthonormalize(op, n, V, k + 1, w, hk.data(), &hn));
+      hn_last = hn;
+      if (std::isnan(hn)) {
+        status = LB_E_NAN;
+        done = true;
+        break;
       }
-      double w2 = 0.0;
-      LB_TRY(dot(op, n, w, w, &w2));
-      const double hn = std::sqrt(w2);
+      for (int i = 0; i <= k; ++i) h(i, k) = hk[i];
       h(k + 1, k) = hn;
-      if (hn > 0.0) LB_TRY(launch_scale(s, n, 1.0 / hn, w));
       for (int i = 0; i < k; ++i) {
         const double a = h(i, k), b = h(i + 1, k);
         h(i, k) = c[i] * a + sn[i] * b;
@@ -392,7 +559,15 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
     if (status == LB_E_NAN) break;
     std::vector<double> y;
     solve_y(k, y);
-    for (int i = 0; i < k; ++i) LB_TRY(launch_axpby(s, n, y[i], zvec(i), 1.0, sol, sol));
+    for (double& t : y) t = -t;
+    LB_TRY(multi_axpy(op, n, Z, n, k, y.data(), 1.0, sol));   // x += Z y
+    // A Z = V H holds for any z_k, so |g_k| is the residual norm of the update,
+    // as in plain GMRES: a cycle that reached eps ends the solve without an
+    // explicit residual (a basis that stopped growing still gets one).
+    if (resn <= eps && hn_last > 0.0) {
+      status = 0;
+      done = true;
+    }
   }
   res->status = status;
   res->iterates = iters;
@@ -404,16 +579,23 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
 
 }  // namespace
 
+// out <- D T^-1 M S in: the scattering-sweep-moments operator K of group g0 on
+// flux moments, out of place.
+int moment_k_dev(lb_op* op, const double* in, double* out, int g0) {
+  const int64_t n = op->NENK * op->NI;
+  LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+  double* psi_t = op->w_a2.as<double>();
+  LB_TRY(launch_moments_to_discrete(op, in, psi_t, g0, 1, 0));
+  LB_TRY(sweep_inplace(op, psi_t, g0));
+  return launch_moments(op, psi_t, out, 1, op->NENK);
+}
+
 // v <- v - D T^-1 M S v: the within-group operator on flux moments of group g0.
 int moment_matvec_dev(lb_op* op, double* v, int g0) {
-  const int64_t n = op->NENK * op->NI, nphi = op->NENK * op->nmom;
-  LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+  const int64_t nphi = op->NENK * op->nmom;
   LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
-  double* psi_t = op->w_a2.as<double>();
   double* tphi = op->w_phi.as<double>();
-  LB_TRY(launch_moments_to_discrete(op, v, psi_t, g0, 1, 0));
-  LB_TRY(sweep_inplace(op, psi_t, g0));
-  LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+  LB_TRY(moment_k_dev(op, v, tphi, g0));
   return launch_axpby(op->stream, nphi, 1.0, v, -1.0, tphi, v);
 }
 
@@ -621,20 +803,14 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
     double* V = op->w_krylov.as<double>();
     double* phi = V + (int64_t)(m + 1) * nphi;
     double* rhs = phi + nphi;
-    double* psi_t = op->w_a2.as<double>();
     LB_TRY(launch_moments(op, s, rhs, 1, op->NENK));
     GmresResult r;
-    Apply A = [&](double* v) -> int {
+    ApplyK K = [&](const double* in, double* out) -> int {
       ++sweeps;
-      LB_TRY(launch_moments_to_discrete(op, v, psi_t, g0, 1, 0));
-      LB_TRY(sweep_inplace(op, psi_t, g0));
-      LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
-      double* tphi = op->w_phi.as<double>();
-      LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
-      return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
+      return moment_k_dev(op, in, out, g0);
     };
     // x0 = 0 (the psi-space solver keeps the reference's q / sigma_t(1))
-    LB_TRY(gmres(op, nphi, A, rhs, phi, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r, true));
+    LB_TRY(gmres_shifted(op, nphi, K, rhs, phi, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r));
     // psi = T^-1 (M S phi + q)
     LB_TRY(launch_copy(str, n, q, s));
     LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
@@ -825,7 +1001,6 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
   LB_TRY(op->w_krylov.reserve((size_t)(flexible ? 2 * m + 3 : m + 2) * nphi * sizeof(double)));
   double* q = op->w_mg.as<double>();
   double* acc = op->w_acc.as<double>();
-  double* psi_t = op->w_a2.as<double>();
   double* V = op->w_krylov.as<double>();
   double* rhs = V + (int64_t)(m + 1) * nphi;
   double* Z = rhs + nphi;
@@ -845,12 +1020,11 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
     int sweeps = 1;
     Apply A = [&](double* v) -> int {
       ++sweeps;
-      LB_TRY(launch_moments_to_discrete(op, v, psi_t, g, 1, 0));
-      LB_TRY(sweep_inplace(op, psi_t, g));
-      LB_TRY(op->w_phi.reserve((size_t)nphi * sizeof(double)));
-      double* tphi = op->w_phi.as<double>();
-      LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
-      return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
+      return moment_matvec_dev(op, v, g);
+    };
+    ApplyK K = [&](const double* in, double* out) -> int {
+      ++sweeps;
+      return moment_k_dev(op, in, out, g);
     };
     double* phi_g = phi_out + (int64_t)g * nphi;
     if (flexible) {
@@ -865,7 +1039,7 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
       st[g].fallback = r.fallback;
     } else {
       GmresResult r;
-      LB_TRY(gmres(op, nphi, A, rhs, phi_g, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r, true));
+      LB_TRY(gmres_shifted(op, nphi, K, rhs, phi_g, V, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, &r));
       st[g].status = r.status;
       st[g].iterates = r.iterates;
       st[g].matvecs = r.matvecs;
