@@ -258,6 +258,23 @@ int lb_op_get_schedule(lb_op* op, int32_t* sl, int32_t* nlev);
  * be checked where no GPU is present. */
 int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t* sl, int32_t* nlev, double* normals);
 
+/* The LDS slot plan of the direction-resident LDS sweep (LB_SWEEP_DIRECTION_LDS)
+ * for G groups per cell and lds_bytes of LDS, built on the host for the
+ * schedule lb_schedule_host returns; no device calls.  No counterpart in the
+ * reference, which has no LDS: it lets the plan be checked where no GPU is
+ * present.  Outputs (NULL skips one):
+ *   uslot     [NI][NE][4]  LDS slot of each upstream trace, -1 = global memory
+ *                          (also -1 on faces that are not upstream)
+ *   wslot     [NI][NE]     slot each cell writes, -1 = none
+ *   stage_off [NI][maxlev + 2]  first staging entry of each consumer level
+ *   stage     2 per entry: (LDS slot, element copied from global memory), filled
+ *             only if *nstage holds at least the entry count; *nstage returns it
+ *   sizes     {ring slots, staging slots per level parity}
+ *   stats     upstream traces from the ring, staged, one level back, overflow */
+int lb_trace_plan_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int G, int lds_bytes, int32_t* uslot,
+                       int32_t* wslot, int32_t* stage_off, int32_t* stage, int64_t* nstage, int32_t* sizes,
+                       int64_t* stats);
+
 /* Outward unit normals n[e][f][3] as PostProcess.f90:48-102 computes them. */
 int lb_op_get_normals(lb_op* op, double* n);
 

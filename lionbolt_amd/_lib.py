@@ -83,6 +83,8 @@ SIGNATURES = {
     "lb_fluence": (ctypes.c_int, [_vp, _d, _d]),
     "lb_op_get_schedule": (ctypes.c_int, [_vp, _i32, _i32]),
     "lb_schedule_host": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), _i32, _i32, _d]),
+    "lb_trace_plan_host": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), ctypes.c_int,
+                                          ctypes.c_int, _i32, _i32, _i32, _i32, _i64, _i32, _i64]),
     "lb_op_get_normals": (ctypes.c_int, [_vp, _d]),
     "lb_op_info": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64]),
     "lb_transport_matinv_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),

@@ -188,6 +188,48 @@ int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t
   return LB_OK;
 }
 
+int lb_trace_plan_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int G, int lds_bytes, int32_t* uslot,
+                       int32_t* wslot, int32_t* stage_off, int32_t* stage, int64_t* nstage, int32_t* sizes,
+                       int64_t* stats) {
+  if (!mesh || !quad || !mesh->rg || !mesh->node || !mesh->nbr || !mesh->mat || !quad->k || !quad->w ||
+      mesh->NE <= 0 || mesh->NE > LB_MAX_ELEMENTS || quad->NI <= 0 || G < 1 || lds_bytes <= 0 ||
+      (mesh->index_base != 0 && mesh->index_base != 1)) {
+    set_error("lb_trace_plan_host: invalid argument");
+    return LB_E_ARG;
+  }
+  lb_op op;
+  op.host_only = true;
+  op.NE = mesh->NE;
+  op.NK = mesh->NK;
+  op.NI = quad->NI;
+  op.NENK = 4 * mesh->NE;
+  op.L = quad->L;
+  op.h_k.assign(quad->k, quad->k + 3 * quad->NI);
+  op.h_w.assign(quad->w, quad->w + quad->NI);
+  LB_TRY(compute_geometry(&op, mesh->rg, mesh->node, mesh->nbr, mesh->mat, mesh->index_base));
+  int ring = 0, st = 0;
+  LB_TRY(trace_plan_sizes(lds_bytes, G, &ring, &st));
+  const int64_t NI = op.NI, NE = op.NE, stride = op.maxlev + 2;
+  std::vector<int32_t> us(NI * NE * 4), ws(NI * NE), so(NI * stride), ent;
+  int64_t sts[4];
+  LB_TRY(build_trace_plan(NI, NE, op.maxlev, op.h_nlev.data(), op.h_lvl_off.data(), op.h_tasks.data(),
+                          op.h_nbr.data(), ring, st, us.data(), ws.data(), so.data(), ent, sts));
+  if (uslot) std::memcpy(uslot, us.data(), us.size() * sizeof(int32_t));
+  if (wslot) std::memcpy(wslot, ws.data(), ws.size() * sizeof(int32_t));
+  if (stage_off) std::memcpy(stage_off, so.data(), so.size() * sizeof(int32_t));
+  const int64_t n = (int64_t)ent.size() / 2;
+  if (nstage) {
+    if (stage && *nstage >= n) std::memcpy(stage, ent.data(), ent.size() * sizeof(int32_t));
+    *nstage = n;
+  }
+  if (sizes) {
+    sizes[0] = ring;
+    sizes[1] = st;
+  }
+  if (stats) std::memcpy(stats, sts, sizeof(sts));
+  return LB_OK;
+}
+
 int lb_op_set_physics(lb_op* op, int Nm, const double* sigma_t, int L, const double* sigma_s) {
   return lb_op_set_physics_groups(op, 1, Nm, sigma_t, L, sigma_s);
 }
@@ -553,7 +595,9 @@ void lb_op_destroy(lb_op* op) {
   op->batch.h_io = nullptr;
   op->d_lvl_off.release();
   op->d_nlev.release();
-  op->d_upos.release();
+  for (auto& kv : op->plans)
+    for (DevBuf* b : {&kv.second.d_uslot, &kv.second.d_wslot, &kv.second.d_stage_off, &kv.second.d_stage})
+      b->release();
   op->d_xsdd.release();
   op->w_phis.release();
   op->w_tq.release();

@@ -83,6 +83,17 @@ struct SweepTune {
   int choice = -1;                       // index into the config table once settled
 };
 
+// LDS slot plan of the direction-resident LDS sweep for one group-batch size
+// (built by lb_trace_plan.cpp on the device sweep order).
+struct TracePlan {
+  int ring = 0, stage = 0;   // ring slots; staging slots per level parity
+  int64_t stats[4] = {0, 0, 0, 0};   // upstream traces read from the ring, staged, one level back, overflow
+  DevBuf d_uslot;            // int32 [NI][NE][4]: LDS slot of each upstream trace, -1 = global memory
+  DevBuf d_wslot;            // int32 [NI][NE]: slot the cell writes, -1 = none
+  DevBuf d_stage_off;        // int32 [NI][maxlev + 2]: first staging entry of each consumer level
+  DevBuf d_stage;            // int2 entries: (LDS slot, element whose flux is copied)
+};
+
 // Most energy groups solved together by the group-batched multigroup solve.
 constexpr int kMaxGroupBatch = 16;
 
@@ -153,8 +164,7 @@ struct lb_op {
   lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
   lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
-  lb::DevBuf d_upos;                // int32 [NI][NE][4]: sweep position of the upstream neighbour across
-                                    // each face in that direction's order, -1 where the face is not upstream
+  std::map<int, lb::TracePlan> plans;  // LDS sweep slot plans per group-batch size (built on first use)
   int lds_max = 0;                  // LDS bytes a workgroup may allocate (queried once)
   void* blas = nullptr;             // rocBLAS handle of the FP64 moment GEMMs (created on first use)
   lb::DevBuf w_phis;                // sigma-scaled flux moments, input of the M GEMM
@@ -191,6 +201,17 @@ int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int
                      int base);
 int compute_schedule(lb_op* op, const int64_t* nbr_c);
 int build_angular(lb_op* op);
+
+// lb_trace_plan.cpp
+// Ring and staging slot counts of the LDS sweep for G groups per cell.
+int trace_plan_sizes(int lds_bytes, int G, int* ring, int* stage);
+// Plans every direction of a level-ordered task array (tasks [NI][NE], level
+// offsets [NI][maxlev + 1]); uslot [NI][NE][4], wslot [NI][NE], stage_off
+// [NI][maxlev + 2] (entry indices into entries, 2 ints per entry); stats: upstream
+// traces from the ring, staged, one level back, staging overflow.
+int build_trace_plan(int64_t NI, int64_t NE, int maxlev, const int32_t* nlev, const int32_t* lvl_off,
+                     const uint32_t* tasks, const int32_t* nbr, int ring, int stage, int32_t* uslot, int32_t* wslot,
+                     int32_t* stage_off, std::vector<int32_t>& entries, int64_t stats[4]);
 LevelTable* level_table(lb_op* op, int G, int* rc);
 
 // lb_kernels.hip (launchers; all enqueue on op->stream)

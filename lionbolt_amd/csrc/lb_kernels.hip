@@ -47,18 +47,20 @@ struct SweepArgs {
   int64_t NE;
   const double* q;   // may alias psi (in-place sweep)
   double* psi;
-  const int4* __restrict__ upos = nullptr;   // LDS sweep: upstream neighbour positions per face
+  // LDS sweep only: the slot plan of this group-batch size (lb_trace_plan.cpp)
+  const int4* __restrict__ uslot = nullptr;      // [dir][pos]: LDS slot of each upstream trace, -1 global
+  const int32_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
+  const int32_t* __restrict__ stage_off = nullptr;   // [dir][maxlev + 2]: staging entries per consumer level
+  const int2* __restrict__ stage = nullptr;      // (LDS slot, element) copied from global memory
 };
 
-// Trace ring of the direction-resident LDS sweep: the flux of the direction's
-// last `mask + 1` sweep positions, laid out [slot][node][group], slot = position
-// & mask.  While level [s0, s1) is solved it writes the slots of positions
-// [s0 - slots, s1 - slots), so an upstream position p (< s0) is still held iff
-// p >= s1 - slots; older ones are read from global memory.
+// LDS of the direction-resident LDS sweep: trace slots laid out [slot][node][group].
+// Which cell writes which slot, and where each upstream trace is read, is fixed
+// by the slot plan (lb_trace_plan.cpp): ring slots held from the producer's level
+// until its last consumer, and staging slots filled from global memory one level
+// ahead.
 struct TraceRing {
   double* buf;
-  int mask;
-  int lim;
 };
 
 // With many groups per cell the records and sources are read once per sweep and
@@ -101,7 +103,7 @@ struct CellPrep {
   double b[4];
   int4 nb;
   uint32_t pm;
-  int s;      // LDS sweep only: sweep position and upstream positions
+  int ws;     // LDS sweep only: slot this cell writes and slots of its upstream traces
   int4 us;
 };
 
@@ -110,8 +112,8 @@ __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int 
   const int64_t k = (int64_t)i * A.NE + s;
   const TaskRec r = load_rec<NT>(A.rec + k);
   if constexpr (LDS) {
-    P.s = s;
-    P.us = A.upos[k];
+    P.ws = A.wslot[k];
+    P.us = A.uslot[k];
   }
   const double4 c = load_ck<NT>(A.ck + k);
   P.w = r.w;
@@ -152,7 +154,7 @@ __device__ __forceinline__ void load_upstream(const SweepArgs& A, int i, const C
       if (up >> f & 1u) {
         const unsigned n0 = (pm >> (6 * f)) & 3u, n1 = (pm >> (6 * f + 2)) & 3u, n2 = (pm >> (6 * f + 4)) & 3u;
         if constexpr (SRC == kTraceRing) {
-          const double* rb = ring.buf + (int64_t)(usv[f] & ring.mask) * 4 * A.G + g;
+          const double* rb = ring.buf + (int64_t)usv[f] * 4 * A.G + g;
           u[f][0] = rb[n0 * A.G];
           u[f][1] = rb[n1 * A.G];
           u[f][2] = rb[n2 * A.G];
@@ -276,9 +278,11 @@ __device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, c
 #pragma unroll
   for (int a = 0; a < 4; ++a) A.psi[base + a * A.G] = x[a];
   if constexpr (SRC != kTraceGlobal) {
-    double* rb = ring.buf + (int64_t)(P.s & ring.mask) * 4 * A.G + g;
+    if (P.ws >= 0) {
+      double* rb = ring.buf + (int64_t)P.ws * 4 * A.G + g;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) rb[a * A.G] = x[a];
+      for (int a = 0; a < 4; ++a) rb[a * A.G] = x[a];
+    }
   }
 }
 
@@ -290,23 +294,22 @@ __device__ __forceinline__ void finish_cell_body(const SweepArgs& A, int i, cons
   solve_with_upstream<SRC>(A, i, P, g, ring, u);
 }
 
-__device__ __forceinline__ bool ring_hit(const CellPrep& P, const TraceRing& ring) {
+__device__ __forceinline__ bool ring_hit(const CellPrep& P) {
   const unsigned up = (P.w >> 24) & 0xFu;
-  return (!(up & 1u) || P.us.x >= ring.lim) && (!(up & 2u) || P.us.y >= ring.lim) &&
-         (!(up & 4u) || P.us.z >= ring.lim) && (!(up & 8u) || P.us.w >= ring.lim);
+  return (!(up & 1u) || P.us.x >= 0) && (!(up & 2u) || P.us.y >= 0) && (!(up & 4u) || P.us.z >= 0) &&
+         (!(up & 8u) || P.us.w >= 0);
 }
 
-// With the trace ring, a cell whose upstream traces are all still held reads
-// them from LDS only; any other reads them all from global memory.  The two
-// paths are separate to the end of the cell, so a wave whose cells all hit
-// never waits on the vector-memory counter, which would also wait for the
-// look-ahead loads issued before it.  The device sweep order puts the hitting
-// cells of each level first, so most waves take one path.
+// In the LDS sweep, a cell whose upstream traces all sit in LDS slots reads them
+// from LDS only; any other reads them all from global memory.  The two paths are
+// separate to the end of the cell, so a wave whose cells all hit never waits on
+// the vector-memory counter, which would also wait for the look-ahead loads
+// issued before it.
 template <bool LDS = false>
 __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const CellPrep& P, int g,
-                                            const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
+                                            const TraceRing& ring = TraceRing{nullptr}) {
   if constexpr (LDS) {
-    if (ring_hit(P, ring))
+    if (ring_hit(P))
       finish_cell_body<kTraceRing>(A, i, P, g, ring);
     else
       finish_cell_body<kTraceGlobalRing>(A, i, P, g, ring);
@@ -317,7 +320,7 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
 
 template <bool NT, bool LDS = false>
 __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g,
-                                           const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
+                                           const TraceRing& ring = TraceRing{nullptr}) {
   CellPrep P;
   prep_cell<NT, LDS>(A, i, s, g, P);
   finish_cell<LDS>(A, i, P, g, ring);
@@ -377,7 +380,7 @@ __device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, in
 template <int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                              bool lane_ok, const CellPrep (&P)[R],
-                                             const TraceRing& ring = TraceRing{nullptr, 0, 0}) {
+                                             const TraceRing& ring = TraceRing{nullptr}) {
   if (!lane_ok) return;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -387,31 +390,68 @@ __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, 
   for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
 }
 
+// Staging copy of the LDS sweep: the traces that the cells of level l read from
+// staging slots (plan entries so[l] .. so[l+1]) are loaded from global memory
+// while level l - 1 is solved and written to LDS before the barrier that ends
+// it.  Their producers lie two or more levels back, so the barrier before level
+// l - 1 has ordered their global stores.  One double per lane: the plan keeps
+// entries x 4 G within the smallest workgroup.
+struct StageCopy {
+  double v;
+  int dst;   // LDS double index, -1 when the lane copies nothing
+};
+
+template <bool LDS>
+__device__ __forceinline__ StageCopy stage_load(const SweepArgs& A, int i, const int32_t* __restrict__ so, int l) {
+  StageCopy c{0.0, -1};
+  if constexpr (LDS) {
+    const int e0 = so[l], per = 4 * A.G;
+    const int t = threadIdx.x;
+    if (t < (so[l + 1] - e0) * per) {
+      const int j = t / per, r = t - j * per;
+      const int2 en = A.stage[e0 + j];
+      c.dst = en.x * per + r;
+      c.v = A.psi[((int64_t)i * A.NE + en.y) * per + r];
+    }
+  }
+  return c;
+}
+
+template <bool LDS>
+__device__ __forceinline__ void stage_store(const TraceRing& ring, const StageCopy& c) {
+  if constexpr (LDS) {
+    if (c.dst >= 0) ring.buf[c.dst] = c.v;
+  }
+}
+
 // Walks the levels of one direction (positions off[0..nl]) with the whole
 // workgroup: BLOCK / G cells per round, a workgroup barrier between levels.  The
 // static data of level l+1 (records, c_f k.n, cross section, source) is loaded
 // at the start of level l into the second of two register buffers, so its
 // latency overlaps level l's upstream loads and solves; only the upstream-trace
-// loads of a level wait for the barrier.
+// loads of a level wait for the barrier.  so: the direction's staging offsets
+// (LDS sweep only).
 template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
-                                            TraceRing ring = TraceRing{nullptr, 0, 0}) {
+                                            TraceRing ring = TraceRing{nullptr},
+                                            const int32_t* __restrict__ so = nullptr) {
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
-  const int slots = ring.mask + 1;
   CellPrep Pa[R], Pb[R];
   if (nl > 0) prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
   for (int l = 0; l < nl; l += 2) {
     if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
-    ring.lim = off[l + 1] - slots;
+    const StageCopy sa = stage_load<LDS>(A, i, so, l + 1);
     finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
+    stage_store<LDS>(ring, sa);
     __syncthreads();
     if (l + 1 >= nl) break;
     if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
-    ring.lim = off[l + 2] - slots;
+    const StageCopy sb = stage_load<LDS>(A, i, so, l + 2);
     finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
+    stage_store<LDS>(ring, sb);
     __syncthreads();
   }
 }
@@ -422,17 +462,20 @@ __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int
 // the look-ahead, which then completes while the cells are solved.  The
 // look-ahead issues the same loads on every path (lanes past the next level's
 // end re-read its last cell), which keeps the compiler's counter waits exact.
+// The staging copy for the next level (LDS sweep) is issued after the
+// look-ahead and written to LDS after the solves.
 template <int R, bool NT, bool LDS>
 __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, int s0, int s1, int n0, int n1,
                                                      int cpr, int c, int g, bool lane_ok, const CellPrep (&P)[R],
-                                                     CellPrep (&Pn)[R], const TraceRing& ring) {
+                                                     CellPrep (&Pn)[R], const TraceRing& ring,
+                                                     const int32_t* __restrict__ so, int lnext) {
   constexpr int kSolve = LDS ? kTraceGlobalRing : kTraceGlobal;
   double u[R][4][3];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
     if (lane_ok && s < s1) {
-      if (LDS && ring_hit(P[r], ring))
+      if (LDS && ring_hit(P[r]))
         load_upstream<kTraceRing>(A, i, P[r], g, ring, u[r]);
       else
         load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
@@ -440,6 +483,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
+  const StageCopy sc = stage_load<LDS>(A, i, so, lnext);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
@@ -447,31 +491,30 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
   }
   if (lane_ok)
     for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
+  stage_store<LDS>(ring, sc);
 }
 
 template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
-                                                           int nl, TraceRing ring = TraceRing{nullptr, 0, 0}) {
+                                                           int nl, TraceRing ring = TraceRing{nullptr},
+                                                           const int32_t* __restrict__ so = nullptr) {
   const int cpr = BLOCK / A.G;
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
-  const int slots = ring.mask + 1;
   CellPrep Pa[R], Pb[R];
   if (nl <= 0) return;
 #pragma unroll
   for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(off[0] + c + r * cpr, off[1] - 1), g, Pa[r]);
   for (int l = 0; l < nl; l += 2) {
     const int l1 = min(l + 1, nl - 1);
-    ring.lim = off[l + 1] - slots;
     level_upstream_first<R, NT, LDS>(A, i, off[l], off[l + 1], off[l1], off[l1 + 1], cpr, c, g, lane_ok, Pa, Pb,
-                                     ring);
+                                     ring, so, l + 1);
     __syncthreads();
     if (l + 1 >= nl) break;
     const int l2 = min(l + 2, nl - 1);
-    ring.lim = off[l + 2] - slots;
     level_upstream_first<R, NT, LDS>(A, i, off[l + 1], off[l + 2], off[l2], off[l2 + 1], cpr, c, g, lane_ok, Pb, Pa,
-                                     ring);
+                                     ring, so, l + 2);
     __syncthreads();
   }
 }
@@ -506,50 +549,24 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
                                                                      nlev[i]);
 }
 
-// Direction-resident sweep with the trace ring in LDS (dynamic shared memory of
-// (mask + 1) * 4 * G doubles): upstream traces of the direction's most recent
-// positions come from LDS, older ones from global memory.  The flux is written
-// to both; the workgroup barrier between levels orders the LDS writes and reads.
+// Direction-resident sweep with LDS trace slots (dynamic shared memory of
+// (ring + 2 stage) * 4 * G doubles, the slot plan of this group-batch size):
+// upstream traces come from LDS where the plan holds them, otherwise from global
+// memory.  The flux is written to both; the workgroup barrier between levels
+// orders the LDS writes and reads.
 template <int BLOCK, bool NT>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
                                                                     const int32_t* __restrict__ nlev, int maxlev,
-                                                                    SweepArgs A, int mask) {
+                                                                    SweepArgs A) {
   extern __shared__ double ring_buf[];
   const int i = blockIdx.x;
+  const int32_t* so = A.stage_off + (int64_t)i * (maxlev + 2);
   if constexpr (NT)
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
-                                                            TraceRing{ring_buf, mask, 0});
+                                                            TraceRing{ring_buf}, so);
   else
     walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true>(
-        A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf, mask, 0});
-}
-
-// pos[i][e] = sweep position of element e in direction i.
-__global__ __launch_bounds__(kBlock) void sweep_positions_kernel(int NI, int64_t NE,
-                                                                 const uint32_t* __restrict__ tasks,
-                                                                 int32_t* __restrict__ pos) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= (int64_t)NI * NE) return;
-  const int64_t i = k / NE;
-  pos[i * NE + (tasks[k] & 0xFFFFFFu)] = int32_t(k - i * NE);
-}
-
-// upos[i][s][f] = sweep position (direction i) of the upstream neighbour across
-// face f of the cell at position s, -1 where f is not upstream.
-__global__ __launch_bounds__(kBlock) void upstream_positions_kernel(int NI, int64_t NE,
-                                                                    const uint32_t* __restrict__ tasks,
-                                                                    const int4* __restrict__ nbr,
-                                                                    const int32_t* __restrict__ pos,
-                                                                    int4* __restrict__ upos) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= (int64_t)NI * NE) return;
-  const int64_t i = k / NE;
-  const uint32_t w = tasks[k];
-  const unsigned up = (w >> 24) & 0xFu;
-  const int4 n = nbr[w & 0xFFFFFFu];
-  const int32_t* p = pos + i * NE;
-  upos[k] = make_int4((up & 1u) ? p[n.x] : -1, (up & 2u) ? p[n.y] : -1, (up & 4u) ? p[n.z] : -1,
-                      (up & 8u) ? p[n.w] : -1);
+        A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf}, so);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
@@ -907,69 +924,78 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
 
 template <bool NT>
 static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
-                                   int maxlev, const SweepArgs& A, int mask, size_t bytes) {
+                                   int maxlev, const SweepArgs& A, size_t bytes) {
   const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
                    : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
                                   : (const void*)&sweep_direction_lds_kernel<1024, NT>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A,
-                       mask);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A);
   else if (block == 512)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A,
-                       mask);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A);
   else
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A,
-                       mask);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A);
   return LB_OK;
 }
 
-// Ring slots for a group batch: the largest power of two whose [slot][4][G]
-// doubles fit the LDS a workgroup may allocate.
-static int trace_ring_size(lb_op* op, int G, int* mask, size_t* bytes) {
+// Slot plan of the LDS sweep for G groups per cell, built on first use from the
+// device sweep order (lb_trace_plan.cpp) and kept per group-batch size.  Every
+// slot index is checked against the LDS the kernel allocates before upload.
+static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
+  auto it = op->plans.find(G);
+  if (it != op->plans.end()) return &it->second;
+  *rc = LB_OK;
   if (op->lds_max == 0) {
     int dev = 0, v = 0;
-    LB_HIP(hipGetDevice(&dev));
-    LB_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                                                 hipSuccess) {
+      set_error("sweep: cannot query the LDS size");
+      *rc = LB_E_HIP;
+      return nullptr;
+    }
     op->lds_max = v;
   }
-  const size_t per_slot = (size_t)4 * G * sizeof(double);
-  if (per_slot > (size_t)op->lds_max) {
-    set_error("sweep: the LDS trace ring cannot hold one position of this group batch");
-    return LB_E_ARG;
+  TracePlan p;
+  if ((*rc = trace_plan_sizes(op->lds_max, G, &p.ring, &p.stage)) != LB_OK) return nullptr;
+  const int64_t NI = op->NI, NE = op->NE, stride = op->maxlev + 2;
+  std::vector<uint32_t> tasks(NI * NE);
+  if (hipMemcpy(tasks.data(), op->d_tasks.p, tasks.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("sweep: cannot read the device sweep order");
+    *rc = LB_E_HIP;
+    return nullptr;
   }
-  size_t slots = 1;
-  while (slots * 2 * per_slot <= (size_t)op->lds_max) slots *= 2;
-  *mask = (int)slots - 1;
-  *bytes = slots * per_slot;
-  return LB_OK;
-}
-
-// Upstream neighbour positions of every (direction, sweep position), for the
-// LDS sweep (built once; a temporary inverse permutation [NI][NE]).
-static int ensure_upstream_positions(lb_op* op) {
-  if (op->d_upos.p) return LB_OK;
-  const int64_t n = op->NI * op->NE;
-  struct Scratch {
-    DevBuf b;
-    ~Scratch() { b.release(); }   // also on the error returns below
-  } pos;
-  LB_TRY(pos.b.reserve(n * sizeof(int32_t)));
-  int rc = op->d_upos.reserve(n * sizeof(int4));
-  if (rc != LB_OK) return rc;
-  hipLaunchKernelGGL(sweep_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_tasks.as<const uint32_t>(), pos.b.as<int32_t>());
-  hipLaunchKernelGGL(upstream_positions_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_tasks.as<const uint32_t>(), op->d_nbr.as<const int4>(), pos.b.as<const int32_t>(),
-                     op->d_upos.as<int4>());
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(op->stream);   // the scratch is freed on return
-  if (e != hipSuccess) {
-    op->d_upos.release();
-    set_error(std::string("upstream positions: ") + hipGetErrorString(e));
-    return LB_E_HIP;
+  std::vector<int32_t> uslot(NI * NE * 4), wslot(NI * NE), soff(NI * stride), ent;
+  build_trace_plan(NI, NE, op->maxlev, op->h_nlev.data(), op->h_lvl_off.data(), tasks.data(), op->h_nbr.data(),
+                   p.ring, p.stage, uslot.data(), wslot.data(), soff.data(), ent, p.stats);
+  const int slots = p.ring + 2 * p.stage;
+  bool ok = true;
+  for (int32_t v : uslot) ok &= v >= -1 && v < slots;
+  for (int32_t v : wslot) ok &= v >= -1 && v < p.ring;
+  for (size_t k = 0; k < ent.size(); k += 2) ok &= ent[k] >= p.ring && ent[k] < slots && ent[k + 1] >= 0 && ent[k + 1] < NE;
+  for (int64_t i = 0; i < NI; ++i)
+    for (int64_t l = 0; l + 1 < stride; ++l) {
+      const int32_t n = soff[i * stride + l + 1] - soff[i * stride + l];
+      ok &= n >= 0 && n <= p.stage;
+    }
+  if (!ok) {
+    set_error("sweep: LDS slot plan out of range");
+    *rc = LB_E_ARG;
+    return nullptr;
   }
-  return LB_OK;
+  if (ent.empty()) ent.assign(2, 0);
+  struct Guard {
+    TracePlan* t;
+    ~Guard() {
+      if (t)
+        for (DevBuf* b : {&t->d_uslot, &t->d_wslot, &t->d_stage_off, &t->d_stage}) b->release();
+    }
+  } guard{&p};
+  if ((*rc = upload(op, p.d_uslot, uslot.data(), uslot.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  if ((*rc = upload(op, p.d_wslot, wslot.data(), wslot.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  if ((*rc = upload(op, p.d_stage_off, soff.data(), soff.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  if ((*rc = upload(op, p.d_stage, ent.data(), ent.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  guard.t = nullptr;
+  return &op->plans.emplace(G, p).first->second;
 }
 
 template <bool NT>
@@ -1094,16 +1120,19 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
     if (mode == LB_SWEEP_DIRECTION_LDS) {
-      int mask = 0;
-      size_t bytes = 0;
-      LB_TRY(trace_ring_size(op, G, &mask, &bytes));
-      LB_TRY(ensure_upstream_positions(op));
-      A.upos = op->d_upos.as<const int4>();
+      int rc = LB_OK;
+      const TracePlan* plan = trace_plan(op, G, &rc);
+      if (!plan) return rc;
+      const size_t bytes = (size_t)(plan->ring + 2 * plan->stage) * 4 * G * sizeof(double);
+      A.uslot = plan->d_uslot.as<const int4>();
+      A.wslot = plan->d_wslot.as<const int32_t>();
+      A.stage_off = plan->d_stage_off.as<const int32_t>();
+      A.stage = plan->d_stage.as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
       if (nt)
-        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, mask, bytes));
+        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
       else
-        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, mask, bytes));
+        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
       LB_HIP(hipGetLastError());
       LB_HIP(hipEventRecord(op->ev1, op->stream));
       return LB_OK;

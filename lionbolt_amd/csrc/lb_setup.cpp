@@ -263,10 +263,12 @@ int compute_schedule(lb_op* op, const int64_t* nb) {
   if (op->host_only) return LB_OK;
   // The device sweeps each level in its own order: the cells whose upstream
   // neighbours all sit in the last `slots` positions before the level's end
-  // (still in the LDS trace ring of a one-group direction-resident sweep) come
-  // first, the others after them, each part in the reference's ascending
-  // element order.  Cells of a level are independent, so every order gives the
-  // same flux; h_tasks keeps the reference order for the schedule queries.
+  // come first, the others after them, each part in the reference's ascending
+  // element order.  Recent producers are the ones the LDS sweep's slot plan
+  // keeps in LDS, so this groups most of the cells that read a trace from
+  // global memory at the end of their level, and most waves take one path.
+  // Cells of a level are independent, so every order gives the same flux;
+  // h_tasks keeps the reference order for the schedule queries.
   int dev = 0, lds = 0;
   LB_HIP(hipGetDevice(&dev));
   LB_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));

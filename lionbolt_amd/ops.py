@@ -298,6 +298,34 @@ def schedule_host(mesh, ang):
     return sl, nlev, nrm
 
 
+def trace_plan_host(mesh, ang, G, lds_bytes):
+    """The LDS slot plan of the direction-resident LDS sweep on the schedule of
+    ``schedule_host`` (no GPU needed): a dict with uslot [NI, NE, 4], wslot [NI, NE],
+    stage_off [NI, maxlev + 2], stage [n, 2] (LDS slot, element), ring, stage_slots
+    and stats (ring, staged, one level back, overflow)."""
+    keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat - 1, ang.k, ang.w)]
+    rg, node, nbr, mat, k, w = keep
+    md = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr), _lib.i64ptr(mat), 0)
+    qd = _lib.QuadDesc(ang.NI, _lib.dptr(k), _lib.dptr(w), ang.L)
+    _, nlev, _ = schedule_host(mesh, ang)
+    maxlev = int(nlev.max())
+    uslot = np.empty((ang.NI, mesh.NE, 4), dtype=np.int32)
+    wslot = np.empty((ang.NI, mesh.NE), dtype=np.int32)
+    soff = np.empty((ang.NI, maxlev + 2), dtype=np.int32)
+    sizes = np.empty(2, dtype=np.int32)
+    stats = np.empty(4, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    L = _lib.lib()
+    _lib.check(L.lb_trace_plan_host(ctypes.byref(md), ctypes.byref(qd), G, lds_bytes, None, None, None, None,
+                                    ctypes.byref(n), None, None))
+    stage = np.empty((max(n.value, 1), 2), dtype=np.int32)
+    _lib.check(L.lb_trace_plan_host(ctypes.byref(md), ctypes.byref(qd), G, lds_bytes, _lib.i32ptr(uslot),
+                                    _lib.i32ptr(wslot), _lib.i32ptr(soff), _lib.i32ptr(stage), ctypes.byref(n),
+                                    _lib.i32ptr(sizes), stats.ctypes.data_as(_lib._i64)))
+    return dict(uslot=uslot, wslot=wslot, stage_off=soff, stage=stage[:n.value], ring=int(sizes[0]),
+                stage_slots=int(sizes[1]), stats=stats, nlev=nlev)
+
+
 class SpaceAngleVector:
     """``SpaceAngleVector(mesh, angular)``: flat FP64 ``v`` of length NENK*NI."""
 

@@ -4,8 +4,9 @@ properties that do not need the oracle, which is too slow at this size:
 * the sweep is linear: T^-1 (q1 + 2 q2) = T^-1 q1 + 2 T^-1 q2 to 1e-12
   relative (FP64 rounding only);
 * every launch mode gives the same flux bit for bit: level launches, the
-  direction-resident kernel and the LDS trace-ring kernel (whose ring covers
-  4,096 of each direction's 384,000 positions, so both of its trace paths run);
+  direction-resident kernel and the LDS kernel (its slot plan at this size
+  reads 98.4 % of the upstream traces from ring slots, 0.7 % from staging
+  copies and 1 % from global memory, so every trace path runs);
 * a three-group sweep equals three single-group sweeps bit for bit;
 * a two-group downscatter solve with the beam source reaches a true relative
   residual below its tolerance in every group.

@@ -2,7 +2,8 @@
 per global level (streamed or replayed from a captured graph) and the
 direction-resident kernels in which a workgroup walks all levels of one
 direction with workgroup barriers, reading upstream traces from global memory
-or from the LDS ring of the direction's recent positions."""
+or from the LDS slots of the slot plan (ring slots, staging copies, and global
+reads where neither holds the trace)."""
 import numpy as np
 import pytest
 
@@ -35,13 +36,18 @@ def test_direction_resident_equals_level_launches(G, block):
 
 
 @pytest.mark.parametrize("G", [1, 3])
-def test_lds_ring_with_global_fallback(G):
-    """A direction longer than the LDS ring (16,464 tets): upstream traces of
-    positions that left the ring come from global memory."""
+def test_lds_slots_with_staging_and_global_reads(G):
+    """82,944 tets, 8 directions.  With one group every trace fits the ring; with
+    three the ring holds 1,664 cells, so the plan also evicts, stages copies from
+    global memory and leaves some traces in global memory (checked on the host
+    plan, whose counts do not depend on the order of cells inside a level)."""
     import torch
-    from lionbolt_amd.ops import Operator
-    m = M.jittered_cube(14, a=6.0, seed=1013, slab=(2.0, 3.5))
+    from lionbolt_amd.ops import Operator, trace_plan_host
+    m = M.jittered_cube(24, a=6.0, seed=1013, slab=(2.0, 3.5))
     ang = angular.product(2, 4, 0)
+    stats = trace_plan_host(m, ang, G, 160 * 1024)["stats"]
+    if G == 3:
+        assert stats[1] > 0 and stats[2] > 0 and stats[3] > 0, stats
     op = Operator(m, ang)
     sigt = np.stack([np.linspace(0.3, 0.9, G), np.linspace(0.8, 2.5, G)], axis=1)
     op.set_physics_groups(sigt, np.zeros((G, 2, 1)))
