@@ -128,14 +128,14 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
  * (source, flux, group range) into a HIP graph and replays it;
  * LB_SWEEP_DIRECTION runs one kernel in which each workgroup owns one
  * direction and walks its levels with workgroup barriers;
- * LB_SWEEP_DIRECTION_LDS does the same and also keeps the flux of the
- * direction's most recent sweep positions in LDS, reading upstream traces from
- * there when they are still held (global memory otherwise); LB_SWEEP_AUTO
- * (default) times the graph mode, the direction-resident mode at 256, 512 and
- * 1024 threads and the LDS one at 256 and 512 threads on the first sweeps of
- * each group-batch size (two sweeps
- * each, synchronising after each) and then keeps the fastest.  All modes give
- * bit-identical results.  No reference counterpart (the reference sweeps one
+ * LB_SWEEP_DIRECTION_LDS does the same and also keeps upstream traces in LDS
+ * slots fixed by a slot plan built on first use per group-batch size (ring
+ * slots by Belady's rule, staging copies one level ahead, global memory for
+ * the rest; see lb_trace_plan_host); LB_SWEEP_AUTO (default) times the graph
+ * mode, the direction-resident mode at 256, 512 and 1024 threads and, below 8
+ * groups per cell, the LDS one at 256 and 512 threads on the first sweeps of
+ * each group-batch size (two sweeps each, synchronising after each) and then
+ * keeps the fastest.  All modes give bit-identical results.  No reference counterpart (the reference sweeps one
  * direction per OpenMP thread). */
 int lb_op_set_sweep_mode(lb_op* op, int mode);
 

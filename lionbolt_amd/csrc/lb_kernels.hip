@@ -405,6 +405,7 @@ template <bool LDS>
 __device__ __forceinline__ StageCopy stage_load(const SweepArgs& A, int i, const int32_t* __restrict__ so, int l) {
   StageCopy c{0.0, -1};
   if constexpr (LDS) {
+    if (l < 0) return c;   // not the last chunk of a level
     const int e0 = so[l], per = 4 * A.G;
     const int t = threadIdx.x;
     if (t < (so[l + 1] - e0) * per) {
@@ -424,13 +425,39 @@ __device__ __forceinline__ void stage_store(const TraceRing& ring, const StageCo
   }
 }
 
+// One chunk of the plain walk: cells [s0, s1) of level l, at most R * cpr of
+// them, with the next chunk (of the same level or the first of the next one)
+// loaded into the second register buffer before they are solved.  Only the last
+// chunk of a level is followed by the barrier and issues the staging copy for
+// the next level (LDS sweep).  Advances (l, s0, s1); false after the last chunk.
+template <int R, bool NT, bool LDS>
+__device__ __forceinline__ bool chunk_plain(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
+                                            int chunk, int cpr, int c, int g, bool lane_ok, int& l, int& s0, int& s1,
+                                            const CellPrep (&P)[R], CellPrep (&Pn)[R], const TraceRing& ring,
+                                            const int32_t* __restrict__ so) {
+  const bool last = s1 == off[l + 1];
+  const int ln = last ? l + 1 : l;
+  const bool more = ln < nl;
+  const int n0 = !more ? s0 : (last ? off[ln] : s1);
+  const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
+  if (more) prep_level<R, NT, LDS>(A, i, n0, n1, cpr, c, g, lane_ok, Pn);
+  const StageCopy sc = stage_load<LDS>(A, i, so, last ? l + 1 : -1);
+  finish_level<R, NT, LDS>(A, i, s0, s1, cpr, c, g, lane_ok, P, ring);
+  stage_store<LDS>(ring, sc);
+  if (last) __syncthreads();
+  l = ln;
+  s0 = n0;
+  s1 = n1;
+  return more;
+}
+
 // Walks the levels of one direction (positions off[0..nl]) with the whole
-// workgroup: BLOCK / G cells per round, a workgroup barrier between levels.  The
-// static data of level l+1 (records, c_f k.n, cross section, source) is loaded
-// at the start of level l into the second of two register buffers, so its
-// latency overlaps level l's upstream loads and solves; only the upstream-trace
-// loads of a level wait for the barrier.  so: the direction's staging offsets
-// (LDS sweep only).
+// workgroup: BLOCK / G cells per round, R rounds per chunk (with few groups per
+// cell), a workgroup barrier between levels.  The static data of the next chunk (records, c_f k.n, cross
+// section, source) is loaded into the second of two register buffers before
+// the current chunk is solved, so its latency overlaps the current chunk's
+// upstream loads and solves; only the upstream-trace loads of a level wait for
+// the barrier.  so: the direction's staging offsets (LDS sweep only).
 template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
                                             TraceRing ring = TraceRing{nullptr},
@@ -440,19 +467,33 @@ __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
   CellPrep Pa[R], Pb[R];
-  if (nl > 0) prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
-  for (int l = 0; l < nl; l += 2) {
-    if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
-    const StageCopy sa = stage_load<LDS>(A, i, so, l + 1);
-    finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
-    stage_store<LDS>(ring, sa);
-    __syncthreads();
-    if (l + 1 >= nl) break;
-    if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
-    const StageCopy sb = stage_load<LDS>(A, i, so, l + 2);
-    finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
-    stage_store<LDS>(ring, sb);
-    __syncthreads();
+  if (nl <= 0) return;
+  if constexpr (NT) {
+    // Many groups per cell: one step per level, its first R rounds fetched ahead
+    // and the rest solved in place (measured: 24 groups at 1024 threads 56.4 ms
+    // per sweep, against 80.7 ms in chunks of R rounds, which spill more).
+    prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
+    for (int l = 0; l < nl; l += 2) {
+      if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+      const StageCopy sa = stage_load<LDS>(A, i, so, l + 1);
+      finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
+      stage_store<LDS>(ring, sa);
+      __syncthreads();
+      if (l + 1 >= nl) break;
+      if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
+      const StageCopy sb = stage_load<LDS>(A, i, so, l + 2);
+      finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
+      stage_store<LDS>(ring, sb);
+      __syncthreads();
+    }
+  } else {
+    const int chunk = R * cpr;
+    int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
+    prep_level<R, NT, LDS>(A, i, s0, s1, cpr, c, g, lane_ok, Pa);
+    while (true) {
+      if (!chunk_plain<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so)) break;
+      if (!chunk_plain<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so)) break;
+    }
   }
 }
 
@@ -494,6 +535,33 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
   stage_store<LDS>(ring, sc);
 }
 
+// One chunk of the upstream-first walk: cells [s0, s1) of level l, at most
+// R * cpr of them.  Its look-ahead is the next chunk, of the same level or the
+// first of the next level; only the last chunk of a level is followed by the
+// barrier (cells of a level are independent) and issues the staging copy for
+// the next level.  Advances (l, s0, s1) and returns false after the last chunk.
+template <int R, bool NT, bool LDS>
+__device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
+                                                     int nl, int chunk, int cpr, int c, int g, bool lane_ok,
+                                                     int& l, int& s0, int& s1, const CellPrep (&P)[R],
+                                                     CellPrep (&Pn)[R], const TraceRing& ring,
+                                                     const int32_t* __restrict__ so) {
+  const bool last = s1 == off[l + 1];
+  const int ln = last ? l + 1 : l;
+  const bool more = ln < nl;
+  const int n0 = !more ? s0 : (last ? off[ln] : s1);
+  const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
+  level_upstream_first<R, NT, LDS>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so, last ? l + 1 : -1);
+  if (last) __syncthreads();
+  l = ln;
+  s0 = n0;
+  s1 = n1;
+  return more;
+}
+
+// Levels larger than R * cpr cells are walked in chunks of that size, so that
+// every cell's static data is fetched one chunk ahead (the short, wide
+// directions of a mesh have levels of thousands of cells).
 template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                            int nl, TraceRing ring = TraceRing{nullptr},
@@ -502,20 +570,17 @@ __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, i
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
+  const int chunk = R * cpr;
   CellPrep Pa[R], Pb[R];
   if (nl <= 0) return;
+  int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(off[0] + c + r * cpr, off[1] - 1), g, Pa[r]);
-  for (int l = 0; l < nl; l += 2) {
-    const int l1 = min(l + 1, nl - 1);
-    level_upstream_first<R, NT, LDS>(A, i, off[l], off[l + 1], off[l1], off[l1 + 1], cpr, c, g, lane_ok, Pa, Pb,
-                                     ring, so, l + 1);
-    __syncthreads();
-    if (l + 1 >= nl) break;
-    const int l2 = min(l + 2, nl - 1);
-    level_upstream_first<R, NT, LDS>(A, i, off[l + 1], off[l + 2], off[l2], off[l2 + 1], cpr, c, g, lane_ok, Pb, Pa,
-                                     ring, so, l + 2);
-    __syncthreads();
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
+  while (true) {
+    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
+      break;
+    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
+      break;
   }
 }
 
@@ -1195,9 +1260,15 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
   if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice]);
   // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
   // sweep timed alone between the operator's events; configurations whose block
-  // cannot hold the group batch are skipped.  Results are identical in every mode.
+  // cannot hold the group batch are skipped, and so is the LDS sweep with many
+  // groups per cell, whose LDS then holds only a few hundred cells (24 groups:
+  // 200 ms against 53 ms per sweep, plus a second slot plan).  Results are
+  // identical in every mode.
+  auto skip = [&](const SweepConfig& k) {
+    return (k.block != 0 && k.block < G) || (k.mode == LB_SWEEP_DIRECTION_LDS && G >= kStreamGroups);
+  };
   int c = tn.trials % nconf;
-  while (table[c].block != 0 && table[c].block < G) {
+  while (skip(table[c])) {
     ++tn.trials;
     c = tn.trials % nconf;
   }
