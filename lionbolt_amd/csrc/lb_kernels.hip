@@ -1047,8 +1047,10 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
     return nullptr;
   }
   std::vector<int32_t> uslot(NI * NE * 4), wslot(NI * NE), soff(NI * stride), ent;
-  build_trace_plan(NI, NE, op->maxlev, op->h_nlev.data(), op->h_lvl_off.data(), tasks.data(), op->h_nbr.data(),
-                   p.ring, p.stage, uslot.data(), wslot.data(), soff.data(), ent, p.stats);
+  if ((*rc = build_trace_plan(NI, NE, op->maxlev, op->h_nlev.data(), op->h_lvl_off.data(), tasks.data(),
+                              op->h_nbr.data(), p.ring, p.stage, uslot.data(), wslot.data(), soff.data(), ent,
+                              p.stats)) != LB_OK)
+    return nullptr;
   const int slots = p.ring + 2 * p.stage;
   bool ok = true;
   for (int32_t v : uslot) ok &= v >= -1 && v < slots;

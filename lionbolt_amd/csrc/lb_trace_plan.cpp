@@ -249,8 +249,15 @@ int build_trace_plan(int64_t NI, int64_t NE, int maxlev, const int32_t* nlev, co
       for (int l = nlev[i] + 1; l < stride; ++l) so[l] = so[nlev[i]];
     }
   }
-  // entries of all directions in one array: offsets become global
+  // entries of all directions in one array: offsets become global (int32)
+  int64_t total = 0;
+  for (int64_t i = 0; i < NI; ++i) total += (int64_t)ent[i].size() / 2;
+  if (total > INT_MAX) {
+    set_error("sweep: LDS slot plan has more staging entries than a 32-bit offset holds");
+    return LB_E_ARG;
+  }
   entries.clear();
+  entries.reserve(total * 2);
   for (int64_t i = 0; i < NI; ++i) {
     const int32_t base = (int32_t)(entries.size() / 2);
     for (int64_t l = 0; l < stride; ++l) stage_off[i * stride + l] += base;
