@@ -3,6 +3,7 @@
 #include <hdf5.h>
 
 #include <algorithm>
+#include <exception>
 #include <cctype>
 #include <cstdint>
 #include <string>
@@ -17,6 +18,21 @@ struct lbh5 {
 namespace {
 
 thread_local std::string g_err;
+
+// Entry points run their bodies through guarded(): a C++ exception (a failed
+// host allocation, say) becomes a status code instead of crossing the C ABI.
+template <class F>
+int guarded(F&& f) noexcept {
+  try {
+    return f();
+  } catch (...) {
+    try {
+      g_err = "unexpected C++ exception (host memory allocation failed?)";
+    } catch (...) {
+    }
+    return -1;
+  }
+}
 
 int fail(const std::string& msg) {
   g_err = msg;
@@ -89,125 +105,139 @@ extern "C" {
 const char* lbh5_last_error(void) { return g_err.c_str(); }
 
 int lbh5_create(const char* path, lbh5** out) {
-  if (!path || !out) return fail("lbh5_create: null argument");
-  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);   // errors come back as return codes
-  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
-  if (f < 0) return fail(std::string("lbh5_create: cannot create ") + path);
-  *out = new lbh5{f};
-  return 0;
+  return guarded([&]() -> int {
+    if (!path || !out) return fail("lbh5_create: null argument");
+    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);   // errors come back as return codes
+    hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) return fail(std::string("lbh5_create: cannot create ") + path);
+    *out = new lbh5{f};
+    return 0;
+  });
 }
 
 int lbh5_write_problem_type(lbh5* h, const char* problem_type) {
-  if (!h || !problem_type) return fail("lbh5_write_problem_type: null argument");
-  return attr_str(h->file, "problem_type", problem_type);
+  return guarded([&]() -> int {
+    if (!h || !problem_type) return fail("lbh5_write_problem_type: null argument");
+    return attr_str(h->file, "problem_type", problem_type);
+  });
 }
 
 int lbh5_write_mesh(lbh5* h, int64_t NE, int64_t NK, const int64_t* node, const double* rg, const int64_t* mat,
                     const double* vol, int slab) {
-  if (!h || !node || !rg || !mat || !vol || NE <= 0 || NK <= 0) return fail("lbh5_write_mesh: invalid argument");
-  const int64_t NENK = 4 * NE;
-  if (!fits_i32(NENK) || !fits_i32(NK)) return fail("lbh5_write_mesh: sizes exceed 32-bit integers");
-  int32_t Nm = 0;
-  for (int64_t e = 0; e < NE; ++e) {
-    if (mat[e] < 1) return fail("lbh5_write_mesh: materials are 1-based");
-    Nm = std::max<int32_t>(Nm, (int32_t)mat[e]);
-  }
-  hid_t g = open_group(h->file, "mesh");
-  if (g < 0) return fail("lbh5_write_mesh: group mesh");
-  int rc = 0;
-  rc = rc ? rc : attr_str(g, "problem_type", slab ? "slab" : "general");
-  rc = rc ? rc : attr_int(g, "number_of_materials", Nm);
-  rc = rc ? rc : attr_int(g, "number_of_elements", (int32_t)NE);
-  rc = rc ? rc : attr_int(g, "number_of_spatial_dofs", (int32_t)NENK);
-  rc = rc ? rc : attr_int(g, "number_of_global_nodes", (int32_t)NK);
-  std::vector<int32_t> conn(NENK), off(NE + 1);
-  for (int64_t e = 0; e < NE; ++e) {
-    off[e] = (int32_t)(4 * e);
-    for (int k = 0; k < 4; ++k) conn[4 * e + k] = (int32_t)(node[4 * e + k] + 1);
-  }
-  off[NE] = (int32_t)NENK;
-  rc = rc ? rc : dset_i32(g, "connectivity", NENK, conn.data());
-  rc = rc ? rc : dset_i32(g, "offset", NE + 1, off.data());
-  const hsize_t rdims[2] = {(hsize_t)NK, 3};
-  rc = rc ? rc : dset_f64(g, "global_nodes", 2, rdims, rg);
-  if (!rc) {
-    hid_t mg = open_group(g, "material_to_spatial_dof");
-    if (mg < 0) {
-      rc = fail("lbh5_write_mesh: group material_to_spatial_dof");
-    } else {
-      for (int32_t m = 1; m <= Nm && !rc; ++m) {
-        std::vector<int32_t> sd;
-        for (int64_t e = 0; e < NE; ++e)
-          if (mat[e] == m)
-            for (int k = 0; k < 4; ++k) sd.push_back((int32_t)(4 * e + k + 1));
-        rc = dset_i32(mg, "material_" + std::to_string(m), (int64_t)sd.size(), sd.data());
-      }
-      H5Gclose(mg);
+  return guarded([&]() -> int {
+    if (!h || !node || !rg || !mat || !vol || NE <= 0 || NK <= 0) return fail("lbh5_write_mesh: invalid argument");
+    const int64_t NENK = 4 * NE;
+    if (!fits_i32(NENK) || !fits_i32(NK)) return fail("lbh5_write_mesh: sizes exceed 32-bit integers");
+    int32_t Nm = 0;
+    for (int64_t e = 0; e < NE; ++e) {
+      if (mat[e] < 1) return fail("lbh5_write_mesh: materials are 1-based");
+      Nm = std::max<int32_t>(Nm, (int32_t)mat[e]);
     }
-  }
-  const hsize_t vdims[1] = {(hsize_t)NE};
-  rc = rc ? rc : dset_f64(g, "element_volumes", 1, vdims, vol);
-  H5Gclose(g);
-  return rc;
+    hid_t g = open_group(h->file, "mesh");
+    if (g < 0) return fail("lbh5_write_mesh: group mesh");
+    int rc = 0;
+    rc = rc ? rc : attr_str(g, "problem_type", slab ? "slab" : "general");
+    rc = rc ? rc : attr_int(g, "number_of_materials", Nm);
+    rc = rc ? rc : attr_int(g, "number_of_elements", (int32_t)NE);
+    rc = rc ? rc : attr_int(g, "number_of_spatial_dofs", (int32_t)NENK);
+    rc = rc ? rc : attr_int(g, "number_of_global_nodes", (int32_t)NK);
+    std::vector<int32_t> conn(NENK), off(NE + 1);
+    for (int64_t e = 0; e < NE; ++e) {
+      off[e] = (int32_t)(4 * e);
+      for (int k = 0; k < 4; ++k) conn[4 * e + k] = (int32_t)(node[4 * e + k] + 1);
+    }
+    off[NE] = (int32_t)NENK;
+    rc = rc ? rc : dset_i32(g, "connectivity", NENK, conn.data());
+    rc = rc ? rc : dset_i32(g, "offset", NE + 1, off.data());
+    const hsize_t rdims[2] = {(hsize_t)NK, 3};
+    rc = rc ? rc : dset_f64(g, "global_nodes", 2, rdims, rg);
+    if (!rc) {
+      hid_t mg = open_group(g, "material_to_spatial_dof");
+      if (mg < 0) {
+        rc = fail("lbh5_write_mesh: group material_to_spatial_dof");
+      } else {
+        for (int32_t m = 1; m <= Nm && !rc; ++m) {
+          std::vector<int32_t> sd;
+          for (int64_t e = 0; e < NE; ++e)
+            if (mat[e] == m)
+              for (int k = 0; k < 4; ++k) sd.push_back((int32_t)(4 * e + k + 1));
+          rc = dset_i32(mg, "material_" + std::to_string(m), (int64_t)sd.size(), sd.data());
+        }
+        H5Gclose(mg);
+      }
+    }
+    const hsize_t vdims[1] = {(hsize_t)NE};
+    rc = rc ? rc : dset_f64(g, "element_volumes", 1, vdims, vol);
+    H5Gclose(g);
+    return rc;
+  });
 }
 
 int lbh5_write_particle(lbh5* h, const char* particle, int64_t NI, const double* w, const double* k, int G,
                         const double* E) {
-  if (!h || !particle || !w || !k || NI <= 0 || !fits_i32(NI) || G < 0 || (G > 0 && !E))
-    return fail("lbh5_write_particle: invalid argument");
-  bool created = false;
-  hid_t p = open_group(h->file, lower(particle), &created);
-  if (p < 0) return fail("lbh5_write_particle: particle group");
-  int rc = 0;
-  if (created) rc = attr_str(p, "object_type", "particle");
-  rc = rc ? rc : attr_int(p, "number_of_angular_dofs", (int32_t)NI);
-  const hsize_t wd[1] = {(hsize_t)NI}, kd[2] = {(hsize_t)NI, 3};
-  rc = rc ? rc : dset_f64(p, "angular_weights", 1, wd, w);
-  rc = rc ? rc : dset_f64(p, "abscissae", 2, kd, k);
-  if (G > 0) {
-    const hsize_t ed[1] = {(hsize_t)(G + 1)};
-    rc = rc ? rc : attr_int(p, "number_of_energy_dofs", G);
-    rc = rc ? rc : dset_f64(p, "energy_grid", 1, ed, E);
-  }
-  H5Gclose(p);
-  return rc;
+  return guarded([&]() -> int {
+    if (!h || !particle || !w || !k || NI <= 0 || !fits_i32(NI) || G < 0 || (G > 0 && !E))
+      return fail("lbh5_write_particle: invalid argument");
+    bool created = false;
+    hid_t p = open_group(h->file, lower(particle), &created);
+    if (p < 0) return fail("lbh5_write_particle: particle group");
+    int rc = 0;
+    if (created) rc = attr_str(p, "object_type", "particle");
+    rc = rc ? rc : attr_int(p, "number_of_angular_dofs", (int32_t)NI);
+    const hsize_t wd[1] = {(hsize_t)NI}, kd[2] = {(hsize_t)NI, 3};
+    rc = rc ? rc : dset_f64(p, "angular_weights", 1, wd, w);
+    rc = rc ? rc : dset_f64(p, "abscissae", 2, kd, k);
+    if (G > 0) {
+      const hsize_t ed[1] = {(hsize_t)(G + 1)};
+      rc = rc ? rc : attr_int(p, "number_of_energy_dofs", G);
+      rc = rc ? rc : dset_f64(p, "energy_grid", 1, ed, E);
+    }
+    H5Gclose(p);
+    return rc;
+  });
 }
 
 int lbh5_append_angular_fluence(lbh5* h, const char* particle, int uncollided, int g, int64_t NI, int64_t NENK,
                                 const double* v) {
-  if (!h || !particle || !v || g < 1 || NI <= 0 || NENK <= 0)
-    return fail("lbh5_append_angular_fluence: invalid argument");
-  hid_t p = open_group(h->file, lower(particle));
-  if (p < 0) return fail("lbh5_append_angular_fluence: particle group");
-  hid_t a = open_group(p, uncollided ? "uncollided_angular_fluence" : "angular_fluence");
-  hid_t eg = a >= 0 ? open_group(a, "energy_" + std::to_string(g)) : -1;
-  int rc = eg < 0 ? fail("lbh5_append_angular_fluence: groups") : 0;
-  const hsize_t d[1] = {(hsize_t)NENK};
-  for (int64_t i = 0; i < NI && !rc; ++i)
-    rc = dset_f64(eg, "angle_" + std::to_string(i + 1), 1, d, v + i * NENK);
-  if (eg >= 0) H5Gclose(eg);
-  if (a >= 0) H5Gclose(a);
-  H5Gclose(p);
-  return rc;
+  return guarded([&]() -> int {
+    if (!h || !particle || !v || g < 1 || NI <= 0 || NENK <= 0)
+      return fail("lbh5_append_angular_fluence: invalid argument");
+    hid_t p = open_group(h->file, lower(particle));
+    if (p < 0) return fail("lbh5_append_angular_fluence: particle group");
+    hid_t a = open_group(p, uncollided ? "uncollided_angular_fluence" : "angular_fluence");
+    hid_t eg = a >= 0 ? open_group(a, "energy_" + std::to_string(g)) : -1;
+    int rc = eg < 0 ? fail("lbh5_append_angular_fluence: groups") : 0;
+    const hsize_t d[1] = {(hsize_t)NENK};
+    for (int64_t i = 0; i < NI && !rc; ++i)
+      rc = dset_f64(eg, "angle_" + std::to_string(i + 1), 1, d, v + i * NENK);
+    if (eg >= 0) H5Gclose(eg);
+    if (a >= 0) H5Gclose(a);
+    H5Gclose(p);
+    return rc;
+  });
 }
 
 int lbh5_append_fluence(lbh5* h, const char* particle, int uncollided, int g, int64_t NENK, const double* phi) {
-  if (!h || !particle || !phi || g < 1 || NENK <= 0) return fail("lbh5_append_fluence: invalid argument");
-  hid_t p = open_group(h->file, lower(particle));
-  if (p < 0) return fail("lbh5_append_fluence: particle group");
-  hid_t f = open_group(p, uncollided ? "uncollided_fluence" : "fluence");
-  const hsize_t d[1] = {(hsize_t)NENK};
-  int rc = f < 0 ? fail("lbh5_append_fluence: group") : dset_f64(f, "energy_" + std::to_string(g), 1, d, phi);
-  if (f >= 0) H5Gclose(f);
-  H5Gclose(p);
-  return rc;
+  return guarded([&]() -> int {
+    if (!h || !particle || !phi || g < 1 || NENK <= 0) return fail("lbh5_append_fluence: invalid argument");
+    hid_t p = open_group(h->file, lower(particle));
+    if (p < 0) return fail("lbh5_append_fluence: particle group");
+    hid_t f = open_group(p, uncollided ? "uncollided_fluence" : "fluence");
+    const hsize_t d[1] = {(hsize_t)NENK};
+    int rc = f < 0 ? fail("lbh5_append_fluence: group") : dset_f64(f, "energy_" + std::to_string(g), 1, d, phi);
+    if (f >= 0) H5Gclose(f);
+    H5Gclose(p);
+    return rc;
+  });
 }
 
 int lbh5_close(lbh5* h) {
-  if (!h) return 0;
-  herr_t e = h->file >= 0 ? H5Fclose(h->file) : 0;
-  delete h;
-  return e < 0 ? fail("lbh5_close: H5Fclose failed") : 0;
+  return guarded([&]() -> int {
+    if (!h) return 0;
+    herr_t e = h->file >= 0 ? H5Fclose(h->file) : 0;
+    delete h;
+    return e < 0 ? fail("lbh5_close: H5Fclose failed") : 0;
+  });
 }
 
 }  // extern "C"

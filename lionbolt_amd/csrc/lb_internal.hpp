@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -23,6 +25,30 @@ void set_error(const std::string& msg);
       return LB_E_HIP;                                                                       \
     }                                                                                        \
   } while (0)
+
+// C entry points run their bodies through guarded(): a C++ exception (a failed
+// host allocation of a std::vector, say) becomes a status code instead of
+// crossing the C ABI.
+template <class F>
+int guarded(F&& f) noexcept {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    try {
+      set_error("host memory allocation failed");
+    } catch (...) {
+    }
+    return LB_E_NOMEM;
+  } catch (const std::exception& e) {
+    try {
+      set_error(std::string("unexpected C++ exception: ") + e.what());
+    } catch (...) {
+    }
+    return LB_E_ARG;
+  } catch (...) {
+    return LB_E_ARG;
+  }
+}
 
 #define LB_TRY(call)              \
   do {                            \

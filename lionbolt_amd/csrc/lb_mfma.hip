@@ -309,62 +309,68 @@ extern "C" {
 
 int lb_pack_dev(int fmt, int rows, int64_t ncols, const double* src, void* dst, void* dst_lo, float* inv_scale,
                 void* stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (rows < 1 || ncols < 1 || !src || !dst || (fmt == FMT_SPLIT_BF16 && !dst_lo) || (fmt == FMT_FP16 && !inv_scale) ||
-      fmt < 0 || fmt > 3) {
-    set_error("lb_pack_dev: invalid argument");
-    return LB_E_ARG;
-  }
-  if (fmt == FMT_FP16)
-    hipLaunchKernelGGL(fp16_scale_kernel, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s, rows, ncols, src,
-                       inv_scale);
-  dim3 grid((unsigned)((ncols + 31) / 32), (unsigned)((rows + 31) / 32));
-  switch (fmt) {
-    case FMT_FP32: hipLaunchKernelGGL(pack_kernel<FMT_FP32>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
-    case FMT_BF16: hipLaunchKernelGGL(pack_kernel<FMT_BF16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
-    case FMT_SPLIT_BF16: hipLaunchKernelGGL(pack_kernel<FMT_SPLIT_BF16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
-    default: hipLaunchKernelGGL(pack_kernel<FMT_FP16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
-  }
-  LB_HIP(hipGetLastError());
-  return LB_OK;
+  return guarded([&]() -> int {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (rows < 1 || ncols < 1 || !src || !dst || (fmt == FMT_SPLIT_BF16 && !dst_lo) || (fmt == FMT_FP16 && !inv_scale) ||
+        fmt < 0 || fmt > 3) {
+      set_error("lb_pack_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    if (fmt == FMT_FP16)
+      hipLaunchKernelGGL(fp16_scale_kernel, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s, rows, ncols, src,
+                         inv_scale);
+    dim3 grid((unsigned)((ncols + 31) / 32), (unsigned)((rows + 31) / 32));
+    switch (fmt) {
+      case FMT_FP32: hipLaunchKernelGGL(pack_kernel<FMT_FP32>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
+      case FMT_BF16: hipLaunchKernelGGL(pack_kernel<FMT_BF16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
+      case FMT_SPLIT_BF16: hipLaunchKernelGGL(pack_kernel<FMT_SPLIT_BF16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
+      default: hipLaunchKernelGGL(pack_kernel<FMT_FP16>, grid, dim3(256), 0, s, rows, ncols, src, dst, dst_lo, inv_scale); break;
+    }
+    LB_HIP(hipGetLastError());
+    return LB_OK;
+  });
 }
 
 int lb_gemm_d_dev(int fmt, int nmom, int NI, const double* D, int64_t ncols, const void* psi, const void* psi_lo,
                   const float* inv_scale, float* phi, void* stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (NI != 256 || nmom < 1 || nmom > 16 || ncols < 1 || !D || !psi || !phi || fmt < 0 || fmt > 3 ||
-      (fmt == FMT_SPLIT_BF16 && !psi_lo) || (fmt == FMT_FP16 && !inv_scale)) {
-    set_error("lb_gemm_d_dev: needs NI = 256, nmom <= 16 and the operands of the format");
-    return LB_E_ARG;
-  }
-  const int g = grid_cols(ncols);
-  switch (fmt) {
-    case FMT_FP32: hipLaunchKernelGGL((d_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
-    case FMT_BF16: hipLaunchKernelGGL((d_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
-    case FMT_SPLIT_BF16: hipLaunchKernelGGL((d_gemm_kernel<FMT_SPLIT_BF16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
-    default: hipLaunchKernelGGL((d_gemm_kernel<FMT_FP16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
-  }
-  LB_HIP(hipGetLastError());
-  return LB_OK;
+  return guarded([&]() -> int {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (NI != 256 || nmom < 1 || nmom > 16 || ncols < 1 || !D || !psi || !phi || fmt < 0 || fmt > 3 ||
+        (fmt == FMT_SPLIT_BF16 && !psi_lo) || (fmt == FMT_FP16 && !inv_scale)) {
+      set_error("lb_gemm_d_dev: needs NI = 256, nmom <= 16 and the operands of the format");
+      return LB_E_ARG;
+    }
+    const int g = grid_cols(ncols);
+    switch (fmt) {
+      case FMT_FP32: hipLaunchKernelGGL((d_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
+      case FMT_BF16: hipLaunchKernelGGL((d_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
+      case FMT_SPLIT_BF16: hipLaunchKernelGGL((d_gemm_kernel<FMT_SPLIT_BF16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
+      default: hipLaunchKernelGGL((d_gemm_kernel<FMT_FP16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
+    }
+    LB_HIP(hipGetLastError());
+    return LB_OK;
+  });
 }
 
 int lb_gemm_m_dev(int fmt, int NI, int nmom, const double* M, int64_t ncols, const void* phi, const void* phi_lo,
                   const float* inv_scale, float* psi, void* stream) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (NI != 256 || nmom != 16 || ncols < 1 || !M || !phi || !psi || fmt < 0 || fmt > 3 ||
-      (fmt == FMT_SPLIT_BF16 && !phi_lo) || (fmt == FMT_FP16 && !inv_scale)) {
-    set_error("lb_gemm_m_dev: needs NI = 256, nmom = 16 and the operands of the format");
-    return LB_E_ARG;
-  }
-  const int g = grid_cols(ncols);
-  switch (fmt) {
-    case FMT_FP32: hipLaunchKernelGGL((m_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
-    case FMT_BF16: hipLaunchKernelGGL((m_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
-    case FMT_SPLIT_BF16: hipLaunchKernelGGL((m_gemm_kernel<FMT_SPLIT_BF16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
-    default: hipLaunchKernelGGL((m_gemm_kernel<FMT_FP16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
-  }
-  LB_HIP(hipGetLastError());
-  return LB_OK;
+  return guarded([&]() -> int {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (NI != 256 || nmom != 16 || ncols < 1 || !M || !phi || !psi || fmt < 0 || fmt > 3 ||
+        (fmt == FMT_SPLIT_BF16 && !phi_lo) || (fmt == FMT_FP16 && !inv_scale)) {
+      set_error("lb_gemm_m_dev: needs NI = 256, nmom = 16 and the operands of the format");
+      return LB_E_ARG;
+    }
+    const int g = grid_cols(ncols);
+    switch (fmt) {
+      case FMT_FP32: hipLaunchKernelGGL((m_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
+      case FMT_BF16: hipLaunchKernelGGL((m_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
+      case FMT_SPLIT_BF16: hipLaunchKernelGGL((m_gemm_kernel<FMT_SPLIT_BF16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
+      default: hipLaunchKernelGGL((m_gemm_kernel<FMT_FP16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
+    }
+    LB_HIP(hipGetLastError());
+    return LB_OK;
+  });
 }
 
 }  // extern "C"

@@ -407,177 +407,181 @@ extern "C" {
 // 256-entry buffer.
 int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* params, int nparams, int steps, int batch,
                      float lr, uint64_t seed, float* loss_hist) {
-  if (!d_X || !d_Y || !params || nparams != kParams || N < 1 || N > INT32_MAX || steps < 1 || batch < 1 ||
-      batch > 4096 || !(lr > 0.f)) {
-    set_error("lb_mlp_train_dev: invalid argument");
-    return LB_E_ARG;
-  }
-  hipStream_t st = nullptr;
-  LB_HIP(hipStreamCreate(&st));
-  const int B = batch;
-  // every buffer in one allocation: params, grads, Adam m and v, activations, indices, loss
-  const size_t nf = (size_t)4 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 256 + steps;
-  const size_t bytes = nf * sizeof(float) + (size_t)steps * B * sizeof(int32_t);
-  void* mem = nullptr;
-  hipError_t e = hipMalloc(&mem, bytes);
-  if (e != hipSuccess) {
+  return guarded([&]() -> int {
+    if (!d_X || !d_Y || !params || nparams != kParams || N < 1 || N > INT32_MAX || steps < 1 || batch < 1 ||
+        batch > 4096 || !(lr > 0.f)) {
+      set_error("lb_mlp_train_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    hipStream_t st = nullptr;
+    LB_HIP(hipStreamCreate(&st));
+    const int B = batch;
+    // every buffer in one allocation: params, grads, Adam m and v, activations, indices, loss
+    const size_t nf = (size_t)4 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 256 + steps;
+    const size_t bytes = nf * sizeof(float) + (size_t)steps * B * sizeof(int32_t);
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, bytes);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(st);
+      set_error(std::string("lb_mlp_train_dev: hipMalloc failed: ") + hipGetErrorString(e));
+      return LB_E_HIP;
+    }
+    float* w = static_cast<float*>(mem);
+    float* g = w + kParams;
+    float* am = g + kParams;
+    float* av = am + kParams;
+    float* h1 = av + kParams;
+    float* h2 = h1 + (size_t)B * kH;
+    float* h3 = h2 + (size_t)B * kH;
+    float* yh = h3 + (size_t)B * kH;
+    float* d3 = yh + (size_t)B * kOut;
+    float* d2 = d3 + (size_t)B * kH;
+    float* d1 = d2 + (size_t)B * kH;
+    float* dy = d1 + (size_t)B * kH;
+    float* part = dy + (size_t)B * kOut;
+    float* loss = part + 256;
+    int32_t* idx = reinterpret_cast<int32_t*>(loss + steps);
+    int rc = LB_OK;
+    auto fail = [&](hipError_t err, const char* what) {
+      set_error(std::string("lb_mlp_train_dev: ") + what + ": " + hipGetErrorString(err));
+      rc = LB_E_HIP;
+    };
+    // minibatch rows: SplitMix64 stream `seed`
+    std::vector<int32_t> h_idx((size_t)steps * B);
+    for (size_t j = 0; j < h_idx.size(); ++j) {
+      uint64_t zz = seed + (uint64_t)(j + 1) * 0x9E3779B97F4A7C15ull;
+      zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+      zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+      zz = zz ^ (zz >> 31);
+      h_idx[j] = (int32_t)(zz % (uint64_t)N);
+    }
+    if ((e = hipMemcpyAsync(idx, h_idx.data(), h_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(w, params, kParams * sizeof(float), hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemsetAsync(am, 0, 2 * kParams * sizeof(float), st)) != hipSuccess)
+      fail(e, "upload");
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float bc1 = 1.f, bc2 = 1.f;
+    for (int s = 0; s < steps && rc == LB_OK; ++s) {
+      const int32_t* ix = idx + (size_t)s * B;
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix,
+                         w + kW1, w + kB1, 1, h1);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
+                         w + kW2, w + kB2, 1, h2);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
+                         w + kW3, w + kB3, 1, h3);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                         w + kW4, w + kB4, 0, yh);
+      const unsigned lb = blocks((int64_t)B * kOut);
+      hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, ix, dy, part);
+      hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss + s);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                         dy, g + kW4, g + kB4);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
+                         h3, d3);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
+                         g + kW3, g + kB3);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
+                         h2, d2);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
+                         g + kW2, g + kB2);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
+                         h1, d1);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix, d1,
+                         g + kW1, g + kB1);
+      bc1 *= b1;
+      bc2 *= b2;
+      hipLaunchKernelGGL(adam_kernel, dim3(blocks(kParams)), dim3(256), 0, st, kParams, w, g, am, av, lr, b1, b2, eps,
+                         1.f - bc1, 1.f - bc2);
+      if ((e = hipGetLastError()) != hipSuccess) fail(e, "launch");
+    }
+    if (rc == LB_OK) {
+      if ((e = hipMemcpyAsync(params, w, kParams * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        fail(e, "download");
+      else if (loss_hist &&
+               (e = hipMemcpyAsync(loss_hist, loss, steps * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        fail(e, "download");
+      else if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        fail(e, "synchronize");
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(mem);
     (void)hipStreamDestroy(st);
-    set_error(std::string("lb_mlp_train_dev: hipMalloc failed: ") + hipGetErrorString(e));
-    return LB_E_HIP;
-  }
-  float* w = static_cast<float*>(mem);
-  float* g = w + kParams;
-  float* am = g + kParams;
-  float* av = am + kParams;
-  float* h1 = av + kParams;
-  float* h2 = h1 + (size_t)B * kH;
-  float* h3 = h2 + (size_t)B * kH;
-  float* yh = h3 + (size_t)B * kH;
-  float* d3 = yh + (size_t)B * kOut;
-  float* d2 = d3 + (size_t)B * kH;
-  float* d1 = d2 + (size_t)B * kH;
-  float* dy = d1 + (size_t)B * kH;
-  float* part = dy + (size_t)B * kOut;
-  float* loss = part + 256;
-  int32_t* idx = reinterpret_cast<int32_t*>(loss + steps);
-  int rc = LB_OK;
-  auto fail = [&](hipError_t err, const char* what) {
-    set_error(std::string("lb_mlp_train_dev: ") + what + ": " + hipGetErrorString(err));
-    rc = LB_E_HIP;
-  };
-  // minibatch rows: SplitMix64 stream `seed`
-  std::vector<int32_t> h_idx((size_t)steps * B);
-  for (size_t j = 0; j < h_idx.size(); ++j) {
-    uint64_t zz = seed + (uint64_t)(j + 1) * 0x9E3779B97F4A7C15ull;
-    zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
-    zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
-    zz = zz ^ (zz >> 31);
-    h_idx[j] = (int32_t)(zz % (uint64_t)N);
-  }
-  if ((e = hipMemcpyAsync(idx, h_idx.data(), h_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st)) !=
-          hipSuccess ||
-      (e = hipMemcpyAsync(w, params, kParams * sizeof(float), hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemsetAsync(am, 0, 2 * kParams * sizeof(float), st)) != hipSuccess)
-    fail(e, "upload");
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  float bc1 = 1.f, bc2 = 1.f;
-  for (int s = 0; s < steps && rc == LB_OK; ++s) {
-    const int32_t* ix = idx + (size_t)s * B;
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix,
-                       w + kW1, w + kB1, 1, h1);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
-                       w + kW2, w + kB2, 1, h2);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
-                       w + kW3, w + kB3, 1, h3);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                       w + kW4, w + kB4, 0, yh);
-    const unsigned lb = blocks((int64_t)B * kOut);
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, ix, dy, part);
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss + s);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                       dy, g + kW4, g + kB4);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
-                       h3, d3);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
-                       g + kW3, g + kB3);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
-                       h2, d2);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
-                       g + kW2, g + kB2);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
-                       h1, d1);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix, d1,
-                       g + kW1, g + kB1);
-    bc1 *= b1;
-    bc2 *= b2;
-    hipLaunchKernelGGL(adam_kernel, dim3(blocks(kParams)), dim3(256), 0, st, kParams, w, g, am, av, lr, b1, b2, eps,
-                       1.f - bc1, 1.f - bc2);
-    if ((e = hipGetLastError()) != hipSuccess) fail(e, "launch");
-  }
-  if (rc == LB_OK) {
-    if ((e = hipMemcpyAsync(params, w, kParams * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
-      fail(e, "download");
-    else if (loss_hist &&
-             (e = hipMemcpyAsync(loss_hist, loss, steps * sizeof(float), hipMemcpyDeviceToHost, st)) != hipSuccess)
-      fail(e, "download");
-    else if ((e = hipStreamSynchronize(st)) != hipSuccess)
-      fail(e, "synchronize");
-  }
-  (void)hipStreamSynchronize(st);
-  (void)hipFree(mem);
-  (void)hipStreamDestroy(st);
-  return rc;
+    return rc;
+  });
 }
 
 // One gradient of the mean squared error over rows 0..B-1 (no update), for tests.
 int lb_mlp_grad_dev(const float* d_X, const float* d_Y, int B, const float* params, int nparams, float* grad,
                     float* loss_out) {
-  if (!d_X || !d_Y || !params || !grad || nparams != kParams || B < 1 || B > 4096) {
-    set_error("lb_mlp_grad_dev: invalid argument");
-    return LB_E_ARG;
-  }
-  void* mem = nullptr;
-  const size_t nf = (size_t)2 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 257;
-  LB_HIP(hipMalloc(&mem, nf * sizeof(float) + (size_t)B * sizeof(int32_t)));
-  float* w = static_cast<float*>(mem);
-  float* g = w + kParams;
-  float* h1 = g + kParams;
-  float* h2 = h1 + (size_t)B * kH;
-  float* h3 = h2 + (size_t)B * kH;
-  float* yh = h3 + (size_t)B * kH;
-  float* d3 = yh + (size_t)B * kOut;
-  float* d2 = d3 + (size_t)B * kH;
-  float* d1 = d2 + (size_t)B * kH;
-  float* dy = d1 + (size_t)B * kH;
-  float* part = dy + (size_t)B * kOut;
-  float* loss = part + 256;
-  int32_t* idx = reinterpret_cast<int32_t*>(loss + 1);
-  std::vector<int32_t> h_idx(B);
-  for (int b = 0; b < B; ++b) h_idx[b] = b;
-  hipStream_t st = nullptr;
-  hipError_t e = hipSuccess;
-  int rc = LB_OK;
-  if ((e = hipMemcpy(idx, h_idx.data(), B * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(w, params, kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
-    set_error(std::string("lb_mlp_grad_dev: ") + hipGetErrorString(e));
-    rc = LB_E_HIP;
-  }
-  if (rc == LB_OK) {
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx,
-                       w + kW1, w + kB1, 1, h1);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
-                       w + kW2, w + kB2, 1, h2);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
-                       w + kW3, w + kB3, 1, h3);
-    hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                       w + kW4, w + kB4, 0, yh);
-    const unsigned lb = blocks((int64_t)B * kOut);
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, idx, dy, part);
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                       dy, g + kW4, g + kB4);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
-                       h3, d3);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
-                       g + kW3, g + kB3);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
-                       h2, d2);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
-                       g + kW2, g + kB2);
-    hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
-                       h1, d1);
-    hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx, d1,
-                       g + kW1, g + kB1);
-    if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
-        (e = hipMemcpy(grad, g, kParams * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess ||
-        (loss_out && (e = hipMemcpy(loss_out, loss, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)) {
+  return guarded([&]() -> int {
+    if (!d_X || !d_Y || !params || !grad || nparams != kParams || B < 1 || B > 4096) {
+      set_error("lb_mlp_grad_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    void* mem = nullptr;
+    const size_t nf = (size_t)2 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 257;
+    LB_HIP(hipMalloc(&mem, nf * sizeof(float) + (size_t)B * sizeof(int32_t)));
+    float* w = static_cast<float*>(mem);
+    float* g = w + kParams;
+    float* h1 = g + kParams;
+    float* h2 = h1 + (size_t)B * kH;
+    float* h3 = h2 + (size_t)B * kH;
+    float* yh = h3 + (size_t)B * kH;
+    float* d3 = yh + (size_t)B * kOut;
+    float* d2 = d3 + (size_t)B * kH;
+    float* d1 = d2 + (size_t)B * kH;
+    float* dy = d1 + (size_t)B * kH;
+    float* part = dy + (size_t)B * kOut;
+    float* loss = part + 256;
+    int32_t* idx = reinterpret_cast<int32_t*>(loss + 1);
+    std::vector<int32_t> h_idx(B);
+    for (int b = 0; b < B; ++b) h_idx[b] = b;
+    hipStream_t st = nullptr;
+    hipError_t e = hipSuccess;
+    int rc = LB_OK;
+    if ((e = hipMemcpy(idx, h_idx.data(), B * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(w, params, kParams * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
       set_error(std::string("lb_mlp_grad_dev: ") + hipGetErrorString(e));
       rc = LB_E_HIP;
     }
-  }
-  (void)hipDeviceSynchronize();
-  (void)hipFree(mem);
-  return rc;
+    if (rc == LB_OK) {
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx,
+                         w + kW1, w + kB1, 1, h1);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
+                         w + kW2, w + kB2, 1, h2);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
+                         w + kW3, w + kB3, 1, h3);
+      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                         w + kW4, w + kB4, 0, yh);
+      const unsigned lb = blocks((int64_t)B * kOut);
+      hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, idx, dy, part);
+      hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
+                         dy, g + kW4, g + kB4);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
+                         h3, d3);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
+                         g + kW3, g + kB3);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
+                         h2, d2);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
+                         g + kW2, g + kB2);
+      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
+                         h1, d1);
+      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx, d1,
+                         g + kW1, g + kB1);
+      if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
+          (e = hipMemcpy(grad, g, kParams * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess ||
+          (loss_out && (e = hipMemcpy(loss_out, loss, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)) {
+        set_error(std::string("lb_mlp_grad_dev: ") + hipGetErrorString(e));
+        rc = LB_E_HIP;
+      }
+    }
+    (void)hipDeviceSynchronize();
+    (void)hipFree(mem);
+    return rc;
+  });
 }
 
 }  // extern "C"
