@@ -36,7 +36,7 @@ enum {
   LB_E_STAGNANT = -2,   /* SweepOrder.f90:178-184 "Sweep is stagnant" */
   LB_E_NOT_READY = -3,  /* operator used before Build (BoltzmannOperator.f90:73-74) */
   LB_E_HIP = -4,        /* HIP runtime failure */
-  LB_E_NOMEM = -5,      /* device allocation failed */
+  LB_E_NOMEM = -5,      /* device or host allocation failed */
   LB_E_NAN = -6,        /* "RESIDUAL NORM is NaN." (SPARSKIT.f90:123-125) */
   LB_E_MAXIT = -7,      /* GMRES ipar(1) = -1 (SPARSKIT.f90:159-162) */
   LB_E_BREAKDOWN = -8   /* GMRES ipar(1) = -3 (SPARSKIT.f90:166-169) */
