@@ -7,6 +7,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -68,7 +69,7 @@ int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int
   }
   for (int64_t e = 0; e < NE; ++e) {
     int64_t m = mat[e] - base;
-    if (m < 0) {
+    if (m < 0 || m >= INT_MAX) {
       set_error("lb_op_create: material index out of range");
       return LB_E_ARG;
     }

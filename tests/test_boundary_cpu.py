@@ -114,3 +114,16 @@ def test_inverted_element_is_rejected():
     with pytest.raises(_lib.LionboltError) as ei:
         schedule_host(M.TetMesh(m.rg, node, m.mat, m.nbr), angular.snprep(1))
     assert "Jacobian" in str(ei.value)
+
+
+@pytest.mark.parametrize("field,value", [("node", "NK"), ("node", -1), ("nbr", "NE"), ("nbr", -2),
+                                         ("mat", 0), ("mat", 2 ** 40)])
+def test_index_out_of_range_is_rejected(field, value):
+    # mesh.mat is 1-based here and passed 0-based, so 0 becomes -1.
+    m = M.jittered_cube(1)
+    node, nbr, mat = m.node.copy(), m.nbr.copy(), m.mat.copy()
+    a = {"node": node, "nbr": nbr, "mat": mat}[field]
+    a.flat[3] = {"NK": m.NK, "NE": m.NE}.get(value, value)
+    with pytest.raises(_lib.LionboltError) as ei:
+        schedule_host(M.TetMesh(m.rg, node, mat, nbr), angular.snprep(1))
+    assert "out of range" in str(ei.value)
