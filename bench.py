@@ -309,6 +309,7 @@ def batch_leg(reps=10, rtol=1e-6):
     modes["auto"] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3), "choice": list(choice)}
     for name, mode, block, per_cu in (("levels", "levels", 0, 0), ("graph", "graph", 0, 0),
                                       ("persistent-1024x2", "persistent", 1024, 2),
+                                      ("persistent-256x4", "persistent", 256, 4),
                                       ("persistent-256x8", "persistent", 256, 8),
                                       ("direction-1024", "direction", 1024, 0)):
         if block:

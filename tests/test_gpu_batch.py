@@ -32,7 +32,7 @@ def batch_op(um, eoff, sigt, sigs, ang):
     return op
 
 
-@pytest.mark.parametrize("block,per_cu", [(256, 4), (512, 2), (1024, 1), (1024, 2)])
+@pytest.mark.parametrize("block,per_cu", [(256, 4), (256, 8), (512, 2), (1024, 1), (1024, 2)])
 def test_persistent_sweep_equals_level_launches(block, per_cu):
     import torch
     um, eoff, sigt, sigs, qmom, probs = small_batch()
