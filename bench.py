@@ -20,18 +20,18 @@ The line also carries:
   (group-by-group within-group GMRES on flux moments, plain, FP64);
 * reference_method_gpu: the same problem with the reference's own method
   (psi-space GMRES(10) per group), for its iteration counts;
-* cfg1_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
-* learned_pc_cfg5: the 24-group problem at within-group c = 0.99 solved with
+* cube48k_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
+* learned_pc: the 24-group problem at within-group c = 0.99 solved with
   plain flux-moment GMRES(30) and with flexible GMRES(30) preconditioned by the
   per-cell-node network (trained first on the solver's residual/error pairs
   from 32 smaller problems; training time reported separately);
-* batch_cfg4: 64 uneven single-group S8 beam problems (2k-56k tets) swept as
+* uneven_beams_batch: 64 uneven single-group S8 beam problems (2k-56k tets) swept as
   one batch (persistent work-queue kernel, autotuned) and solved with
   per-problem GMRES in lockstep;
 * high_order_operator: one sweep and the two moment products at the
   scattering orders of the reference's input decks (7, 11, 15) on an
   82,944-tet cube;
-* cpu_baseline also carries the oracle's cfg1 solve time, projected
+* cpu_baseline also carries the oracle's 48,000-tet solve time, projected
   (lower-bound) CPU times of the 24-group solve by the reference's method and
   by the GPU solve's own algorithm, and the oracle's sweep rate on the batch
   problems.
@@ -74,8 +74,8 @@ def cpu_baseline(m, ang, ndirs=32, reps=2):
 def cpu_solve_baseline(ref_method):
     """The oracle's end-to-end solves on this host.
 
-    cfg1 (48,000 tets, S8, P0 c = 0.9, GMRES(30) at 1e-8) is run to convergence.
-    The 24-group cfg2 solve is projected: the reference's own method (psi-space
+    The 48,000-tet problem (S8, P0 c = 0.9, GMRES(30) at 1e-8) is run to convergence.
+    The 24-group 384,000-tet solve is projected: the reference's own method (psi-space
     GMRES(10) with the dense direction x direction scattering kernel) at the
     iteration counts that method takes on the GPU in this run, times the oracle's
     full-size sweep and dense-K times measured here.  The projection counts only
@@ -83,16 +83,16 @@ def cpu_solve_baseline(ref_method):
     from lionbolt_amd import problems, angular
     from oracle import oracle
     out = {}
-    m, ang, sigt, sigs, q = problems.cfg1()
+    m, ang, sigt, sigs, q = problems.cube_source_48k()
     op = oracle.OracleOp(m, ang)
     op.set_physics(sigt, sigs)
     t0 = time.perf_counter()
     _, st = op.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)
-    out["cfg1_solve"] = {"seconds": time.perf_counter() - t0, "iterates": st["iters"], "status": st["status"],
+    out["cube48k_solve"] = {"seconds": time.perf_counter() - t0, "iterates": st["iters"], "status": st["status"],
                          "method": "psi-space GMRES(30), dense K, tol 1e-8"}
-    # cfg4: the oracle sweeps 4 of the 64 batch problems (all 80 directions) one
+    # uneven beams: the oracle sweeps 4 of the 64 batch problems (all 80 directions) one
     # after another; rate in cell-angle sweeps/s
-    probs = problems.cfg4()
+    probs = problems.uneven_beams()
     s8 = angular.level_symmetric_s8()
     tot_ca, tot_s = 0, 0.0
     for pr in probs[:4]:
@@ -103,11 +103,11 @@ def cpu_solve_baseline(ref_method):
         o.sweep_only(x)
         tot_s += time.perf_counter() - t0
         tot_ca += pr["mesh"].NE * s8.NI
-    out["cfg4_sweep"] = {"cell_angle_sweeps_per_s": tot_ca / tot_s,
+    out["uneven_beams_sweep"] = {"cell_angle_sweeps_per_s": tot_ca / tot_s,
                          "sample": "problems 1-4 of the 64, one sweep each, all 80 directions"}
     if ref_method:
-        m2 = problems.cfg2_mesh(10)                 # dense K timed on 6,000 tets x 256 directions, scaled by NE
-        ang2 = problems.cfg2_angular()
+        m2 = problems.slab_cube_mesh(10)                 # dense K timed on 6,000 tets x 256 directions, scaled by NE
+        ang2 = problems.product_8x32()
         op2 = oracle.OracleOp(m2, ang2)
         op2.set_physics([0.3, 0.9], [[0.2, 0.1, 0.05, 0.02], [0.6, 0.3, 0.15, 0.06]])
         x = np.random.default_rng(1003).random(op2.n)
@@ -118,7 +118,7 @@ def cpu_solve_baseline(ref_method):
         sweep_s = ref_method["sweep_s_full"]
         sweeps = ref_method["sweeps"]
         proj = sweeps * (sweep_s + k_s)
-        out["cfg2_solve_projected"] = {
+        out["solve_projected"] = {
             "seconds": proj, "sweeps_and_K": sweeps, "cpu_sweep_s": sweep_s, "cpu_dense_K_s": k_s,
             "note": "lower bound: sweeps + dense K at the reference method's measured iteration counts"}
         if ref_method.get("plain_sweeps"):
@@ -130,7 +130,7 @@ def cpu_solve_baseline(ref_method):
             op2.scatter_moment_matvec(x, D, Mm, deg)
             km_s = (time.perf_counter() - t0) * (384000 / m2.NE)
             n_pl = ref_method["plain_sweeps"]
-            out["cfg2_solve_projected_moment_form"] = {
+            out["solve_projected_moment_form"] = {
                 "seconds": n_pl * (sweep_s + km_s), "sweeps_and_K": n_pl, "cpu_sweep_s": sweep_s,
                 "cpu_moment_K_s": km_s,
                 "note": "lower bound: sweeps + moment-form K at the GPU solve leg's sweep count, so the "
@@ -178,13 +178,13 @@ def settle_autotuner(op, m, ang, G=1):
 
 
 def solve_leg(m, ang, rtol, group_batch=1):
-    """cfg2 multigroup downscatter solve with the +z beam first-collision source;
+    """The 384,000-tet multigroup downscatter solve with the +z beam first-collision source;
     ``group_batch`` consecutive groups per coupled solve (1 = group by group)."""
     import torch
     from lionbolt_amd import angular, problems
     from lionbolt_amd.ops import Operator
     G = 24
-    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_group_batch(group_batch)
@@ -217,7 +217,7 @@ def reference_method_leg(m, ang, rtol):
     from lionbolt_amd import angular, problems
     from lionbolt_amd.ops import Operator
     G = 24
-    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     settle_autotuner(op, m, ang)
@@ -251,12 +251,12 @@ def _beam_qmom(m, ang, sigt, S):
     return qmom
 
 
-def cfg1_leg():
-    """cfg1 end to end on the GPU through the host entry point (vectors cross PCIe):
+def cube48k_leg():
+    """The 48,000-tet problem end to end on the GPU through the host entry point (vectors cross PCIe):
     psi-space GMRES(30) at 1e-8 as the config states, and the moment-space solve."""
     from lionbolt_amd import problems
     from lionbolt_amd.ops import Operator
-    m, ang, sigt, sigs, q = problems.cfg1()
+    m, ang, sigt, sigs, q = problems.cube_source_48k()
     op = Operator(m, ang)
     op.set_physics(sigt, sigs)
     op.boltzmann_matinv(q, kind="gmres", space="moment", m=30, rtol=1e-8)    # warm-up (autotuner, graphs)
@@ -271,13 +271,13 @@ def cfg1_leg():
 
 
 def batch_leg(reps=10, rtol=1e-6):
-    """cfg4: 64 uneven single-group S8 beam problems as one batch.  Sweep rate of the
+    """64 uneven single-group S8 beam problems as one batch.  Sweep rate of the
     batched sweep per launch mode (the autotuned choice included), the same 64
     problems swept one operator at a time, and the batched per-problem GMRES."""
     import torch
     from lionbolt_amd import angular, problems
     from lionbolt_amd.ops import Operator
-    um, eoff, sigt, sigs, qmom, probs = problems.cfg4_batch()
+    um, eoff, sigt, sigs, qmom, probs = problems.uneven_beams_batch()
     ang = angular.level_symmetric_s8()
     ca = um.NE * ang.NI
     op = Operator(um, ang)
@@ -361,7 +361,7 @@ def batch_leg(reps=10, rtol=1e-6):
 
 
 def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
-    """cfg5: the 24-group P3 problem with within-group c = 0.99, solved group by
+    """The 24-group P3 problem with within-group c = 0.99, solved group by
     group with plain flux-moment GMRES(30) and with flexible GMRES(30) right-
     preconditioned by the per-cell-node network, trained first on the solver's own
     residual/error pairs from 32 smaller seeded problems."""
@@ -372,7 +372,7 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     params, tinfo = learned_pc.train(X, Y)
     del X, Y
     G = 24
-    sigt, S = problems.cfg5_xs(G=G, L=ang.L)
+    sigt, S = problems.scattering_dominated_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_preconditioner(params, fmt)
@@ -448,7 +448,7 @@ def single_group_leg(m, ang, reps=10):
     import torch
     from lionbolt_amd import problems
     from lionbolt_amd.ops import Operator
-    sigt, _ = problems.cfg2_xs(G=1, L=ang.L)
+    sigt, _ = problems.downscatter_xs(G=1, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((1, 2, ang.L + 1)))
     q = torch.rand(m.NE * ang.NI * 4, dtype=torch.float64, device="cuda")
@@ -491,10 +491,10 @@ def run(args):
     if args.calib:
         out_extra["copy_calibration"] = copy_calibration()
     t0 = time.perf_counter()
-    m = problems.cfg2_mesh(args.n)
-    ang = problems.cfg2_angular()
+    m = problems.slab_cube_mesh(args.n)
+    ang = problems.product_8x32()
     G = args.groups
-    sigt, _ = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, _ = problems.downscatter_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
     op.set_sweep_mode(args.sweep_mode)
@@ -606,12 +606,12 @@ def run(args):
         ref = {"sweeps": out["reference_method_gpu"]["sweeps"],
                "sweep_s_full": None,
                "plain_sweeps": out["solve"]["sweeps"] if "solve" in out else None}
-    if args.cfg1:
-        out["cfg1_gpu"] = cfg1_leg()
+    if args.cube48k:
+        out["cube48k_gpu"] = cube48k_leg()
     if args.batch:
-        out["batch_cfg4"] = batch_leg()
+        out["uneven_beams_batch"] = batch_leg()
     if args.learned_pc:
-        out["learned_pc_cfg5"] = learned_pc_leg(m, ang)
+        out["learned_pc"] = learned_pc_leg(m, ang)
     if args.gemm_study:
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
@@ -620,13 +620,13 @@ def run(args):
         if ref is not None:
             ref["sweep_s_full"] = m.NE * ang.NI / out["cpu_baseline"]["value"]
         out["cpu_baseline"].update(cpu_solve_baseline(ref))
-        for key in ("cfg2_solve_projected", "cfg2_solve_projected_moment_form"):
+        for key in ("solve_projected", "solve_projected_moment_form"):
             proj = out["cpu_baseline"].get(key)
             if proj and "solve" in out:
                 proj["gpu_solve_s"] = out["solve"]["seconds"]
                 proj["speedup_vs_gpu_solve"] = proj["seconds"] / out["solve"]["seconds"]
-        if "cfg1_gpu" in out:
-            out["cfg1_gpu"]["speedup_vs_cpu"] = out["cpu_baseline"]["cfg1_solve"]["seconds"] / out["cfg1_gpu"]["psi"]["seconds"]
+        if "cube48k_gpu" in out:
+            out["cube48k_gpu"]["speedup_vs_cpu"] = out["cpu_baseline"]["cube48k_solve"]["seconds"] / out["cube48k_gpu"]["psi"]["seconds"]
     print(json.dumps(out))
 
 
@@ -648,10 +648,10 @@ def main():
                    help="energy groups per coupled solve to try in the solve leg (e.g. 1,2,4)")
     p.add_argument("--no-ref-method", dest="ref_method", action="store_false",
                    help="skip the reference-method (psi-space GMRES(10)) solve on the GPU")
-    p.add_argument("--no-cfg1", dest="cfg1", action="store_false")
-    p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-problem batch (cfg4)")
+    p.add_argument("--no-cube48k", dest="cube48k", action="store_false")
+    p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-problem batch")
     p.add_argument("--no-learned-pc", dest="learned_pc", action="store_false",
-                   help="skip the learned-preconditioner solve (cfg5)")
+                   help="skip the learned-preconditioner solve")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction", "direction_lds"])
     p.add_argument("--block", type=int, default=1024, choices=[256, 512, 1024],

@@ -107,7 +107,7 @@ def build_training_set(nmesh=32, seed=2000, rows_per_snap=512, L=3, log=None):
     """Runs the training problems and returns (X, Y) cuda float32 tensors."""
     import torch
     from .ops import Operator
-    ang = problems.cfg2_angular() if L == 3 else angular.product(8, 32, L)
+    ang = problems.product_8x32() if L == 3 else angular.product(8, 32, L)
     Xs, Ys = [], []
     info = []
     t0 = time.perf_counter()

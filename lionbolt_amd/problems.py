@@ -2,10 +2,10 @@
 
 Decisions recorded here (the reference never ran these configurations):
 
-* cfg1: 10 cm cube, 20^3 hexes x 6 tets (48,000), LQ S8, one group, P0,
+* cube_source_48k: 10 cm cube, 20^3 hexes x 6 tets (48,000), LQ S8, one group, P0,
   sigma_t = 1.0, sigma_s = 0.9 /cm, vacuum; source q_i = 1/(4 pi) on every node
   of elements whose centroid lies in the central cube [4, 6]^3.
-* cfg2: 40^3 x 6 = 384,000 tets, Gauss-Legendre 8 polar x Chebyshev 32 azimuthal
+* slab_cube_mesh / product_8x32 / downscatter_xs: 40^3 x 6 = 384,000 tets, Gauss-Legendre 8 polar x Chebyshev 32 azimuthal
   (256 directions), P3, 24 groups, downscatter only.  sigma_t,g log-uniform on
   [0.02, 0.5] /cm (SplitMix64 stream 1002); the slab z in [4, 6] cm (elements by
   centroid) is material 2 with 3x all cross sections.  Transfer moments
@@ -18,7 +18,7 @@ Decisions recorded here (the reference never ran these configurations):
   is sum_l (2l+1)/(4 pi) sigma_l(1 -> g) P_l(k . z) phi_0, i.e. moments
   sigma_l(1 -> g) Y_lm(z) phi_0 (nodal interpolation, where the reference
   projects a ray-traced source with the inverse mass matrix, FCSSource.f90:134-215).
-* cfg4: 64 single-group problems (SplitMix64 stream 1004, six values per
+* uneven_beams: 64 single-group problems (SplitMix64 stream 1004, six values per
   problem): tets drawn uniform on [2,000, 60,000] and quantised to the jittered
   cube 6 n^3 with n = round((N/6)^(1/3)) clipped to [7, 21]; edge length
   U[5, 20] cm; sigma_t U[0.05, 1.0] /cm; c = sigma_s/sigma_t U[0.5, 0.95]; P0;
@@ -28,14 +28,14 @@ Decisions recorded here (the reference never ran these configurations):
   uncollided flux is exp(-sigma_t l) with l the distance from the node back along
   -k0 to the cube surface; the first-collision source is isotropic,
   sigma_s phi_0 / (4 pi).
-* cfg5: cfg2's mesh, directions, groups and sigma_t with a within-group
+* scattering_dominated_xs: the 384,000-tet mesh, directions, groups and sigma_t with a within-group
   scattering ratio sigma_s0(g -> g) / sigma_t,g = 0.99 in every group (the
   scattering-dominated regime the configuration asks for) and a weak downscatter
-  tail of 0.005 sigma_t,g' (shares 0.5^(g - g') as cfg2) so that the +z beam's
+  tail of 0.005 sigma_t,g' (shares 0.5^(g - g') as downscatter_xs) so that the +z beam's
   first-collision source reaches every group; sigma_l = 0.6^l sigma_0.
 * learned-preconditioner training problems: 32 jittered cubes (10 cm, n = 10..16
   hexes per side, jitter stream 2000 + p, the 3x slab) with one group each,
-  sigma_t log-uniform on [0.02, 0.5] /cm and the cfg5 ratios (SplitMix64 stream
+  sigma_t log-uniform on [0.02, 0.5] /cm and the scattering-dominated ratios (SplitMix64 stream
   2000); sources: the +z beam's first-collision source and a uniform isotropic
   source.
 """
@@ -46,7 +46,7 @@ import numpy as np
 from . import angular, mesh, rng
 
 
-def cfg1():
+def cube_source_48k():
     m = mesh.jittered_cube(20, a=10.0, jitter=0.2, seed=1001)
     ang = angular.level_symmetric_s8(0)
     c = m.centroids()
@@ -55,15 +55,15 @@ def cfg1():
     return m, ang, np.array([1.0]), np.array([[0.9]]), q
 
 
-def cfg2_mesh(n=40):
+def slab_cube_mesh(n=40):
     return mesh.jittered_cube(n, a=10.0, jitter=0.2, seed=1001, slab=(4.0, 6.0))
 
 
-def cfg2_angular():
+def product_8x32():
     return angular.product(8, 32, 3)
 
 
-def cfg2_xs(G=24, L=3, ratio=0.85, within_only=False):
+def downscatter_xs(G=24, L=3, ratio=0.85, within_only=False):
     """sigma_t (G, 2) and transfer moments S (G, G, 2, L+1) indexed [g', g, mat, l]."""
     u = rng.uniform(1002, G)
     st = np.exp(np.log(0.02) + u * (np.log(0.5) - np.log(0.02)))
@@ -125,7 +125,7 @@ def cube_beam_uncollided(m, a, k0, sigt):
     return np.exp(-sigt * ell)
 
 
-def cfg4(nprob=64, seed=1004):
+def uneven_beams(nprob=64, seed=1004):
     """The 64 uneven single-group beam problems (see the module docstring)."""
     u = rng.uniform(seed, 6 * nprob).reshape(nprob, 6)
     out = []
@@ -144,11 +144,11 @@ def cfg4(nprob=64, seed=1004):
     return out
 
 
-def cfg4_batch(nprob=64, seed=1004, probs=None):
-    """The cfg4 problems as one batch: union mesh, element offsets, per-problem
+def uneven_beams_batch(nprob=64, seed=1004, probs=None):
+    """The uneven-beam problems as one batch: union mesh, element offsets, per-problem
     (material) sigma_t and P0 sigma_s, and the first-collision source moments
     qmom (1, NENK) of the union (P0: q = Y00 sigma_s phi_0, so M qmom = sigma_s phi_0/(4 pi))."""
-    probs = cfg4(nprob, seed) if probs is None else probs
+    probs = uneven_beams(nprob, seed) if probs is None else probs
     um, eoff = mesh.union([p["mesh"] for p in probs])
     sigt = np.array([p["sigt"] for p in probs])
     sigs = np.array([[p["c"] * p["sigt"]] for p in probs])
@@ -158,9 +158,9 @@ def cfg4_batch(nprob=64, seed=1004, probs=None):
     return um, eoff, sigt, sigs, qmom, probs
 
 
-def cfg5_xs(G=24, L=3, within=0.99, down=0.005):
-    """cfg5 cross sections: sigma_t (G, 2) of cfg2 and transfer moments (G, G, 2, L+1)."""
-    sigt, _ = cfg2_xs(G=G, L=L)
+def scattering_dominated_xs(G=24, L=3, within=0.99, down=0.005):
+    """Scattering-dominated cross sections: sigma_t (G, 2) of downscatter_xs and transfer moments (G, G, 2, L+1)."""
+    sigt, _ = downscatter_xs(G=G, L=L)
     S = np.zeros((G, G, 2, L + 1))
     for gp in range(G):
         w = 0.5 ** np.arange(1, G - gp)

@@ -27,6 +27,6 @@ def small_mesh():
 
 
 @pytest.fixture(scope="session")
-def cfg1_mesh():
+def cube48k_mesh():
     from lionbolt_amd import mesh
     return mesh.jittered_cube(20, a=10.0, seed=1001)

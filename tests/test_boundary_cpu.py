@@ -73,8 +73,8 @@ def test_size_limits_are_rejected_before_any_work():
     assert L.lb_schedule_host(ctypes.byref(md), ctypes.byref(qd), None, None, None) == -1
 
 
-@pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cfg1_s8", "grazing_s8"])
-def test_host_schedule_and_normals_bit_identical_to_oracle(case, small_mesh, cfg1_mesh):
+@pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cube48k_s8", "grazing_s8"])
+def test_host_schedule_and_normals_bit_identical_to_oracle(case, small_mesh, cube48k_mesh):
     if case == "grazing_s8":
         # unjittered Kuhn cube: S8's mu = eta directions are exactly parallel to
         # the (1,-1,0)/sqrt2 faces (k.n == 0 on 6,144 interior face-directions)
@@ -84,7 +84,7 @@ def test_host_schedule_and_normals_bit_identical_to_oracle(case, small_mesh, cfg
     elif case == "small_snprep3":
         m, ang = small_mesh, angular.snprep(3)
     else:
-        m, ang = cfg1_mesh, angular.level_symmetric_s8()
+        m, ang = cube48k_mesh, angular.level_symmetric_s8()
     sl, nlev, nrm = schedule_host(m, ang)
     orc = OracleOp(m, ang)
     osl, onlev, _ = orc.schedule()

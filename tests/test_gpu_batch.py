@@ -21,7 +21,7 @@ def small_batch():
         s = math.sqrt(1 - mu * mu)
         probs.append({"n": n, "a": a, "sigt": st, "c": c, "k0": np.array([s * math.cos(fi), s * math.sin(fi), mu]),
                       "mesh": M.jittered_cube(n, a=a, seed=3000 + p)})
-    return problems.cfg4_batch(probs=probs)
+    return problems.uneven_beams_batch(probs=probs)
 
 
 def batch_op(um, eoff, sigt, sigs, ang):

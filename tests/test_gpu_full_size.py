@@ -21,8 +21,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def full():
-    m = problems.cfg2_mesh(40)
-    ang = problems.cfg2_angular()
+    m = problems.slab_cube_mesh(40)
+    ang = problems.product_8x32()
     assert m.NE == 384000 and ang.NI == 256
     return m, ang
 
@@ -31,7 +31,7 @@ def test_full_size_sweep_linearity_and_modes(full):
     import torch
     from lionbolt_amd.ops import Operator
     m, ang = full
-    sigt, _ = problems.cfg2_xs(G=1, L=ang.L)
+    sigt, _ = problems.downscatter_xs(G=1, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((1, 2, ang.L + 1)))
     n = m.NE * 4 * ang.NI
@@ -71,7 +71,7 @@ def test_full_size_group_batch_equals_single_groups(full):
     from lionbolt_amd.ops import Operator
     m, ang = full
     G = 3
-    sigt, _ = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, _ = problems.downscatter_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
     op.set_sweep_mode("levels")
@@ -95,7 +95,7 @@ def test_full_size_two_group_beam_solve(full):
     from lionbolt_amd.ops import Operator
     m, ang = full
     G = 2
-    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     # first-collision source moments of the +z beam (its uncollided flux lives in group 1)

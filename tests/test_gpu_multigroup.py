@@ -35,7 +35,7 @@ def test_multigroup_downscatter_matches_oracle():
     m = M.jittered_cube(4, a=10.0, seed=1001, slab=(4.0, 6.0))
     ang = angular.product(2, 8, 1)
     G = 4
-    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     sigt = sigt * 10.0                   # optically thicker so groups couple noticeably
     S = S * 10.0
     qmom, _ = problems.beam_source_moments(m, ang, S, sigt)
@@ -63,7 +63,7 @@ def test_group_batched_solve_matches_group_by_group(batch):
     m = M.jittered_cube(4, a=10.0, seed=1001, slab=(4.0, 6.0))
     ang = angular.product(2, 8, 3)
     G = 5
-    sigt, S = problems.cfg2_xs(G=G, L=ang.L)
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     sigt = sigt * 10.0
     S = S * 10.0
     qmom, _ = problems.beam_source_moments(m, ang, S, sigt)

@@ -95,8 +95,8 @@ def test_single_tet(ops_module):
     assert st["status"] == 0 and st["iterates"] == ost["iters"] and rel(a, b) <= 1e-7
 
 
-def test_sweep_cfg1_source(ops_module, cfg1_mesh):
-    m = cfg1_mesh
+def test_sweep_cube48k_source(ops_module, cube48k_mesh):
+    m = cube48k_mesh
     ang = angular.level_symmetric_s8()
     lib_op, orc = make_pair(ops_module, m, ang, [1.0], [[0.9]])
     c = m.centroids()
@@ -291,11 +291,11 @@ def test_graph_and_level_launch_modes_agree(ops_module):
             assert np.array_equal(v, ref), mode
 
 
-def test_cfg1_full_solve_matches_oracle(ops_module):
+def test_cube48k_full_solve_matches_oracle(ops_module):
     """48,000 tets, S8, P0 c = 0.9, central-cube source, GMRES(30) at 1e-8: the
     reference's own CPU-runnable configuration, solved end to end."""
     from lionbolt_amd import problems
-    m, ang, sigt, sigs, q = problems.cfg1()
+    m, ang, sigt, sigs, q = problems.cube_source_48k()
     lib_op, orc = make_pair(ops_module, m, ang, sigt, sigs)
     a, st = lib_op.boltzmann_matinv(q, kind="gmres", space="psi", m=30, rtol=1e-8)
     b, ost = orc.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)

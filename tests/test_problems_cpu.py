@@ -1,4 +1,4 @@
-"""Synthetic workloads: the batch union mesh, the cfg4 draw and the cube beam's
+"""Synthetic workloads: the batch union mesh, the uneven-beam draw and the cube beam's
 uncollided flux (CPU only)."""
 import math
 
@@ -19,9 +19,9 @@ def test_union_neighbours_match_a_fresh_face_search():
         assert inside.min() >= eoff[p] and inside.max() < eoff[p + 1]
 
 
-def test_cfg4_draw_is_seeded_and_in_range():
-    a = problems.cfg4(nprob=64)
-    b = problems.cfg4(nprob=64)
+def test_uneven_beams_draw_is_seeded_and_in_range():
+    a = problems.uneven_beams(nprob=64)
+    b = problems.uneven_beams(nprob=64)
     assert [p["n"] for p in a] == [p["n"] for p in b]
     for p in a:
         assert 7 <= p["n"] <= 21 and 5.0 <= p["a"] <= 20.0
