@@ -284,7 +284,9 @@ int lb_op_info(lb_op* op, int64_t* NE, int64_t* NI, int64_t* nmom, int64_t* maxl
 /* ---- device-resident entry points (no reference counterpart: they expose
  * the same operators on HBM-resident data so callers skip the PCIe copies) -- */
 
-/* T^-1 over G groups starting at group g0: psi = T^-1 q (in place allowed). */
+/* T^-1 over G groups starting at group g0: psi = T^-1 q (in place allowed).
+ * Asynchronous: the sweep is enqueued on the operator's stream and may still
+ * run on return (lb_op_synchronize waits for it). */
 int lb_transport_matinv_dev(lb_op* op, const double* d_q, double* d_psi, int g0, int G);
 
 /* phi[lm][sdof][g] = sum_i D[lm][i] psi[i][sdof][g] (discrete to moment). */

@@ -123,6 +123,10 @@ int set_problems(lb_op* op, int P, const int64_t* eoff) {
   BatchTable& B = op->batch;
   B.d_rec.release();
   B.d_ck.release();
+  for (auto& kv : B.seg) {
+    kv.second.d_tiles.release();
+    kv.second.d_tile_off.release();
+  }
   B.seg.clear();
   B.P = P;
   B.eoff.assign(eoff, eoff + P + 1);
@@ -203,9 +207,15 @@ SegTable* seg_table(lb_op* op, int rows, int* rc) {
         t.tiles.insert(t.tiles.end(), {p, r * op->NENK + c, std::min(kTileLen, c1 - c), 0});
     t.tile_off.push_back((int32_t)(t.tiles.size() / 4));
   }
-  if ((*rc = upload(op, t.d_tiles, t.tiles.data(), t.tiles.size() * sizeof(int64_t))) != LB_OK) return nullptr;
-  if ((*rc = upload(op, t.d_tile_off, t.tile_off.data(), t.tile_off.size() * sizeof(int32_t))) != LB_OK)
+  if ((*rc = upload(op, t.d_tiles, t.tiles.data(), t.tiles.size() * sizeof(int64_t))) != LB_OK) {
+    t.d_tiles.release();
     return nullptr;
+  }
+  if ((*rc = upload(op, t.d_tile_off, t.tile_off.data(), t.tile_off.size() * sizeof(int32_t))) != LB_OK) {
+    t.d_tiles.release();
+    t.d_tile_off.release();
+    return nullptr;
+  }
   return &B.seg.emplace(rows, std::move(t)).first->second;
 }
 

@@ -436,6 +436,9 @@ struct FgmresResult {
 // the solve: when any 3 consecutive preconditioned steps reduce the residual
 // estimate by less than a factor 2 per step on average (a useful approximate
 // inverse does better than that), or when P returns a non-finite or zero vector.
+// A cycle that ends because the basis stopped growing leaves the decision to the
+// next explicit residual, which must be smaller than the cycle's first one, else
+// the solve ends with LB_E_BREAKDOWN (as gmres_shifted).
 int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const double* rhs, double* sol, double* V,
            double* Z, double* R, int m, double rtol, double atol, int maxit, int verbose, const Snapshot* snap,
            FgmresResult* res, bool zero_x0 = false) {
@@ -463,6 +466,7 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
   };
   bool done = false;
   double hn_last = 1.0;
+  double last_beta = -1.0;   // explicit residual before a cycle that ended on an invariant space
   if (zero_x0) LB_HIP(hipMemsetAsync(sol, 0, n * sizeof(double), s));
   bool initial = zero_x0;   // x0 = 0: the first residual is rhs, no operator application
   while (!done) {
@@ -486,6 +490,10 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
     }
     if (beta <= eps) {
       status = 0;
+      break;
+    }
+    if (last_beta >= 0.0 && !(beta < last_beta)) {   // a cycle that ended early made no progress
+      status = LB_E_BREAKDOWN;
       break;
     }
     if (maxit > 0 && iters >= maxit) break;
@@ -568,6 +576,9 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
       status = 0;
       done = true;
     }
+    // a basis that stopped growing hands the decision to the explicit residual,
+    // which must then be smaller than this cycle's starting residual
+    last_beta = hn_last > 0.0 ? -1.0 : beta;
   }
   res->status = status;
   res->iterates = iters;

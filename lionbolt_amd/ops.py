@@ -32,6 +32,25 @@ def _torch_done():
     torch.cuda.current_stream().synchronize()
 
 
+def _dev(t, need, name):
+    """Device pointer of a tensor handed to a ``lb_*_dev`` entry point, after
+    checking what the library assumes of it: float64, on the GPU, contiguous and
+    holding at least ``need`` entries.  A tensor that fails any check would let
+    the kernels read or write past its end."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name}: expected float64, got {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: expected a tensor on the GPU")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    if t.numel() < need:
+        raise ValueError(f"{name}: {t.numel()} entries, at least {need} needed")
+    return t.data_ptr()
+
+
 class XSType:
     """Cross sections of one group: ``t(Nm)`` and Legendre moments ``s(0:L, Nm)``."""
 
@@ -135,10 +154,12 @@ class Operator:
     def batch_solve_dev(self, qmom, phi, m=30, rtol=1e-8, atol=1e-12, maxit=0, raise_on_fail=True):
         """Per-problem flux-moment GMRES(m) of the whole batch in lockstep (single group).
         ``qmom``/``phi``: cuda float64 tensors of nmom * NENK entries."""
+        nphi = self.nmom * self.NENK
+        q_p, phi_p = _dev(qmom, nphi, "qmom"), _dev(phi, nphi, "phi")
         _torch_done()
         opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
         stats = (_lib.SolveStats * self.P)()
-        rc = _lib.lib().lb_batch_solve_dev(self.h, qmom.data_ptr(), ctypes.byref(opts), phi.data_ptr(), stats)
+        rc = _lib.lib().lb_batch_solve_dev(self.h, q_p, ctypes.byref(opts), phi_p, stats)
         if raise_on_fail:
             _lib.check(rc)
         return [s.as_dict() for s in stats]
@@ -199,30 +220,40 @@ class Operator:
 
     # -- device entry points (torch tensors on cuda, group-batched layout) --------
     def transport_matinv_dev(self, q, psi, g0=0, G=1):
+        """psi = T^-1 q over groups g0 .. g0 + G - 1 (layout [i][e][k][g]).  Asynchronous:
+        the sweep is enqueued on the operator's stream and may still run on return;
+        ``synchronize()`` waits for it (as the other device calls do before returning)."""
+        need = self.n * int(G)
+        q_p, psi_p = _dev(q, need, "q"), _dev(psi, need, "psi")
         _torch_done()
-        _lib.check(_lib.lib().lb_transport_matinv_dev(self.h, q.data_ptr(), psi.data_ptr(), g0, G))
+        _lib.check(_lib.lib().lb_transport_matinv_dev(self.h, q_p, psi_p, g0, G))
 
     def moments_dev(self, psi, phi, G=1):
+        psi_p = _dev(psi, self.n * int(G), "psi")
+        phi_p = _dev(phi, self.nmom * self.NENK * int(G), "phi")
         _torch_done()
-        _lib.check(_lib.lib().lb_moments_dev(self.h, psi.data_ptr(), phi.data_ptr(), G))
+        _lib.check(_lib.lib().lb_moments_dev(self.h, psi_p, phi_p, G))
         self.synchronize()
 
     def moments_to_discrete_dev(self, phi, psi, g0=0, G=1, accumulate=False):
+        phi_p = _dev(phi, self.nmom * self.NENK * int(G), "phi")
+        psi_p = _dev(psi, self.n * int(G), "psi")
         _torch_done()
-        _lib.check(_lib.lib().lb_moments_to_discrete_dev(self.h, phi.data_ptr(), psi.data_ptr(), g0, G,
-                                                         int(accumulate)))
+        _lib.check(_lib.lib().lb_moments_to_discrete_dev(self.h, phi_p, psi_p, g0, G, int(accumulate)))
         self.synchronize()
 
     def multigroup_solve_dev(self, transfer, phi, qmom=None, kind="gmres", space="moment", m=30, rtol=1e-8,
                              atol=1e-12, maxit=0, raise_on_fail=True):
         """Group-by-group downscatter solve; ``transfer`` (G, G, Nm, L+1) host array [g', g]."""
+        need = self.G * self.nmom * self.NENK
+        phi_p = _dev(phi, need, "phi")
+        q_p = None if qmom is None else _dev(qmom, need, "qmom")
         _torch_done()
         transfer = np.ascontiguousarray(transfer, dtype=np.float64)
         opts = _lib.SolverOpts({"gmres": _lib.SOLVER_GMRES, "si": _lib.SOLVER_SI, "fgmres": _lib.SOLVER_FGMRES}[kind],
                                {"psi": _lib.SPACE_PSI, "moment": _lib.SPACE_MOMENT}[space], m, maxit, rtol, atol, 0)
         stats = (_lib.SolveStats * self.G)()
-        rc = _lib.lib().lb_multigroup_solve_dev(self.h, None if qmom is None else qmom.data_ptr(),
-                                                _lib.dptr(transfer), ctypes.byref(opts), phi.data_ptr(), stats)
+        rc = _lib.lib().lb_multigroup_solve_dev(self.h, q_p, _lib.dptr(transfer), ctypes.byref(opts), phi_p, stats)
         if raise_on_fail:
             _lib.check(rc)
         return [s.as_dict() for s in stats]
@@ -235,44 +266,53 @@ class Operator:
         _lib.check(_lib.lib().lb_op_set_preconditioner(self.h, p.ctypes.data_as(_lib._f), p.size, f))
 
     def mlp_apply_dev(self, r, z, g0=0):
+        nphi = self.nmom * self.NENK
+        r_p, z_p = _dev(r, nphi, "r"), _dev(z, nphi, "z")
         _torch_done()
-        _lib.check(_lib.lib().lb_mlp_apply_dev(self.h, r.data_ptr(), z.data_ptr(), int(g0)))
+        _lib.check(_lib.lib().lb_mlp_apply_dev(self.h, r_p, z_p, int(g0)))
         self.synchronize()
 
     def mlp_features_dev(self, r, g0=0):
+        r_p = _dev(r, self.nmom * self.NENK, "r")
         _torch_done()
         import torch
         feat = torch.empty((self.NENK, 18), dtype=torch.float32, device=r.device)
         s = ctypes.c_double()
-        _lib.check(_lib.lib().lb_mlp_features_dev(self.h, r.data_ptr(), int(g0), feat.data_ptr(), ctypes.byref(s)))
+        _lib.check(_lib.lib().lb_mlp_features_dev(self.h, r_p, int(g0), feat.data_ptr(), ctypes.byref(s)))
         self.synchronize()
         return feat, s.value
 
     def moment_rhs_dev(self, q, b, g0=0):
+        q_p, b_p = _dev(q, self.n, "q"), _dev(b, self.nmom * self.NENK, "b")
         _torch_done()
-        _lib.check(_lib.lib().lb_moment_rhs_dev(self.h, q.data_ptr(), b.data_ptr(), int(g0)))
+        _lib.check(_lib.lib().lb_moment_rhs_dev(self.h, q_p, b_p, int(g0)))
         self.synchronize()
 
     def discrete_source_dev(self, qmom, q):
+        qm_p, q_p = _dev(qmom, self.nmom * self.NENK, "qmom"), _dev(q, self.n, "q")
         _torch_done()
-        _lib.check(_lib.lib().lb_discrete_source_dev(self.h, qmom.data_ptr(), q.data_ptr()))
+        _lib.check(_lib.lib().lb_discrete_source_dev(self.h, qm_p, q_p))
         self.synchronize()
 
     def moment_matvec_dev(self, x, y, g0=0):
+        nphi = self.nmom * self.NENK
+        x_p, y_p = _dev(x, nphi, "x"), _dev(y, nphi, "y")
         _torch_done()
-        _lib.check(_lib.lib().lb_moment_matvec_dev(self.h, x.data_ptr(), y.data_ptr(), int(g0)))
+        _lib.check(_lib.lib().lb_moment_matvec_dev(self.h, x_p, y_p, int(g0)))
         self.synchronize()
 
     def gmres_snapshots_dev(self, q, phi, snaps, m=30, rtol=1e-10, atol=1e-12, maxit=0):
         """Flux-moment GMRES(m) recording the iterate after every Arnoldi step into
         ``snaps`` (max_snaps x nmom*NENK); returns (count, stats)."""
+        nphi = self.nmom * self.NENK
+        q_p, phi_p = _dev(q, self.n, "q"), _dev(phi, nphi, "phi")
+        s_p = _dev(snaps, snaps.shape[0] * nphi, "snaps")
         _torch_done()
         opts = _lib.SolverOpts(_lib.SOLVER_GMRES, _lib.SPACE_MOMENT, m, maxit, rtol, atol, 0)
         st = _lib.SolveStats()
         cnt = ctypes.c_int()
-        _lib.check(_lib.lib().lb_gmres_snapshots_dev(self.h, q.data_ptr(), ctypes.byref(opts), phi.data_ptr(),
-                                                     snaps.data_ptr(), snaps.shape[0], ctypes.byref(cnt),
-                                                     ctypes.byref(st)))
+        _lib.check(_lib.lib().lb_gmres_snapshots_dev(self.h, q_p, ctypes.byref(opts), phi_p, s_p, snaps.shape[0],
+                                                     ctypes.byref(cnt), ctypes.byref(st)))
         return cnt.value, st.as_dict()
 
     def synchronize(self):

@@ -183,6 +183,10 @@ struct OrcOp {
   bool phys = false;
   bool deltadown = false;           // ScatteringOp%BuildDeltaDown
   std::vector<double> xsdd /*[m]*/;
+  // K applied in moment form (orc_use_moment_kernel): D [nmom][NI], M [NI][nmom]
+  int knmom = 0;
+  std::vector<double> kD, kM;
+  std::vector<int32_t> kdeg;
   // workspaces
   std::vector<double> xw, sxw;
 };
@@ -679,6 +683,8 @@ void orc_build_delta_down(void* h, const double* xsdd /*[Nm]*/) {
 // SNMGXS (ScatteringAlgebra.f90:13-55): x(NENK,NI) <- x . YYS; dgemm for one
 // material, per-dof dgemv('T') over mat2sd otherwise.  With a delta-down kernel,
 // ApplyDeltaDown (ScatteringAlgebra.f90:62-90): x(sd + i NENK) *= XSdd(mat).
+void orc_scatter_moment_matvec(void* h, const double* D, const double* M, const int32_t* deg, int nmom, double* x);
+
 void orc_scatter_matvec(void* h, double* x) {
   OrcOp* op = static_cast<OrcOp*>(h);
   const i64 NI = op->NI, NENK = op->NENK;
@@ -686,6 +692,10 @@ void orc_scatter_matvec(void* h, double* x) {
     for (int m = 0; m < op->Nm; ++m)
       for (i64 sd : op->mat2sd[m])
         for (i64 i = 0; i < NI; ++i) x[sd + i * NENK] *= op->xsdd[m];
+    return;
+  }
+  if (op->knmom > 0) {
+    orc_scatter_moment_matvec(h, op->kD.data(), op->kM.data(), op->kdeg.data(), op->knmom, x);
     return;
   }
   double* xw = op->xw.data();
@@ -754,6 +764,22 @@ void orc_scatter_moment_matvec(void* h, const double* D /*[nmom][NI]*/, const do
   }
 }
 
+// Makes every later K application (scatter matvec, Boltzmann matvec, GMRES, SI)
+// use the moment form above with these D, M and degrees; nmom = 0 returns to
+// the dense kernel.  Not a reference routine: at 256 directions the dense kernel
+// costs 2 NI^2 flops per dof, which makes full-size oracle solves take many
+// minutes; the two forms agree to rounding by the addition theorem (pinned in
+// tests/test_oracle_scatter_krylov.py).
+void orc_use_moment_kernel(void* h, const double* D /*[nmom][NI]*/, const double* M /*[NI][nmom]*/,
+                           const int32_t* deg, int nmom) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  op->knmom = nmom > 0 ? nmom : 0;
+  if (op->knmom == 0) return;
+  op->kD.assign(D, D + (size_t)nmom * op->NI);
+  op->kM.assign(M, M + (size_t)nmom * op->NI);
+  op->kdeg.assign(deg, deg + nmom);
+}
+
 // BoltzmannOp%MatVec (BoltzmannOperator.f90:59-104), Tprecond: x <- x - T^{-1} K x.
 void orc_boltzmann_matvec(void* h, double* x) {
   OrcOp* op = static_cast<OrcOp*>(h);
@@ -765,10 +791,34 @@ void orc_boltzmann_matvec(void* h, double* x) {
   for (i64 j = 0; j < n; ++j) x[j] = op->sxw[j] - x[j];
 }
 
+// distdot (distdot.f:1-7, a BLAS ddot): products summed in index order inside
+// fixed blocks of 32,768 entries, then the block sums in block order, so the
+// result does not depend on the thread count (one block: the plain loop).
 static double ddot(i64 n, const double* a, const double* b) {
+  const i64 B = i64(1) << 15;
+  const i64 nb = (n + B - 1) / B;
+  if (nb <= 1) {
+    double s = 0.0;
+    for (i64 i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+  }
+  std::vector<double> part(nb);
+#pragma omp parallel for schedule(static)
+  for (i64 j = 0; j < nb; ++j) {
+    const i64 i1 = std::min(n, (j + 1) * B);
+    double s = 0.0;
+    for (i64 i = j * B; i < i1; ++i) s += a[i] * b[i];
+    part[j] = s;
+  }
   double s = 0.0;
-  for (i64 i = 0; i < n; ++i) s += a[i] * b[i];
+  for (i64 j = 0; j < nb; ++j) s += part[j];
   return s;
+}
+
+// y = x (elementwise, threaded)
+static void vcopy(i64 n, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (i64 j = 0; j < n; ++j) y[j] = x[j];
 }
 
 // SPARSKIT gmres (iters.f:1901-2206) with mgsro (:3465-3586) and givens (:3225-3260),
@@ -797,9 +847,10 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
   while (true) {
     // restart (label 100/10/20): r = rhs - A sol
     double* v0 = V.data();
-    std::memcpy(v0, sol, sizeof(double) * n);
+    vcopy(n, sol, v0);
     orc_boltzmann_matvec(h, v0);
     ++nmv;
+#pragma omp parallel for schedule(static)
     for (i64 j = 0; j < n; ++j) v0[j] = rhs[j] - v0[j];
     double alpha = std::sqrt(ddot(n, v0, v0));
     if (first) {
@@ -815,6 +866,7 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
       break;
     }
     alpha = 1.0 / alpha;
+#pragma omp parallel for schedule(static)
     for (i64 j = 0; j < n; ++j) v0[j] = alpha * v0[j];
     int k = 0;
     int ierr = 0;
@@ -822,7 +874,7 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
       ++k;
       record(fpar5);  // return requesting A * v_{k-1}
       double* vk = &V[(i64)k * n];
-      std::memcpy(vk, &V[(i64)(k - 1) * n], sizeof(double) * n);
+      vcopy(n, &V[(i64)(k - 1) * n], vk);
       orc_boltzmann_matvec(h, vk);
       ++nmv;
       // mgsro(.false., n, n, k+1, k+1, ...): orthogonalise v_k against v_0..v_{k-1}
@@ -836,10 +888,12 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
           const double* vi = &V[(i64)i * n];
           double fct = ddot(n, vk, vi);
           hh[i] = fct;
+#pragma omp parallel for schedule(static)
           for (i64 j = 0; j < n; ++j) vk[j] = vk[j] - fct * vi[j];
           if (fct * fct > thr) {
             fct = ddot(n, vk, vi);
             hh[i] = hh[i] + fct;
+#pragma omp parallel for schedule(static)
             for (i64 j = 0; j < n; ++j) vk[j] = vk[j] - fct * vi[j];
           }
           nrm0 = nrm0 - hh[i] * hh[i];
@@ -852,6 +906,7 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
           ierr = -3;
         } else {
           double fct = 1.0 / nrm1;
+#pragma omp parallel for schedule(static)
           for (i64 j = 0; j < n; ++j) vk[j] = vk[j] * fct;
         }
       }
@@ -908,12 +963,16 @@ int orc_gmres(void* h, int m, const double* rhs, double* sol, double rtol, doubl
         y[i] = y[i] / hcol(i)[i - 1];
       }
       double* u = V.data();
-      for (i64 j = 0; j < n; ++j) u[j] = u[j] * y[1];
+      const double y1 = y[1];
+#pragma omp parallel for schedule(static)
+      for (i64 j = 0; j < n; ++j) u[j] = u[j] * y1;
       for (int i = 1; i <= kk - 1; ++i) {
         const double* vi = &V[(i64)i * n];
         double yi = y[i + 1];
+#pragma omp parallel for schedule(static)
         for (i64 j = 0; j < n; ++j) u[j] = u[j] + yi * vi[j];
       }
+#pragma omp parallel for schedule(static)
       for (i64 j = 0; j < n; ++j) sol[j] = sol[j] + u[j];
     }
     if (ierr != 0 || singular) {
@@ -945,6 +1004,7 @@ int orc_source_iteration(void* h, double* s, double rtol, double atol, int maxit
   double norms = norm2_scaled(s, n);
   orc_scatter_matvec(h, w.data());
   orc_transport_matinv(h, w.data());
+#pragma omp parallel for schedule(static)
   for (i64 j = 0; j < n; ++j) s[j] = s[j] + w[j];
   int iter = 1;
   int status = -1;
@@ -958,6 +1018,7 @@ int orc_source_iteration(void* h, double* s, double rtol, double atol, int maxit
     }
     orc_scatter_matvec(h, w.data());
     orc_transport_matinv(h, w.data());
+#pragma omp parallel for schedule(static)
     for (i64 j = 0; j < n; ++j) s[j] = s[j] + w[j];
   }
   if (iters) *iters = iter;
@@ -973,7 +1034,9 @@ int orc_boltzmann_matinv(void* h, double* s, int kind, int m, double rtol, doubl
   orc_transport_matinv(h, s);
   if (kind == 0) {
     std::vector<double> xg(n);
-    for (i64 j = 0; j < n; ++j) xg[j] = s[j] / op->sigt[0];
+    const double st0 = op->sigt[0];
+#pragma omp parallel for schedule(static)
+    for (i64 j = 0; j < n; ++j) xg[j] = s[j] / st0;
     int st = orc_gmres(h, m, s, xg.data(), rtol, atol, maxit, hist, hist_cap, iters, nmatvec);
     std::memcpy(s, xg.data(), sizeof(double) * n);
     return st;

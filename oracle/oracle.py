@@ -54,6 +54,7 @@ def lib():
                                            ctypes.c_double, ctypes.c_int, _d, ctypes.c_int, _ci, _ci]
         L.orc_build_delta_down.argtypes = [ctypes.c_void_p, _d]
         L.orc_scatter_moment_matvec.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int, _d]
+        L.orc_use_moment_kernel.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int]
         L.orc_fluence.argtypes = [ctypes.c_void_p, _d, _d]
         L.orc_norm2.restype = ctypes.c_double
         L.orc_norm2.argtypes = [_d, ctypes.c_int64]
@@ -149,6 +150,20 @@ class OracleOp:
         assert D.shape == (deg.size, self.NI) and M.shape == (self.NI, deg.size)
         lib().orc_scatter_moment_matvec(self.h, _p(D), _p(M), deg.ctypes.data_as(_i32), deg.size, _p(x))
         return x
+
+    def use_moment_kernel(self, D=None, M=None, deg=None):
+        """Apply K in moment form from now on (every scatter/Boltzmann product and
+        solve); no arguments restores the dense kernel.  For full-size checks, where
+        the dense kernel's 2 NI^2 flops per dof would take minutes per product."""
+        if D is None:
+            lib().orc_use_moment_kernel(self.h, None, None, None, 0)
+            return
+        self._kD = np.ascontiguousarray(D, dtype=np.float64)
+        self._kM = np.ascontiguousarray(M, dtype=np.float64)
+        self._kdeg = np.ascontiguousarray(deg, dtype=np.int32)
+        assert self._kD.shape == (self._kdeg.size, self.NI) and self._kM.shape == (self.NI, self._kdeg.size)
+        lib().orc_use_moment_kernel(self.h, _p(self._kD), _p(self._kM), self._kdeg.ctypes.data_as(_i32),
+                                    self._kdeg.size)
 
     def boltzmann_matvec(self, s):
         x = self._vec(s)

@@ -1,22 +1,97 @@
-"""The benchmark's full shape (384,000 tets, 256 directions), checked through
-properties that do not need the oracle, which is too slow at this size:
+"""The benchmark's full shape (384,000 tets, 256 directions).
 
+Against the oracle at this size (one group; the oracle sweeps on all host cores):
+* one sweep T^-1 q and one Boltzmann product (I - T^-1 K) x to 1e-12;
+* the scalar flux of the first group's beam solve (GMRES to 1e-8; the oracle's
+  psi-space GMRES(10) from q / sigma_t as BoltzmannOperator.f90:106-151, with K
+  in moment form, which the oracle's tests pin to its dense kernel) to 1e-5.
+Through properties that hold at any size:
 * the sweep is linear: T^-1 (q1 + 2 q2) = T^-1 q1 + 2 T^-1 q2 to 1e-12
   relative (FP64 rounding only);
 * every launch mode gives the same flux bit for bit: level launches, the
   direction-resident kernel and the LDS kernel (its slot plan at this size
   reads 98.4 % of the upstream traces from ring slots, 0.7 % from staging
   copies and 1 % from global memory, so every trace path runs);
-* a three-group sweep equals three single-group sweeps bit for bit;
+* three- and 24-group sweeps equal single-group sweeps bit for bit (24 groups
+  is the benchmark's sweep, streaming loads and graph-replayed level launches);
 * a two-group downscatter solve with the beam source reaches a true relative
-  residual below its tolerance in every group.
-The oracle itself pins the same code at small sizes (test_gpu_parity.py)."""
+  residual below its tolerance in every group."""
 import numpy as np
 import pytest
 
-from lionbolt_amd import problems
+from lionbolt_amd import angular, problems
 
 pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+@pytest.fixture(scope="module")
+def full_oracle(full):
+    """The oracle operator at full size, with the first group's physics and K in moment form."""
+    from oracle.oracle import OracleOp
+    m, ang = full
+    sigt, S = problems.downscatter_xs(G=1, L=ang.L)
+    orc = OracleOp(m, ang)
+    orc.set_physics(sigt[0], S[0, 0])
+    D, Mm = angular.moment_matrices(ang)
+    orc.use_moment_kernel(D, Mm, angular.moment_degree(ang.L))
+    print("full size: oracle operator built", flush=True)
+    yield orc
+    del orc
+
+
+@pytest.mark.timeout(600)
+def test_full_size_sweep_and_boltzmann_product_match_oracle(full, full_oracle):
+    from lionbolt_amd.ops import Operator
+    m, ang = full
+    sigt, S = problems.downscatter_xs(G=1, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics(sigt[0], S[0, 0])
+    q = np.random.default_rng(31).standard_normal(op.n)
+    a = op.transport_matinv(q)
+    b = full_oracle.transport_matinv(q)
+    assert rel(a, b) <= 1e-12
+    print("full size: sweep matches the oracle", flush=True)
+    del a, b
+    full_oracle.use_moment_kernel()              # the reference's dense direction x direction kernel
+    try:
+        a = op.boltzmann_matvec(q)
+        b = full_oracle.boltzmann_matvec(q)
+    finally:
+        D, Mm = angular.moment_matrices(ang)
+        full_oracle.use_moment_kernel(D, Mm, angular.moment_degree(ang.L))
+    assert rel(a, b) <= 1e-12
+    op.close()
+
+
+@pytest.mark.timeout(900)
+def test_full_size_beam_flux_matches_oracle_gmres(full, full_oracle):
+    """The first group of the 24-group beam problem (its first-collision source,
+    P3, the 3x slab): GPU flux-moment GMRES(30) against the oracle's psi-space
+    GMRES(10), both to 1e-8; scalar flux (LinearOperatorClass.f90:154-174) to 1e-5."""
+    from lionbolt_amd.ops import Operator
+    m, ang = full
+    G = 1
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
+    qmom, _ = problems.beam_source_moments(m, ang, S, sigt)
+    _, Mm = angular.moment_matrices(ang)
+    q = (Mm @ qmom[0]).reshape(-1)
+    op = Operator(m, ang)
+    op.set_physics(sigt[0], S[0, 0])
+    a, st = op.boltzmann_matinv(q, kind="gmres", space="moment", m=30, rtol=1e-8)
+    assert st["status"] == 0 and st["true_residual"] <= 1e-8
+    phi_gpu = op.fluence(a)
+    del a
+    op.close()
+    print(f"full size: GPU solve {st['iterates']} iterates", flush=True)
+    b, ost = full_oracle.boltzmann_matinv(q, kind="gmres", m=10, rtol=1e-8)
+    assert ost["status"] == 0
+    phi_orc = full_oracle.fluence(b)
+    print(f"full size: oracle solve {ost['iters']} iterates, flux rel diff {rel(phi_gpu, phi_orc):.2e}", flush=True)
+    assert rel(phi_gpu, phi_orc) <= 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -87,6 +162,36 @@ def test_full_size_group_batch_equals_single_groups(full):
         op.transport_matinv_dev(qg, pg, g, 1)
         op.synchronize()
         assert torch.equal(pg, pv[:, g]), g
+    op.close()
+
+
+def test_full_size_24_group_sweep_equals_single_groups(full):
+    """The benchmark's sweep (24 groups, streaming loads, graph-replayed level
+    launches): groups 0, 11 and 23 equal single-group sweeps bit for bit."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m, ang = full
+    G = 24
+    sigt, _ = problems.downscatter_xs(G=G, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, np.zeros((G, 2, ang.L + 1)))
+    op.set_sweep_mode("graph")
+    cells = m.NE * 4 * ang.NI
+    gen = torch.Generator(device="cuda").manual_seed(24)
+    q = torch.rand(cells * G, dtype=torch.float64, device="cuda", generator=gen)
+    psi = torch.full_like(q, float("nan"))
+    op.transport_matinv_dev(q, psi, 0, G)
+    op.synchronize()
+    qv, pv = q.view(cells, G), psi.view(cells, G)
+    op.set_sweep_mode("direction_lds")
+    op.set_direction_block(512)
+    for g in (0, 11, 23):
+        qg = qv[:, g].contiguous()
+        pg = torch.full_like(qg, float("nan"))
+        op.transport_matinv_dev(qg, pg, g, 1)
+        op.synchronize()
+        assert torch.equal(pg, pv[:, g]), g
+        print(f"full size: 24-group sweep, group {g} equals its single-group sweep", flush=True)
     op.close()
 
 

@@ -199,12 +199,13 @@ def test_moment_space_gmres_solution(ops_module):
     assert rel(lib_op.fluence(a), orc.fluence(b)) <= 1e-7
 
 
-def test_batched_groups_equal_single_group_sweeps(ops_module):
+@pytest.mark.parametrize("G", [5, 8, 24])
+def test_batched_groups_equal_single_group_sweeps(ops_module, G):
+    """G = 8 and 24 take the streaming (non-temporal) loads of the many-group sweep."""
     import torch
     from lionbolt_amd.ops import Operator
     m = M.jittered_cube(4, a=3.0)
     ang = angular.level_symmetric_s8()
-    G = 5
     sigt = np.linspace(0.2, 2.0, G)[:, None]
     op = Operator(m, ang)
     op.set_physics_groups(sigt, np.zeros((G, 1, 1)))

@@ -79,6 +79,34 @@ def test_moment_form_equals_dense_kernel(small_mesh):
     assert np.abs(mom - dense).max() <= 1e-13 * np.abs(dense).max()
 
 
+def test_moment_kernel_mode_equals_dense_solves():
+    """orc_use_moment_kernel: the Boltzmann product and a GMRES solve with K in
+    moment form equal the dense-kernel ones (the full-size GPU checks run the
+    oracle this way).  49,152 unknowns, so the blocked dot products of the
+    oracle's GMRES span two blocks."""
+    m = M.jittered_cube(4, a=4.0, seed=5)
+    m = M.TetMesh(m.rg, m.node, np.where(m.centroids()[:, 2] > 2.0, 2, 1))
+    ang = angular.snprep(3)
+    op = OracleOp(m, ang)
+    assert op.n > 2 ** 15
+    sigs = np.array([[0.5, 0.2, 0.1, 0.05], [1.6, 0.7, 0.3, 0.1]])
+    op.set_physics([1.0, 2.0], sigs)
+    q = np.random.default_rng(4).random(op.n)
+    dense_mv = op.boltzmann_matvec(q)
+    dense_x, dst = op.boltzmann_matinv(q, kind="gmres", m=10, rtol=1e-11)
+    si_x, sst = op.boltzmann_matinv(q, kind="si", rtol=1e-12)
+    D, Mm = angular.moment_matrices(ang)
+    op.use_moment_kernel(D, Mm, angular.moment_degree(ang.L))
+    mom_mv = op.boltzmann_matvec(q)
+    mom_x, mst = op.boltzmann_matinv(q, kind="gmres", m=10, rtol=1e-11)
+    assert dst["status"] == 0 and mst["status"] == 0 and sst["status"] == 0
+    assert np.abs(mom_mv - dense_mv).max() <= 1e-13 * np.abs(dense_mv).max()
+    assert np.abs(mom_x - dense_x).max() <= 1e-10 * np.abs(dense_x).max()
+    assert np.abs(si_x - dense_x).max() <= 1e-9 * np.abs(dense_x).max()
+    op.use_moment_kernel()
+    assert np.array_equal(op.boltzmann_matvec(q), dense_mv)
+
+
 def test_delta_down_kernel(small_mesh):
     """deltadownMMS (validation/operators/deltadownMMS.f90): XSdd = (1, 10) on two
     materials, s = 1 -> K s = XSdd(mat) in every direction; then any vector is
