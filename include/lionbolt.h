@@ -178,6 +178,18 @@ int lb_op_set_preconditioner(lb_op* op, const float* params, int nparams, int fm
 /* d_z [16][NENK] = P(d_r [16][NENK]) for group g0 (single-group moment vectors). */
 int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0);
 
+/* lb_mlp_apply_dev without any allocation or host synchronisation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight solver loops).  The residual's norm
+ * is reduced in a file-scope device array of LB_NOALLOC_SCRATCH_BYTES per GPU
+ * (its need does not grow with the mesh, so no size is refused for it), held
+ * until the process ends; the network must be set first (lb_op_set_preconditioner,
+ * which also uploads the element sizes the inputs use).  Calls on one GPU share
+ * that array: calls that overlap on one GPU race.  Asynchronous on the
+ * operator's stream; the same output as lb_mlp_apply_dev. */
+#define LB_NOALLOC_SCRATCH_BYTES 16448
+int lb_mlp_apply_noalloc_dev(lb_op* op, const double* d_r, double* d_z, int g0);
+
 /* The network's inputs for d_r: d_feat [NENK][18] (FP32) and the scale s. */
 int lb_mlp_features_dev(lb_op* op, const double* d_r, int g0, float* d_feat, double* scale);
 

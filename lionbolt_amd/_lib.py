@@ -65,6 +65,7 @@ SIGNATURES = {
     "lb_batch_solve_dev": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(SolverOpts), _vp, ctypes.POINTER(SolveStats)]),
     "lb_op_set_preconditioner": (ctypes.c_int, [_vp, _f, ctypes.c_int, ctypes.c_int]),
     "lb_mlp_apply_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_mlp_apply_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_mlp_features_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_double)]),
     "lb_mlp_train_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_uint64, _f]),

@@ -364,6 +364,17 @@ int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0) {
   });
 }
 
+int lb_mlp_apply_noalloc_dev(lb_op* op, const double* d_r, double* d_z, int g0) {
+  return guarded([&]() -> int {
+    LB_TRY(check_ready(op, "lb_mlp_apply_noalloc_dev"));
+    if (!d_r || !d_z || g0 < 0 || g0 >= op->G) {
+      set_error("lb_mlp_apply_noalloc_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    return mlp_apply_noalloc(op, d_r, d_z, g0);
+  });
+}
+
 int lb_mlp_features_dev(lb_op* op, const double* d_r, int g0, float* d_feat, double* scale) {
   return guarded([&]() -> int {
     LB_TRY(check_ready(op, "lb_mlp_features_dev"));

@@ -255,6 +255,11 @@ int launch_scale(hipStream_t s, int64_t n, double a, double* x);
 int launch_delta_down(lb_op* op, int64_t n, double* x);
 void release_blas(lb_op* op);
 int launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
+// Blocks of the deterministic dot product's first stage (fixed grid).
+constexpr int kDotPartials = 2048;
+// Enqueues res[0] = <x, y> through part[kDotPartials] (fixed grid and reduction
+// tree, so the bits do not vary from run to run); no host synchronisation.
+int launch_dot(hipStream_t s, int64_t n, const double* x, const double* y, double* part, double* res);
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 // out[i] = <w, V_i> for i < c (V_i = V + i ld) and, with self, out[c] = <w, w>;
 // one host synchronisation.
@@ -275,6 +280,7 @@ int group_dot(lb_op* op, int64_t n, int B, const double* x, const double* y, dou
 // lb_mlp.hip
 int mlp_set(lb_op* op, const float* params, int nparams, int fmt);
 int mlp_apply(lb_op* op, const double* r, double* z, int g0);
+int mlp_apply_noalloc(lb_op* op, const double* r, double* z, int g0);
 int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale);
 
 // lb_batch.hip

@@ -850,7 +850,7 @@ __global__ __launch_bounds__(kBlock) void scale_kernel(int64_t n, double a, doub
 }
 
 // Deterministic two-stage dot product: fixed grid, fixed reduction tree.
-constexpr int kDotBlocks = 2048;
+constexpr int kDotBlocks = kDotPartials;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   sh[threadIdx.x] = v;
@@ -1479,6 +1479,13 @@ int launch_copy(hipStream_t s, int64_t n, const double* x, double* y) {
   return LB_OK;
 }
 
+int launch_dot(hipStream_t s, int64_t n, const double* x, const double* y, double* part, double* res) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kBlock), 0, s, n, x, y, part);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kBlock), 0, s, kDotBlocks, part, res);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
 int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   LB_TRY(op->w_red.reserve((kDotBlocks + 8) * sizeof(double)));
   if (!op->w_host.p) {
@@ -1487,9 +1494,7 @@ int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out) {
   }
   double* part = op->w_red.as<double>();
   double* res = part + kDotBlocks;
-  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kBlock), 0, op->stream, n, x, y, part);
-  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kBlock), 0, op->stream, kDotBlocks, part, res);
-  LB_HIP(hipGetLastError());
+  LB_TRY(launch_dot(op->stream, n, x, y, part, res));
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
   *out = *static_cast<double*>(op->w_host.p);

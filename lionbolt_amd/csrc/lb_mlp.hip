@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lb_internal.hpp"
@@ -54,10 +55,16 @@ struct FeatArgs {
   const double* sigs;      // [Nm][Gtot][L+1]
   int Gtot, g0, L;
   const double* elem_h;    // [NE]
+  // when set, {1 / rms(r), output scale} computed on the device replace inv_s and
+  // the kernels' scale argument (the allocation-free entry point)
+  const double* dev_scale = nullptr;
 };
 
-__device__ __forceinline__ float feature(const FeatArgs& F, int64_t sd, int f) {
-  if (f < 16) return (float)(F.r[(int64_t)f * F.NENK + sd] * F.inv_s);
+__device__ __forceinline__ double input_scale(const FeatArgs& F) { return F.dev_scale ? F.dev_scale[0] : F.inv_s; }
+__device__ __forceinline__ double output_scale(const FeatArgs& F, double s) { return F.dev_scale ? F.dev_scale[1] : s; }
+
+__device__ __forceinline__ float feature(const FeatArgs& F, int64_t sd, int f, double inv_s) {
+  if (f < 16) return (float)(F.r[(int64_t)f * F.NENK + sd] * inv_s);
   const int64_t e = sd >> 2;
   const int m = F.mat[e];
   const double st = F.sigt[(int64_t)m * F.Gtot + F.g0];
@@ -69,8 +76,9 @@ __device__ __forceinline__ float feature(const FeatArgs& F, int64_t sd, int f) {
 __global__ __launch_bounds__(256) void features_kernel(FeatArgs F, float* __restrict__ out) {
   const int64_t sd = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (sd >= F.NENK) return;
+  const double inv_s = input_scale(F);
 #pragma unroll
-  for (int f = 0; f < kIn; ++f) out[sd * kIn + f] = feature(F, sd, f);
+  for (int f = 0; f < kIn; ++f) out[sd * kIn + f] = feature(F, sd, f, inv_s);
 }
 
 __device__ __forceinline__ float act(float x) { return tanhf(x); }
@@ -83,9 +91,11 @@ __global__ __launch_bounds__(256) void mlp_fp32_kernel(FeatArgs F, const float* 
   __syncthreads();
   const int64_t sd = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (sd >= F.NENK) return;
+  const double inv_s = input_scale(F);
+  s = output_scale(F, s);
   float x[kIn], h[kH], g[kH];
 #pragma unroll
-  for (int f = 0; f < kIn; ++f) x[f] = feature(F, sd, f);
+  for (int f = 0; f < kIn; ++f) x[f] = feature(F, sd, f, inv_s);
   for (int j = 0; j < kH; ++j) {
     float a = w[kB1 + j];
 #pragma unroll
@@ -141,6 +151,8 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
 #pragma unroll
   for (int r = 0; r < 4; ++r) b4[r] = W[kB4 + 4 * kb + r];
 
+  const double inv_s = input_scale(F);
+  s = output_scale(F, s);
   const int64_t ntiles = (F.NENK + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
     const int64_t sd = tile * 16 + col;
@@ -149,7 +161,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int f = 8 * kb + q;
-      xb[q] = (__bf16)((ok && f < kIn) ? feature(F, sd, f) : 0.f);
+      xb[q] = (__bf16)((ok && f < kIn) ? feature(F, sd, f, inv_s) : 0.f);
     }
     f32x4 a[4];
 #pragma unroll
@@ -304,23 +316,73 @@ int slot_input(int layer, int kb, int st, int q) {
   return 16 * (2 * st + (q >> 2)) + 4 * kb + (q & 3);
 }
 
+// Element sizes h = (6 V)^(1/3), uploaded once per operator.
+int ensure_elem_h(lb_op* op) {
+  if (op->d_elem_h.p) return LB_OK;
+  std::vector<double> h(op->NE);
+  for (int64_t e = 0; e < op->NE; ++e) h[e] = std::cbrt(op->h_detJ[e]);
+  return upload(op, op->d_elem_h, h.data(), h.size() * sizeof(double));
+}
+
+// Network inputs of residual r for group g0, except the residual's scale.
+FeatArgs feat_base(lb_op* op, const double* r, int g0) {
+  return FeatArgs{r, op->NENK, 0.0, op->d_mat.as<const int32_t>(), op->d_sigt.as<const double>(),
+                  op->d_sigs.as<const double>(), op->G, g0, op->L, op->d_elem_h.as<const double>()};
+}
+
 int feat_args(lb_op* op, const double* r, int g0, FeatArgs* F) {
   if (op->nmom != 16) {
     set_error("learned preconditioner: the network takes 16 flux moments (P3)");
     return LB_E_ARG;
   }
-  if (!op->d_elem_h.p) {
-    std::vector<double> h(op->NE);
-    for (int64_t e = 0; e < op->NE; ++e) h[e] = std::cbrt(op->h_detJ[e]);
-    LB_TRY(upload(op, op->d_elem_h, h.data(), h.size() * sizeof(double)));
-  }
+  LB_TRY(ensure_elem_h(op));
   const int64_t nphi = op->nmom * op->NENK;
   double rr = 0.0;
   LB_TRY(dot(op, nphi, r, r, &rr));
   const double rms = std::sqrt(rr / (double)nphi);
-  *F = FeatArgs{r, op->NENK, rms > 0.0 ? 1.0 / rms : 0.0, op->d_mat.as<const int32_t>(),
-                op->d_sigt.as<const double>(), op->d_sigs.as<const double>(), op->G, g0, op->L,
-                op->d_elem_h.as<const double>()};
+  *F = feat_base(op, r, g0);
+  F->inv_s = rms > 0.0 ? 1.0 / rms : 0.0;
+  return LB_OK;
+}
+
+// {1 / rms, output scale} from the squared norm rr of n values, with the
+// arithmetic of feat_args() and mlp_apply() on the host (IEEE square root and
+// division on both sides, so the two entry points get the same bits).
+__global__ void rms_scale_kernel(const double* __restrict__ rr, int64_t n, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(rr[0] / (double)n);
+    const double inv_s = rms > 0.0 ? 1.0 / rms : 0.0;
+    out[0] = inv_s;
+    out[1] = inv_s > 0.0 ? 1.0 / inv_s : 0.0;
+  }
+}
+
+// Scratch of the allocation-free apply: the dot product's partial sums, its
+// result and the two scales.  Its size does not depend on the mesh.  Each GPU
+// holds one copy for the life of the process; calls on one GPU share it.
+constexpr int kScratchDoubles = kDotPartials + 8;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current (the host-side name is only a stand-in for it).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_ptr(double** out) {
+  int dev = 0;
+  LB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("lb_mlp_apply_noalloc_dev: device ordinal out of range");
+    return LB_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    LB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
   return LB_OK;
 }
 
@@ -361,6 +423,7 @@ int mlp_set(lb_op* op, const float* params, int nparams, int fmt) {
   LB_HIP(hipMemcpy(op->d_mlp_w.p, params, kParams * sizeof(float), hipMemcpyHostToDevice));
   LB_TRY(op->d_mlp_pack.reserve(kPackLen * sizeof(uint16_t)));
   LB_HIP(hipMemcpy(op->d_mlp_pack.p, pk.data(), kPackLen * sizeof(uint16_t), hipMemcpyHostToDevice));
+  LB_TRY(ensure_elem_h(op));   // so that applying the network needs no allocation
   op->mlp = true;
   op->mlp_fmt = fmt;
   return LB_OK;
@@ -375,14 +438,8 @@ int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale)
   return LB_OK;
 }
 
-int mlp_apply(lb_op* op, const double* r, double* z, int g0) {
-  if (!op->mlp) {
-    set_error("learned preconditioner: no network set (lb_op_set_preconditioner)");
-    return LB_E_NOT_READY;
-  }
-  FeatArgs F;
-  LB_TRY(feat_args(op, r, g0, &F));
-  const double s = F.inv_s > 0.0 ? 1.0 / F.inv_s : 0.0;
+// The network's kernel for the operator's format, scales from F (host) or F.dev_scale.
+int launch_mlp(lb_op* op, const FeatArgs& F, double s, double* z) {
   if (op->mlp_fmt == LB_FMT_FP32) {
     hipLaunchKernelGGL(mlp_fp32_kernel, dim3(blocks(op->NENK)), dim3(256), 0, op->stream, F,
                        op->d_mlp_w.as<const float>(), s, z);
@@ -394,6 +451,42 @@ int mlp_apply(lb_op* op, const double* r, double* z, int g0) {
   }
   LB_HIP(hipGetLastError());
   return LB_OK;
+}
+
+int mlp_apply(lb_op* op, const double* r, double* z, int g0) {
+  if (!op->mlp) {
+    set_error("learned preconditioner: no network set (lb_op_set_preconditioner)");
+    return LB_E_NOT_READY;
+  }
+  FeatArgs F;
+  LB_TRY(feat_args(op, r, g0, &F));
+  const double s = F.inv_s > 0.0 ? 1.0 / F.inv_s : 0.0;
+  return launch_mlp(op, F, s, z);
+}
+
+// mlp_apply() without allocation or host synchronisation: the residual's norm
+// and the scales live in `scratch` (every entry is written before it is read,
+// so nothing needs clearing), and the kernels read the scales from there.
+int mlp_apply_noalloc(lb_op* op, const double* r, double* z, int g0) {
+  if (!op->mlp || !op->d_elem_h.p) {
+    set_error("lb_mlp_apply_noalloc_dev: no network set (lb_op_set_preconditioner)");
+    return LB_E_NOT_READY;
+  }
+  if (op->nmom != 16) {
+    set_error("lb_mlp_apply_noalloc_dev: the network takes 16 flux moments (P3)");
+    return LB_E_ARG;
+  }
+  static_assert(kDotPartials + 3 <= kScratchDoubles, "scratch holds the partial sums, the norm and two scales");
+  double* sp = nullptr;
+  LB_TRY(scratch_ptr(&sp));
+  const int64_t nphi = op->nmom * op->NENK;
+  LB_TRY(launch_dot(op->stream, nphi, r, r, sp, sp + kDotPartials));
+  hipLaunchKernelGGL(rms_scale_kernel, dim3(1), dim3(64), 0, op->stream, sp + kDotPartials, nphi,
+                     sp + kDotPartials + 1);
+  LB_HIP(hipGetLastError());
+  FeatArgs F = feat_base(op, r, g0);
+  F.dev_scale = sp + kDotPartials + 1;
+  return launch_mlp(op, F, 0.0, z);
 }
 
 }  // namespace lb

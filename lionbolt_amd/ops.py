@@ -272,6 +272,15 @@ class Operator:
         _lib.check(_lib.lib().lb_mlp_apply_dev(self.h, r_p, z_p, int(g0)))
         self.synchronize()
 
+    def mlp_apply_noalloc_dev(self, r, z, g0=0):
+        """``mlp_apply_dev`` through the entry point that allocates nothing (its
+        reduction scratch is a per-GPU device array; overlapping calls on one GPU race)."""
+        nphi = self.nmom * self.NENK
+        r_p, z_p = _dev(r, nphi, "r"), _dev(z, nphi, "z")
+        _torch_done()
+        _lib.check(_lib.lib().lb_mlp_apply_noalloc_dev(self.h, r_p, z_p, int(g0)))
+        self.synchronize()
+
     def mlp_features_dev(self, r, g0=0):
         r_p = _dev(r, self.nmom * self.NENK, "r")
         _torch_done()
