@@ -13,11 +13,14 @@ The line also carries:
   over the sweep time measured with HIP events on the operator's stream,
   against the 8 TB/s HBM3E peak;
 * cpu_baseline: the FP64 C++/OpenMP oracle (restatement of the reference's
-  per-direction OpenMP sweep) timed on this host on a bounded sample: one
-  group, 32 of the 256 directions, the full 384k-tet mesh;
+  per-direction OpenMP sweep) timed on this host's cores (child process,
+  OMP_PROC_BIND=spread, CPU model recorded): one full sweep of one group over
+  the 384k-tet mesh and all 256 directions, and the full-size dense and
+  moment-form scattering products, median of 3 after a warm-up;
 * solve: time to a 1e-6 relative transport residual of the full 24-group,
-  P3, downscatter problem with the first-collision source of a +z beam
-  (group-by-group within-group GMRES on flux moments, plain, FP64);
+  P3, downscatter problem with the first-collision source of a +z beam built
+  as the reference builds it (group-by-group within-group GMRES on flux
+  moments, plain, FP64), with the collided and uncollided scalar flux sums;
 * reference_method_gpu: the same problem with the reference's own method
   (psi-space GMRES(10) per group), for its iteration counts;
 * cube48k_gpu: the 48,000-tet S8 problem solved end to end (GMRES(30), 1e-8);
@@ -38,6 +41,7 @@ The line also carries:
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -51,90 +55,46 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CAG = 64
 
 
-def cpu_baseline(m, ang, ndirs=32, reps=2):
-    from lionbolt_amd import angular
-    from oracle import oracle
-    sub = angular.AngularSet(ang.k[:: ang.NI // ndirs][:ndirs], ang.w[:: ang.NI // ndirs][:ndirs], 0)
-    op = oracle.OracleOp(m, sub)
-    op.set_physics([0.2, 0.6], [[0.0], [0.0]])
-    q = np.random.default_rng(1003).random(op.n)
-    op.sweep_only(q.copy())          # warm-up
-    times = []
-    for _ in range(reps):
-        x = q.copy()
-        t0 = time.perf_counter()
-        op.sweep_only(x)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    cag = m.NE * sub.NI
-    return {"value": cag / t, "unit": "cell-angle-group sweeps/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"1 group x {sub.NI} of 256 directions x {m.NE} tets, FP64 oracle sweep, median of {reps}"}
-
-
-def cpu_solve_baseline(ref_method):
-    """The oracle's end-to-end solves on this host.
-
-    The 48,000-tet problem (S8, P0 c = 0.9, GMRES(30) at 1e-8) is run to convergence.
-    The 24-group 384,000-tet solve is projected: the reference's own method (psi-space
-    GMRES(10) with the dense direction x direction scattering kernel) at the
-    iteration counts that method takes on the GPU in this run, times the oracle's
-    full-size sweep and dense-K times measured here.  The projection counts only
-    sweeps and K applications (no Krylov vector work), so it is a lower bound."""
-    from lionbolt_amd import problems, angular
-    from oracle import oracle
-    out = {}
-    m, ang, sigt, sigs, q = problems.cube_source_48k()
-    op = oracle.OracleOp(m, ang)
-    op.set_physics(sigt, sigs)
-    t0 = time.perf_counter()
-    _, st = op.boltzmann_matinv(q, kind="gmres", m=30, rtol=1e-8)
-    out["cube48k_solve"] = {"seconds": time.perf_counter() - t0, "iterates": st["iters"], "status": st["status"],
-                         "method": "psi-space GMRES(30), dense K, tol 1e-8"}
-    # uneven beams: the oracle sweeps 4 of the 64 batch problems (all 80 directions) one
-    # after another; rate in cell-angle sweeps/s
-    probs = problems.uneven_beams()
-    s8 = angular.level_symmetric_s8()
-    tot_ca, tot_s = 0, 0.0
-    for pr in probs[:4]:
-        o = oracle.OracleOp(pr["mesh"], s8)
-        o.set_physics([pr["sigt"]], [[pr["c"] * pr["sigt"]]])
-        x = np.random.default_rng(1004).random(o.n)
-        t0 = time.perf_counter()
-        o.sweep_only(x)
-        tot_s += time.perf_counter() - t0
-        tot_ca += pr["mesh"].NE * s8.NI
-    out["uneven_beams_sweep"] = {"cell_angle_sweeps_per_s": tot_ca / tot_s,
-                         "sample": "problems 1-4 of the 64, one sweep each, all 80 directions"}
+def cpu_baseline(n, ref_method):
+    """The oracle (FP64 C++/OpenMP restatement of the reference's algorithm) on this
+    host, run as a child process (oracle/cpu_baseline.py) with OMP_PROC_BIND=spread
+    and the thread count the environment gives: the full one-group sweep of the
+    384k-tet mesh over all 256 directions and the full-size dense and moment-form
+    scattering products, median of 3 after a warm-up; the 48,000-tet problem
+    solved end to end; the batch problems' sweep rate.  The 24-group solve is
+    projected from these: the reference's own method (psi-space GMRES(10), dense
+    K) at the iteration counts that method takes on the GPU in this run, and the
+    GPU solve's algorithm (moment-form K) at its sweep count; both count sweeps and
+    K applications only (no Krylov vector work), so they are lower bounds."""
+    import subprocess
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "spread"
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), str(n)], env=env,
+                       capture_output=True, text=True, timeout=1200)
+    if r.returncode != 0:
+        return {"error": r.stderr[-2000:]}
+    c = json.loads(r.stdout.strip().splitlines()[-1])
+    sw = c["sweep"]
+    out = {"value": sw["cell_angle_sweeps_per_s"], "unit": "cell-angle-group sweeps/s", "cores": c["threads"],
+           "kind": "port", "sample": sw["sample"], "cpu_model": c["cpu_model"],
+           "cores_available": c["cores_available"], "omp_proc_bind": c["omp_proc_bind"],
+           "build_s": c["build_s"], "sweep_s": sw["seconds"], "sweep_runs_s": sw["runs"],
+           "dense_K_s": c["dense_K"]["seconds"], "moment_K_s": c["moment_K"]["seconds"],
+           "cube48k_solve": c["cube48k_solve"], "uneven_beams_sweep": c["uneven_beams_sweep"]}
     if ref_method:
-        m2 = problems.slab_cube_mesh(10)                 # dense K timed on 6,000 tets x 256 directions, scaled by NE
-        ang2 = problems.product_8x32()
-        op2 = oracle.OracleOp(m2, ang2)
-        op2.set_physics([0.3, 0.9], [[0.2, 0.1, 0.05, 0.02], [0.6, 0.3, 0.15, 0.06]])
-        x = np.random.default_rng(1003).random(op2.n)
-        op2.scatter_matvec(x)
-        t0 = time.perf_counter()
-        op2.scatter_matvec(x)
-        k_s = (time.perf_counter() - t0) * (384000 / m2.NE)
-        sweep_s = ref_method["sweep_s_full"]
         sweeps = ref_method["sweeps"]
-        proj = sweeps * (sweep_s + k_s)
         out["solve_projected"] = {
-            "seconds": proj, "sweeps_and_K": sweeps, "cpu_sweep_s": sweep_s, "cpu_dense_K_s": k_s,
-            "note": "lower bound: sweeps + dense K at the reference method's measured iteration counts"}
+            "seconds": sweeps * (sw["seconds"] + c["dense_K"]["seconds"]), "sweeps_and_K": sweeps,
+            "cpu_sweep_s": sw["seconds"], "cpu_dense_K_s": c["dense_K"]["seconds"],
+            "note": "lower bound: full-size sweep + dense K times at the reference method's measured iteration "
+                    "counts"}
         if ref_method.get("plain_sweeps"):
-            # the same algorithm as the GPU solve leg (flux-moment K, its sweep count), on the CPU
-            D, Mm = angular.moment_matrices(ang2)
-            deg = angular.moment_degree(ang2.L)
-            op2.scatter_moment_matvec(x, D, Mm, deg)
-            t0 = time.perf_counter()
-            op2.scatter_moment_matvec(x, D, Mm, deg)
-            km_s = (time.perf_counter() - t0) * (384000 / m2.NE)
             n_pl = ref_method["plain_sweeps"]
             out["solve_projected_moment_form"] = {
-                "seconds": n_pl * (sweep_s + km_s), "sweeps_and_K": n_pl, "cpu_sweep_s": sweep_s,
-                "cpu_moment_K_s": km_s,
-                "note": "lower bound: sweeps + moment-form K at the GPU solve leg's sweep count, so the "
-                        "ratio to the GPU solve is not the dense -> moment change"}
+                "seconds": n_pl * (sw["seconds"] + c["moment_K"]["seconds"]), "sweeps_and_K": n_pl,
+                "cpu_sweep_s": sw["seconds"], "cpu_moment_K_s": c["moment_K"]["seconds"],
+                "note": "lower bound: sweeps + moment-form K at the GPU solve leg's sweep count, so the ratio to "
+                        "the GPU solve is not the dense -> moment change"}
     return out
 
 
@@ -189,7 +149,7 @@ def solve_leg(m, ang, rtol, group_batch=1):
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_group_batch(group_batch)
     tune_s = settle_autotuner(op, m, ang, group_batch)
-    qmom = _beam_qmom(m, ang, sigt, S)
+    qmom, umom = _beam_qmom(op, m, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -197,12 +157,16 @@ def solve_leg(m, ang, rtol, group_batch=1):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     choice = list(op.sweep_choice(group_batch))
+    # scalar flux = sqrt(4 pi) x the l = 0 moment; group 1 adds its uncollided flux (Main.f90:443)
+    s4pi = math.sqrt(4.0 * math.pi)
+    flux = {"collided_sum_all_groups": float(phi[:, 0].sum()) * s4pi,
+            "uncollided_sum_group1": float(umom[0].sum()) * s4pi}
     op.close()
     max_res = max(s["true_residual"] for s in stats)
     return {"seconds": wall, "groups": G, "rtol": rtol, "autotune_s": tune_s, "sweep_choice": choice,
             "max_true_residual": max_res, "target_residual": 1e-6, "target_met": bool(max_res <= 1e-6),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
-            "group_batch": group_batch,
+            "group_batch": group_batch, "scalar_flux": flux,
             "method": ("plain GMRES(30) on flux moments per group, FP64" if group_batch == 1 else
                        f"plain GMRES(30) on flux moments, {group_batch} groups per coupled system, FP64")}
 
@@ -221,7 +185,7 @@ def reference_method_leg(m, ang, rtol):
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     settle_autotuner(op, m, ang)
-    qmom = _beam_qmom(m, ang, sigt, S)
+    qmom, _ = _beam_qmom(op, m, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -235,20 +199,14 @@ def reference_method_leg(m, ang, rtol):
             "method": "psi-space GMRES(10) per group (the reference's method), FP64"}
 
 
-def _beam_qmom(m, ang, sigt, S):
-    """First-collision source moments of the +z beam, per group (G x nmom x NENK)."""
-    import torch
-    from lionbolt_amd import angular, problems
-    G = sigt.shape[0]
-    phi0 = torch.from_numpy(problems.beam_uncollided(m, sigt[0])).cuda()
-    y0 = torch.from_numpy(angular.real_sph_harm(ang.L, np.array([[0.0, 0.0, 1.0]]))[:, 0]).cuda()
-    deg = torch.from_numpy(angular.moment_degree(ang.L)).cuda()
-    mat_dof = torch.from_numpy(np.repeat(m.mat - 1, 4)).cuda()
-    S_d = torch.from_numpy(S).cuda()
-    qmom = torch.empty((G, ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
-    for g in range(G):
-        qmom[g] = S_d[0, g][mat_dof][:, deg].T * y0[:, None] * phi0[None, :]
-    return qmom
+def _beam_qmom(op, m, sigt, S):
+    """Source moments of the +z beam per group (G x nmom x NENK), built as the
+    reference builds its first-collision source (ray-traced uncollided fluence
+    projected on the nodal basis, FCSSource.f90:134-215), and the moments of the
+    beam group's uncollided angular fluence (nmom x NENK)."""
+    from lionbolt_amd import problems
+    qmom, _, umom = problems.beam_group_sources(op, m, S, sigt)
+    return qmom, umom
 
 
 def cube48k_leg():
@@ -277,12 +235,13 @@ def batch_leg(reps=10, rtol=1e-6):
     import torch
     from lionbolt_amd import angular, problems
     from lionbolt_amd.ops import Operator
-    um, eoff, sigt, sigs, qmom, probs = problems.uneven_beams_batch()
+    um, eoff, sigt, sigs, probs = problems.uneven_beams_batch()
     ang = angular.level_symmetric_s8()
     ca = um.NE * ang.NI
     op = Operator(um, ang)
     op.set_physics(sigt, sigs)
     op.set_problems(eoff)
+    qmom = problems.uneven_beams_sources(op, eoff, probs, sigt, sigs)
     q = torch.rand(op.n, dtype=torch.float64, device="cuda")
     psi = torch.empty_like(q)
 
@@ -345,7 +304,7 @@ def batch_leg(reps=10, rtol=1e-6):
     del singles
     # batched solve from the first-collision sources
     op.set_sweep_mode("auto")
-    qm = torch.from_numpy(np.ascontiguousarray(qmom.reshape(-1))).cuda()
+    qm = qmom.reshape(-1).contiguous()
     phi = torch.zeros_like(qm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -377,7 +336,7 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_preconditioner(params, fmt)
     settle_autotuner(op, m, ang)
-    qmom = _beam_qmom(m, ang, sigt, S)
+    qmom, _ = _beam_qmom(op, m, sigt, S)
     res = {"training": {"rows": data["rows"], "solves": len(data["solves"]),
                         "data_seconds": data["seconds"], **tinfo}}
     for kind in ("gmres", "fgmres"):
@@ -604,7 +563,6 @@ def run(args):
     if args.ref_method:
         out["reference_method_gpu"] = reference_method_leg(m, ang, 1e-6)
         ref = {"sweeps": out["reference_method_gpu"]["sweeps"],
-               "sweep_s_full": None,
                "plain_sweeps": out["solve"]["sweeps"] if "solve" in out else None}
     if args.cube48k:
         out["cube48k_gpu"] = cube48k_leg()
@@ -616,16 +574,13 @@ def run(args):
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(m, ang)
-        if ref is not None:
-            ref["sweep_s_full"] = m.NE * ang.NI / out["cpu_baseline"]["value"]
-        out["cpu_baseline"].update(cpu_solve_baseline(ref))
+        out["cpu_baseline"] = cpu_baseline(args.n, ref)
         for key in ("solve_projected", "solve_projected_moment_form"):
             proj = out["cpu_baseline"].get(key)
             if proj and "solve" in out:
                 proj["gpu_solve_s"] = out["solve"]["seconds"]
                 proj["speedup_vs_gpu_solve"] = proj["seconds"] / out["solve"]["seconds"]
-        if "cube48k_gpu" in out:
+        if "cube48k_gpu" in out and "cube48k_solve" in out["cpu_baseline"]:
             out["cube48k_gpu"]["speedup_vs_cpu"] = out["cpu_baseline"]["cube48k_solve"]["seconds"] / out["cube48k_gpu"]["psi"]["seconds"]
     print(json.dumps(out))
 

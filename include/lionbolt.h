@@ -231,6 +231,49 @@ int lb_gmres_snapshots_dev(lb_op* op, const double* d_q, const lb_solver_opts* o
 int lb_batch_solve_dev(lb_op* op, const double* d_qmom, const lb_solver_opts* opts, double* d_phi,
                        lb_solve_stats* stats);
 
+/* ---- planar external beam (the reference's ExternalBeam / FieldGeometry,
+ * src/sources/Sources.f90:13-80) -------------------------------------------- */
+enum { LB_CUTOUT_NONE = 0, LB_CUTOUT_CIRCLE = 1, LB_CUTOUT_RECTANGLE = 2 };
+
+typedef struct {
+  double axis[3];    /* FieldGeometry%axis: unit direction of the beam's particles */
+  double origin[3];  /* FieldGeometry%origin: a point of the source plane, upstream of the mesh */
+  double e1[3];      /* FieldGeometry%e1, %e2: orthonormal beam frame normal to the axis (the */
+  double e2[3];      /* rectangle cutout is measured along them) */
+  int cutout;        /* FieldGeometry%cutout: LB_CUTOUT_* */
+  double params[4];  /* FieldGeometry%cutoutparams: circle radius params[1]; rectangle widths params[2], params[3] */
+  double n;          /* ExternalBeam%n: particles in the beam */
+} lb_beam_desc;
+
+/* The beam's uncollided fluence projected on the nodal basis, the phi0ek that
+ * FCSSource.f90:134-176 and AddAngularFluence.f90:109-151 build:
+ * d_phi0 [NENK] = IMr (sum_kq w_kq u_k(kq) phi0q(kq)) per element, phi0q =
+ * n exp(-sum_m attn[m] l_m) at the four tetrahedral quadrature points that lie in
+ * the field (InField.f90), l_m the path length in material m from the point back
+ * along the axis to the mesh surface (RayTracing.f90:149-199,
+ * FluenceQuadrature.f90:3-33; traced element to element on the device).
+ * attn [Nm] on the host: sigma_t of each material.  Synchronous. */
+int lb_beam_uncollided_dev(lb_op* op, const lb_beam_desc* beam, const double* attn, double* d_phi0);
+
+/* lb_beam_uncollided_dev on the elements [e0, e1) only (d_phi0 entries 4 e0 ..
+ * 4 e1 - 1 written).  No reference counterpart (one beam lights the whole mesh
+ * there): a batch of independent problems in consecutive element ranges, each
+ * lit by its own beam (lb_op_set_problems). */
+int lb_beam_uncollided_range_dev(lb_op* op, const lb_beam_desc* beam, const double* attn, int64_t e0, int64_t e1,
+                                 double* d_phi0);
+
+/* Flux moments of a beam of projected fluence d_phi0 [NENK] (either output may be
+ * NULL): d_fcs [nmom][NENK] = Y_lm(axis) phi0, whose discrete form with
+ * sigma_l(mat) applied, M diag(sigma_l) d_fcs, is the first-collision source of
+ * FCSSource.f90:178-215 (addition theorem); d_unc [nmom][NENK] = (D g)_lm phi0,
+ * the moments of the uncollided angular fluence phi0 g_i of
+ * AddAngularFluence.f90:153-166, g_i = sum_l (2l+1)/(4 pi) P_l(k_i . axis).
+ * Synchronous. */
+int lb_beam_moments_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_fcs, double* d_unc);
+
+/* AddAngularFluence.f90:153-166: d_unc [NI][NENK] += phi0 g_i.  Synchronous. */
+int lb_beam_angular_fluence_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_unc);
+
 /* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
 int lb_op_set_direction_block(lb_op* op, int threads);
 

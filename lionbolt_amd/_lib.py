@@ -40,6 +40,15 @@ class SolverOpts(ctypes.Structure):
                 ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("verbose", ctypes.c_int)]
 
 
+CUTOUT_NONE, CUTOUT_CIRCLE, CUTOUT_RECTANGLE = 0, 1, 2
+
+
+class BeamDesc(ctypes.Structure):
+    _fields_ = [("axis", ctypes.c_double * 3), ("origin", ctypes.c_double * 3), ("e1", ctypes.c_double * 3),
+                ("e2", ctypes.c_double * 3), ("cutout", ctypes.c_int), ("params", ctypes.c_double * 4),
+                ("n", ctypes.c_double)]
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterates", ctypes.c_int), ("matvecs", ctypes.c_int),
                 ("sweeps", ctypes.c_int), ("final_norm", ctypes.c_double), ("true_residual", ctypes.c_double),
@@ -66,6 +75,11 @@ SIGNATURES = {
     "lb_op_set_preconditioner": (ctypes.c_int, [_vp, _f, ctypes.c_int, ctypes.c_int]),
     "lb_mlp_apply_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_mlp_apply_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_beam_uncollided_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, _vp]),
+    "lb_beam_uncollided_range_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, ctypes.c_int64, ctypes.c_int64,
+                                                    _vp]),
+    "lb_beam_moments_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _vp, _vp, _vp]),
+    "lb_beam_angular_fluence_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _vp, _vp]),
     "lb_mlp_features_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_double)]),
     "lb_mlp_train_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_uint64, _f]),

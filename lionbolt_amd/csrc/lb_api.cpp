@@ -364,6 +364,47 @@ int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0) {
   });
 }
 
+int lb_beam_uncollided_dev(lb_op* op, const lb_beam_desc* beam, const double* attn, double* d_phi0) {
+  return guarded([&]() -> int {
+    if (!op) {
+      set_error("lb_beam_uncollided_dev: null operator");
+      return LB_E_ARG;
+    }
+    return beam_uncollided(op, beam, attn, 0, op->NE, d_phi0);
+  });
+}
+
+int lb_beam_uncollided_range_dev(lb_op* op, const lb_beam_desc* beam, const double* attn, int64_t e0, int64_t e1,
+                                 double* d_phi0) {
+  return guarded([&]() -> int {
+    if (!op) {
+      set_error("lb_beam_uncollided_range_dev: null operator");
+      return LB_E_ARG;
+    }
+    return beam_uncollided(op, beam, attn, e0, e1, d_phi0);
+  });
+}
+
+int lb_beam_moments_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_fcs, double* d_unc) {
+  return guarded([&]() -> int {
+    if (!op) {
+      set_error("lb_beam_moments_dev: null operator");
+      return LB_E_ARG;
+    }
+    return beam_moments(op, beam, d_phi0, d_fcs, d_unc);
+  });
+}
+
+int lb_beam_angular_fluence_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_unc) {
+  return guarded([&]() -> int {
+    if (!op) {
+      set_error("lb_beam_angular_fluence_dev: null operator");
+      return LB_E_ARG;
+    }
+    return beam_angular_fluence(op, beam, d_phi0, d_unc);
+  });
+}
+
 int lb_mlp_apply_noalloc_dev(lb_op* op, const double* d_r, double* d_z, int g0) {
   return guarded([&]() -> int {
     LB_TRY(check_ready(op, "lb_mlp_apply_noalloc_dev"));
@@ -669,7 +710,7 @@ void lb_op_destroy(lb_op* op) {
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
     b->release();
-  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat}) b->release();
+  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat, &op->d_vert, &op->w_beam}) b->release();
   for (auto& kv : op->batch.seg) {
     kv.second.d_tiles.release();
     kv.second.d_tile_off.release();

@@ -201,6 +201,9 @@ struct lb_op {
   std::map<int, lb::SweepTune> btune;  // autotuner state of batched sweeps, per group-batch size
   std::vector<int32_t> h_nbr;       // int32 [NE][4] neighbours (-1 boundary)
   std::vector<double> h_detJ;       // [NE] 6 x volume
+  std::vector<double> h_vert;       // [NE][4][3] vertex coordinates in local node order
+  lb::DevBuf d_vert;                // the same on the device (uploaded on the first beam)
+  lb::DevBuf w_beam;                // beam scratch: attenuation, failure counter, moment coefficients
   // learned right preconditioner (per cell-node MLP), see lb_mlp.hip
   bool mlp = false;
   int mlp_fmt = 1;                  // LB_FMT_BF16 (MFMA) or LB_FMT_FP32
@@ -227,6 +230,8 @@ int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int
                      int base);
 int compute_schedule(lb_op* op, const int64_t* nbr_c);
 int build_angular(lb_op* op);
+// Real orthonormal spherical harmonics Y_lm(k), l <= L, at index l^2 + l + m.
+void real_sph_harm(int L, const double* k, double* y);
 
 // lb_trace_plan.cpp
 // Ring and staging slot counts of the LDS sweep for G groups per cell.
@@ -282,6 +287,11 @@ int mlp_set(lb_op* op, const float* params, int nparams, int fmt);
 int mlp_apply(lb_op* op, const double* r, double* z, int g0);
 int mlp_apply_noalloc(lb_op* op, const double* r, double* z, int g0);
 int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale);
+
+// lb_beam.hip
+int beam_uncollided(lb_op* op, const lb_beam_desc* beam, const double* attn, int64_t e0, int64_t e1, double* d_phi0);
+int beam_moments(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_fcs, double* d_unc);
+int beam_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_unc);
 
 // lb_batch.hip
 int set_problems(lb_op* op, int P, const int64_t* eoff);

@@ -77,6 +77,11 @@ int compute_geometry(lb_op* op, const double* rg, const int64_t* node, const int
     Nm = std::max<int>(Nm, (int)m + 1);
   }
   op->Nm = Nm;
+  // element vertices in local node order, for ray tracing an external beam
+  op->h_vert.assign(NE * 12, 0.0);
+  for (int64_t e = 0; e < NE; ++e)
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) op->h_vert[e * 12 + k * 3 + c] = rg[nd[e * 4 + k] * 3 + c];
   // the neighbour relation must be symmetric
   for (int64_t e = 0; e < NE; ++e)
     for (int f = 0; f < 4; ++f) {
@@ -318,6 +323,32 @@ int compute_schedule(lb_op* op, const int64_t* nb) {
 // Angular matrices: YY (SNPrep.f90:106-133) for the dense kernel and the real
 // spherical-harmonic moment matrices D (w_i Y_lm(k_i)) and M (Y_lm(k_i)), which
 // factor it exactly: sum_m Y_lm(k) Y_lm(k') = (2l+1)/(4 pi) P_l(k.k').
+void real_sph_harm(int L, const double* k, double* y) {
+  double mu = std::max(-1.0, std::min(1.0, k[2]));
+  double s = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+  double phi = std::atan2(k[1], k[0]);
+  // associated Legendre P_l^m(mu) without the Condon-Shortley phase, P[l*(L+1)+m]
+  std::vector<double> P((L + 1) * (L + 1), 0.0);
+  P[0] = 1.0;
+  for (int m = 1; m <= L; ++m) P[m * (L + 1) + m] = (2 * m - 1) * s * P[(m - 1) * (L + 1) + m - 1];
+  for (int m = 0; m < L; ++m) P[(m + 1) * (L + 1) + m] = (2 * m + 1) * mu * P[m * (L + 1) + m];
+  for (int m = 0; m <= L; ++m)
+    for (int l = m + 2; l <= L; ++l)
+      P[l * (L + 1) + m] = ((2 * l - 1) * mu * P[(l - 1) * (L + 1) + m] - (l + m - 1) * P[(l - 2) * (L + 1) + m]) / (l - m);
+  for (int l = 0; l <= L; ++l)
+    for (int m = -l; m <= l; ++m) {
+      int am = std::abs(m);
+      double ratio = 1.0;  // (l-|m|)! / (l+|m|)!
+      for (int t = l - am + 1; t <= l + am; ++t) ratio /= t;
+      double nrm = std::sqrt((2 * l + 1) / (4.0 * kPi) * ratio);
+      double v;
+      if (m == 0) v = nrm * P[l * (L + 1)];
+      else if (m > 0) v = std::sqrt(2.0) * nrm * P[l * (L + 1) + am] * std::cos(am * phi);
+      else v = std::sqrt(2.0) * nrm * P[l * (L + 1) + am] * std::sin(am * phi);
+      y[l * l + l + m] = v;
+    }
+}
+
 int build_angular(lb_op* op) {
   const int64_t NI = op->NI;
   const int L = op->L;
@@ -340,35 +371,15 @@ int build_angular(lb_op* op) {
   op->h_D.assign((size_t)nm * NI, 0.0);
   op->h_M.assign((size_t)NI * nm, 0.0);
   op->h_deg.assign(nm, 0);
-  std::vector<double> P((L + 1) * (L + 1));
+  std::vector<double> y(nm);
+  for (int l = 0; l <= L; ++l)
+    for (int m = -l; m <= l; ++m) op->h_deg[l * l + l + m] = l;
   for (int64_t i = 0; i < NI; ++i) {
-    const double* k = &op->h_k[i * 3];
-    double mu = std::max(-1.0, std::min(1.0, k[2]));
-    double s = std::sqrt(std::max(0.0, 1.0 - mu * mu));
-    double phi = std::atan2(k[1], k[0]);
-    // associated Legendre P_l^m(mu) without the Condon-Shortley phase, P[l*(L+1)+m]
-    std::fill(P.begin(), P.end(), 0.0);
-    P[0] = 1.0;
-    for (int m = 1; m <= L; ++m) P[m * (L + 1) + m] = (2 * m - 1) * s * P[(m - 1) * (L + 1) + m - 1];
-    for (int m = 0; m < L; ++m) P[(m + 1) * (L + 1) + m] = (2 * m + 1) * mu * P[m * (L + 1) + m];
-    for (int m = 0; m <= L; ++m)
-      for (int l = m + 2; l <= L; ++l)
-        P[l * (L + 1) + m] = ((2 * l - 1) * mu * P[(l - 1) * (L + 1) + m] - (l + m - 1) * P[(l - 2) * (L + 1) + m]) / (l - m);
-    for (int l = 0; l <= L; ++l)
-      for (int m = -l; m <= l; ++m) {
-        int am = std::abs(m);
-        double ratio = 1.0;  // (l-|m|)! / (l+|m|)!
-        for (int t = l - am + 1; t <= l + am; ++t) ratio /= t;
-        double nrm = std::sqrt((2 * l + 1) / (4.0 * kPi) * ratio);
-        double y;
-        if (m == 0) y = nrm * P[l * (L + 1)];
-        else if (m > 0) y = std::sqrt(2.0) * nrm * P[l * (L + 1) + am] * std::cos(am * phi);
-        else y = std::sqrt(2.0) * nrm * P[l * (L + 1) + am] * std::sin(am * phi);
-        int r = l * l + l + m;
-        op->h_deg[r] = l;
-        op->h_M[i * nm + r] = y;
-        op->h_D[(size_t)r * NI + i] = op->h_w[i] * y;
-      }
+    real_sph_harm(L, &op->h_k[i * 3], y.data());
+    for (int r = 0; r < nm; ++r) {
+      op->h_M[i * nm + r] = y[r];
+      op->h_D[(size_t)r * NI + i] = op->h_w[i] * y[r];
+    }
   }
   LB_TRY(upload(op, op->d_D, op->h_D.data(), op->h_D.size() * sizeof(double)));
   LB_TRY(upload(op, op->d_M, op->h_M.data(), op->h_M.size() * sizeof(double)));

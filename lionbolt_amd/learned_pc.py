@@ -60,17 +60,18 @@ def forward_reference(params, X):
     return h @ p["W4"] + p["b4"]
 
 
-def _source_moments(m, ang, sigt, sigs, kind):
-    """Single-group source moments (nmom, NENK) of a training problem."""
-    mat = np.repeat(m.mat - 1, 4)
-    deg = angular.moment_degree(ang.L)
+def _source_moments(op, m, ang, sigt, sigs, kind):
+    """Single-group source moments (nmom * NENK,) of a training problem (cuda):
+    the +z beam's first-collision source, or psi = 1 in every direction."""
+    import torch
     if kind == "beam":
-        phi0 = problems.beam_uncollided(m, sigt)
-        y0 = angular.real_sph_harm(ang.L, np.array([[0.0, 0.0, 1.0]]))[:, 0]
-        return sigs[mat][:, deg].T * y0[:, None] * phi0[None, :]
-    q = np.zeros((ang.n_moments, m.NENK))
-    q[0] = math.sqrt(4.0 * math.pi)          # psi = 1 in every direction
-    return q
+        S = np.zeros((1, 1) + sigs.shape)
+        S[0, 0] = sigs
+        q, _, _ = problems.beam_group_sources(op, m, S, np.atleast_2d(sigt))
+        return q[0].reshape(-1).contiguous()
+    q = torch.zeros((ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
+    q[0] = math.sqrt(4.0 * math.pi)
+    return q.reshape(-1)
 
 
 def collect(op, qmom, m=30, rtol=1e-10, max_snaps=80, rows_per_snap=512, seed=0):
@@ -116,8 +117,8 @@ def build_training_set(nmesh=32, seed=2000, rows_per_snap=512, L=3, log=None):
         op = Operator(m, ang)
         op.set_physics(pr["sigt"], pr["sigs"])
         for j, kind in enumerate(("beam", "uniform")):
-            qm = torch.from_numpy(np.ascontiguousarray(_source_moments(m, ang, pr["sigt"], pr["sigs"], kind)))
-            X, Y, st, cnt = collect(op, qm.reshape(-1).cuda(), rows_per_snap=rows_per_snap, seed=seed + 2 * p + j)
+            qm = _source_moments(op, m, ang, pr["sigt"], pr["sigs"], kind)
+            X, Y, st, cnt = collect(op, qm, rows_per_snap=rows_per_snap, seed=seed + 2 * p + j)
             Xs += X
             Ys += Y
             info.append({"problem": p, "source": kind, "tets": m.NE, "iterates": st["iterates"], "snapshots": cnt})

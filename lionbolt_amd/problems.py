@@ -12,22 +12,25 @@ Decisions recorded here (the reference never ran these configurations):
   sigma_0(g' -> g) for g >= g' follow 0.5^(g - g') normalised so that the
   out-scatter of g' is 0.85 sigma_t,g' (the "row sum" read as out-scatter);
   sigma_l = 0.6^l sigma_0.
-* first-collision beam: a unit mono-directional beam along +z entering the face
-  z = 0 in group 1.  Its uncollided flux at a node is exp(-tau(z)) with the
-  optical depth of the planar slab layout; the first-collision source of group g
-  is sum_l (2l+1)/(4 pi) sigma_l(1 -> g) P_l(k . z) phi_0, i.e. moments
-  sigma_l(1 -> g) Y_lm(z) phi_0 (nodal interpolation, where the reference
-  projects a ray-traced source with the inverse mass matrix, FCSSource.f90:134-215).
+* first-collision beam: a unit mono-directional planar beam along +z entering
+  the face z = 0 in group 1 (source plane z = -1, no cutout), built as the
+  reference builds it (beam.py over lb_beam_*: the uncollided fluence ray-traced
+  to the tetrahedral quadrature points and projected with IMr,
+  FCSSource.f90:134-176).  Group 1's source is its first-collision source
+  sigma_l(1 -> 1) Y_lm(z) phi_0 (FCSSource.f90:178-215); group g > 1 receives
+  sigma_l(1 -> g) times the moments of group 1's uncollided angular fluence, which
+  the reference adds to the solution (Main.f90:443) before the energy coupling
+  applies K(1 -> g) to it (Main.f90:584-629).
 * uneven_beams: 64 single-group problems (SplitMix64 stream 1004, six values per
   problem): tets drawn uniform on [2,000, 60,000] and quantised to the jittered
   cube 6 n^3 with n = round((N/6)^(1/3)) clipped to [7, 21]; edge length
   U[5, 20] cm; sigma_t U[0.05, 1.0] /cm; c = sigma_s/sigma_t U[0.5, 0.95]; P0;
-  LQ S8; a broad collimated beam of unit flux along
+  LQ S8; a broad collimated planar beam of unit flux along
   k0 = (sin t cos f, sin t sin f, cos t) with cos t U[0.5, 1.0] and f U[0, 2 pi)
-  entering every face it illuminates; jitter stream 1001 + 7919 (p + 1).  Its
-  uncollided flux is exp(-sigma_t l) with l the distance from the node back along
-  -k0 to the cube surface; the first-collision source is isotropic,
-  sigma_s phi_0 / (4 pi).
+  entering every face it illuminates (no cutout, source plane 2a upstream of the
+  cube's centre); jitter stream 1001 + 7919 (p + 1).  Its first-collision source
+  is built as the reference builds it (beam.py); with P0 scattering it is
+  isotropic, sigma_s Y_00 phi_0 in moments.
 * scattering_dominated_xs: the 384,000-tet mesh, directions, groups and sigma_t with a within-group
   scattering ratio sigma_s0(g -> g) / sigma_t,g = 0.99 in every group (the
   scattering-dominated regime the configuration asks for) and a weak downscatter
@@ -90,39 +93,31 @@ def within_group(S):
     return np.stack([S[g, g] for g in range(G)])
 
 
-def beam_uncollided(m, sigt_group0, a=10.0, slab=(4.0, 6.0)):
-    """exp(-tau) at every spatial dof for the +z beam (planar slab layout)."""
-    z = m.dof_coords()[:, 2]
-    thick = np.clip(z - slab[0], 0.0, slab[1] - slab[0])
-    tau = sigt_group0[0] * (z - thick) + sigt_group0[1] * thick
-    return np.exp(-tau)
+def plus_z_beam(a=10.0):
+    """The unit planar beam along +z entering the face z = 0 of the cube [0, a]^3."""
+    from .beam import ExternalBeam, FieldGeometry
+    return ExternalBeam(FieldGeometry([0.0, 0.0, 1.0], [0.5 * a, 0.5 * a, -1.0]), n=1.0)
 
 
-def beam_source_moments(m, ang, S, sigt):
-    """First-collision source moments qmom (G, nmom, NENK) of the +z beam."""
+def beam_group_sources(op, m, S, sigt, beam=None):
+    """Source moments (G, nmom, NENK) of a beam in group 1 (cuda float64), with its
+    projected uncollided fluence phi0 (NENK) and the moments of its uncollided
+    angular fluence (nmom, NENK): group 1 gets its first-collision source, group
+    g > 1 the scattering of the uncollided fluence into g (see the module docstring).
+    ``op`` is any operator on ``m`` with the quadrature of the solve."""
+    import torch
+    from . import angular
+    beam = plus_z_beam() if beam is None else beam
+    phi0 = beam.uncollided(op, sigt[0])
+    fcs, umom = beam.moments(op, phi0)
     G = S.shape[0]
-    L = ang.L
-    phi0 = beam_uncollided(m, sigt[0])
-    y0 = angular.real_sph_harm(L, np.array([[0.0, 0.0, 1.0]]))[:, 0]      # Y_lm(k0)
-    deg = angular.moment_degree(L)
-    mat_dof = np.repeat(m.mat - 1, 4)
-    q = np.empty((G, (L + 1) ** 2, m.NENK))
+    deg = torch.from_numpy(angular.moment_degree(op.ang.L)).cuda()
+    mat_dof = torch.from_numpy(np.repeat(m.mat - 1, 4)).cuda()
+    S0 = torch.from_numpy(np.ascontiguousarray(S[0])).cuda()          # (G, Nm, L+1) for 1 -> g
+    q = torch.empty((G, op.nmom, m.NENK), dtype=torch.float64, device="cuda")
     for g in range(G):
-        sig = S[0, g][mat_dof][:, deg].T            # (nmom, NENK)
-        q[g] = sig * y0[:, None] * phi0[None, :]
-    return q, phi0
-
-
-def cube_beam_uncollided(m, a, k0, sigt):
-    """exp(-sigma_t l) at every spatial dof of a homogeneous cube [0, a]^3 lit by a
-    broad parallel beam along k0: l = distance from the node back along -k0 to the
-    cube surface."""
-    x = m.dof_coords()
-    k0 = np.asarray(k0, dtype=np.float64)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        back = np.where(k0 > 0, x / k0, np.where(k0 < 0, (a - x) / -k0, np.inf))
-    ell = np.clip(back.min(axis=1), 0.0, None)
-    return np.exp(-sigt * ell)
+        q[g] = S0[g][mat_dof][:, deg].T * (fcs if g == 0 else umom)
+    return q, phi0, umom
 
 
 def uneven_beams(nprob=64, seed=1004):
@@ -146,16 +141,28 @@ def uneven_beams(nprob=64, seed=1004):
 
 def uneven_beams_batch(nprob=64, seed=1004, probs=None):
     """The uneven-beam problems as one batch: union mesh, element offsets, per-problem
-    (material) sigma_t and P0 sigma_s, and the first-collision source moments
-    qmom (1, NENK) of the union (P0: q = Y00 sigma_s phi_0, so M qmom = sigma_s phi_0/(4 pi))."""
+    (material) sigma_t and P0 sigma_s, and the problems."""
     probs = uneven_beams(nprob, seed) if probs is None else probs
     um, eoff = mesh.union([p["mesh"] for p in probs])
     sigt = np.array([p["sigt"] for p in probs])
     sigs = np.array([[p["c"] * p["sigt"]] for p in probs])
-    y00 = 1.0 / math.sqrt(4.0 * math.pi)
-    qmom = np.concatenate([y00 * p["c"] * p["sigt"] * cube_beam_uncollided(p["mesh"], p["a"], p["k0"], p["sigt"])
-                           for p in probs])[None, :]
-    return um, eoff, sigt, sigs, qmom, probs
+    return um, eoff, sigt, sigs, probs
+
+
+def uneven_beams_sources(op, eoff, probs, sigt, sigs):
+    """First-collision source moments (1, NENK) of the batch (cuda): each problem's
+    own beam traced through the union mesh (rays never leave a problem: no face
+    joins two), its projected fluence kept on that problem's elements; P0, so the
+    source is sigma_s Y_00 phi_0."""
+    import torch
+    from .beam import ExternalBeam, FieldGeometry
+    phi0 = torch.zeros(op.NENK, dtype=torch.float64, device="cuda")
+    for p, pr in enumerate(probs):
+        c = 0.5 * pr["a"] * np.ones(3)
+        beam = ExternalBeam(FieldGeometry(pr["k0"], c - 2.0 * pr["a"] * pr["k0"]), n=1.0)
+        beam.uncollided(op, sigt, elems=(int(eoff[p]), int(eoff[p + 1])), out=phi0)
+    sig_dof = torch.from_numpy(np.repeat(sigs[:, 0][np.asarray(op.mesh.mat) - 1], 4)).cuda()
+    return ((1.0 / math.sqrt(4.0 * math.pi)) * sig_dof * phi0)[None, :]
 
 
 def scattering_dominated_xs(G=24, L=3, within=0.99, down=0.005):

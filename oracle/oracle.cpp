@@ -189,6 +189,8 @@ struct OrcOp {
   std::vector<int32_t> kdeg;
   // workspaces
   std::vector<double> xw, sxw;
+  // inverse reference mass (FiniteElementAnalysis.f90:167-181), row-major
+  double IMr[16] = {0};
 };
 
 extern "C" {
@@ -225,6 +227,7 @@ static void fea(OrcOp* op) {
     }
   }
   invert(4, IMr);  // dgetrf/dgetri, FiniteElementAnalysis.f90:171-180
+  std::memcpy(op->IMr, IMr, sizeof(IMr));
   // FEAFaceMassRef(3) (FEAInnerProducts.f90:263-283)
   double IFMr[3][3] = {{0}};
   for (int ii = 0; ii < Nq; ++ii) {
@@ -1043,6 +1046,299 @@ int orc_boltzmann_matinv(void* h, double* s, int kind, int m, double rtol, doubl
   }
   if (nmatvec) *nmatvec = 0;
   return orc_source_iteration(h, s, rtol, atol, maxit, hist, hist_cap, iters);
+}
+
+// ---------------------------------------------------------------------------
+// Planar external beam: the uncollided fluence at tetrahedral quadrature points
+// (init_ExternalBeam.f90:62-95, InField.f90, RayTracing.f90:24-205,
+// PrepFluence.f90:42-58, FluenceQuadrature.f90:3-33), its projection on the
+// nodal basis and the first-collision source (FCSSource.f90:134-215), and the
+// uncollided angular fluence (AddAngularFluence.f90:109-166).
+// ---------------------------------------------------------------------------
+
+// core::PrepTetrahedralQuadrature (NittanyPhysics, not vendored): restated as
+// the symmetric 4-point rule exact for quadratics, barycentric points
+// (a, b, b, b) and their permutations with a = (5 + 3 sqrt 5) / 20 and
+// b = (5 - sqrt 5) / 20, weight 1/24 each on the reference tetrahedron of volume
+// 1/6; point kq is the one nearest node kq.  Its points are parity unpinned.
+static void tet_quadrature(double cq[4][3], double wq[4]) {
+  const double a = (5.0 + 3.0 * std::sqrt(5.0)) / 20.0, b = (5.0 - std::sqrt(5.0)) / 20.0;
+  for (int kq = 0; kq < 4; ++kq) {
+    for (int c = 0; c < 3; ++c) cq[kq][c] = (kq == c + 1) ? a : b;   // (xi, eta, zeta) = weights of nodes 2..4
+    wq[kq] = 1.0 / 24.0;
+  }
+}
+
+struct OrcBeam {
+  double axis[3], origin[3], e1[3], e2[3];
+  int cutout;        // 0 none, 1 circle, 2 rectangle
+  double params[4];  // FieldGeometry%cutoutparams
+  double n;          // particles in the beam
+};
+
+// InField.f90:3-62, planar beams.
+static bool in_field(const OrcBeam& B, const double* r) {
+  const double TOL = 1.0e-12;
+  if (B.cutout == 0) return true;
+  double u[3], v[3];
+  for (int c = 0; c < 3; ++c) u[c] = r[c] - B.origin[c];
+  const double ua = dot3(u, B.axis);
+  for (int c = 0; c < 3; ++c) v[c] = u[c] - ua * B.axis[c];
+  if (B.cutout == 1) return norm2_scaled(v, 3) <= B.params[1] + TOL;
+  if (!(std::fabs(dot3(v, B.e1)) <= 0.5 * B.params[2] + TOL)) return false;
+  return std::fabs(dot3(v, B.e2)) <= 0.5 * B.params[3] + TOL;
+}
+
+// A region for ray tracing: a connected set of elements of one material
+// (ReadGMSH.f90:259-312 joins mesh blocks of equal material that touch) and the
+// faces enclosing it, each as (the first element sharing it, its local face)
+// (PostProcess.f90:106-130; one material: every boundary face, ReadGMSH.f90:261-265).
+struct OrcRegion {
+  int mat;
+  std::vector<std::pair<i64, int>> faces;
+};
+
+static void build_regions(OrcOp* op, std::vector<OrcRegion>& reg) {
+  const i64 NE = op->NE;
+  reg.clear();
+  if (op->Nm == 1) {
+    reg.push_back({1, {}});
+    for (i64 e = 0; e < NE; ++e)
+      for (int f = 0; f < 4; ++f)
+        if (op->nbr[e * 4 + f] < 0) reg[0].faces.push_back({e, f});
+    return;
+  }
+  std::vector<i64> rid(NE, -1);
+  for (i64 s = 0; s < NE; ++s) {
+    if (rid[s] >= 0) continue;
+    const i64 r = (i64)reg.size();
+    reg.push_back({(int)op->mat[s], {}});
+    std::vector<i64> stack{s};
+    rid[s] = r;
+    while (!stack.empty()) {
+      const i64 e = stack.back();
+      stack.pop_back();
+      for (int f = 0; f < 4; ++f) {
+        const i64 ep = op->nbr[e * 4 + f];
+        if (ep >= 0 && rid[ep] < 0 && op->mat[ep] == op->mat[e]) {
+          rid[ep] = r;
+          stack.push_back(ep);
+        }
+      }
+    }
+  }
+  for (i64 e = 0; e < NE; ++e)
+    for (int f = 0; f < 4; ++f) {
+      const i64 ep = op->nbr[e * 4 + f];
+      if (ep < 0) {
+        reg[rid[e]].faces.push_back({e, f});
+      } else if (ep > e && rid[ep] != rid[e]) {
+        reg[rid[e]].faces.push_back({e, f});
+        reg[rid[ep]].faces.push_back({e, f});
+      }
+    }
+}
+
+// TraceVector (RayTracing.f90:207-369): path length in each material of the
+// segment from R0 to R0 + r, from its intersections (Moller-Trumbore with the
+// reference's tolerances) with the faces enclosing each region.  Returns false
+// when a one-material trace finds no entry face (the reference stops there).
+static bool trace_vector(const OrcOp* op, const std::vector<OrcRegion>& reg, const double* R0, const double* r,
+                         double* pathlen) {
+  const double TOL = 1.0e-8;
+  const int Nm = op->Nm;
+  for (int m = 0; m < Nm; ++m) pathlen[m] = 0.0;
+  const double length = norm2_scaled(r, 3);
+  bool success = false;
+  double t = 0.0;
+  std::vector<double> llist;
+  for (const OrcRegion& R : reg) {
+    const int mat = Nm > 1 ? R.mat : 1;
+    llist.clear();
+    for (const auto& ef : R.faces) {
+      const i64 e = ef.first;
+      const int f = ef.second;
+      i64 fn[3] = {op->node[e * 4 + F2K[f][0]], op->node[e * 4 + F2K[f][1]], op->node[e * 4 + F2K[f][2]]};
+      std::sort(fn, fn + 3);
+      const double* v0 = &op->rg[fn[0] * 3];
+      const double* v1 = &op->rg[fn[1] * 3];
+      const double* v2 = &op->rg[fn[2] * 3];
+      double edge1[3], edge2[3], nref[3], pvec[3], tvec[3], qvec[3];
+      for (int c = 0; c < 3; ++c) {
+        edge1[c] = v1[c] - v0[c];
+        edge2[c] = v2[c] - v0[c];
+      }
+      const double* n = &op->nrm[e * 12 + f * 3];
+      double det = dot3(n, r);
+      if (Nm == 1) {
+        if (det > -TOL) continue;
+      } else if (std::fabs(det) < TOL) {
+        continue;
+      }
+      cross(edge1, edge2, nref);
+      if (dot3(n, nref) < 0.0)
+        for (int c = 0; c < 3; ++c) std::swap(edge1[c], edge2[c]);
+      cross(r, edge2, pvec);
+      det = dot3(edge1, pvec);
+      if (std::fabs(det) < TOL) continue;
+      for (int c = 0; c < 3; ++c) tvec[c] = R0[c] - v0[c];
+      const double u = dot3(tvec, pvec);
+      cross(tvec, edge1, qvec);
+      const double v = dot3(r, qvec);
+      if (det > 0.0) {
+        if (u < -TOL || u > det + TOL) continue;
+        if (v < -TOL || u + v > det + TOL) continue;
+      } else {
+        if (u > TOL || u < det - TOL) continue;
+        if (v > TOL || u + v < det - TOL) continue;
+      }
+      t = dot3(edge2, qvec) / det;
+      if (Nm == 1) {
+        success = true;
+        break;
+      }
+      if (t > 1.0) continue;
+      llist.push_back(t);
+    }
+    if (Nm == 1) break;
+    if (llist.empty() || (llist.size() == 1 && std::fabs(llist[0]) < 1.0e-12)) continue;
+    std::sort(llist.begin(), llist.end());
+    size_t nkeep = 1;
+    for (size_t i = 1; i < llist.size(); ++i)
+      if (std::fabs(llist[i] - llist[nkeep - 1]) > 1.0e-12) llist[nkeep++] = llist[i];
+    llist.resize(nkeep);
+    for (size_t il = 0; il + 1 < nkeep; il += 2) pathlen[mat - 1] += llist[il + 1] - llist[il];
+    if (nkeep % 2 == 1) pathlen[mat - 1] += 1.0 - llist[nkeep - 1];
+  }
+  if (Nm == 1) {
+    if (!success) return false;
+    pathlen[0] = (1.0 - t) * length;
+  } else {
+    for (int m = 0; m < Nm; ++m) pathlen[m] *= length;
+  }
+  return true;
+}
+
+// phi0ek [NENK]: the uncollided fluence n exp(-sum_m attn_m l_m) at the
+// quadrature points in the beam (rays from the back-projection of each point on
+// the source plane, RayTracing.f90:149-199), integrated against the shape
+// functions and multiplied by IMr (FCSSource.f90:134-176, identical in
+// AddAngularFluence.f90:109-151).  quadmask [NENK] (may be NULL) receives the
+// in-beam flag of each quadrature point.  Returns -1 if a ray found no entry.
+int orc_beam_uncollided(void* h, const double* axis, const double* origin, const double* e1, const double* e2,
+                        int cutout, const double* params, double n, const double* attn, double* phi0ek,
+                        int32_t* quadmask) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  OrcBeam B;
+  for (int c = 0; c < 3; ++c) {
+    B.axis[c] = axis[c];
+    B.origin[c] = origin[c];
+    B.e1[c] = e1[c];
+    B.e2[c] = e2[c];
+  }
+  B.cutout = cutout;
+  for (int c = 0; c < 4; ++c) B.params[c] = params ? params[c] : 0.0;
+  B.n = n;
+  std::vector<OrcRegion> reg;
+  build_regions(op, reg);
+  double cq[4][3], wq[4], u[4][4];
+  tet_quadrature(cq, wq);
+  for (int kq = 0; kq < 4; ++kq)
+    for (int k = 0; k < 4; ++k) u[k][kq] = tet_sf(k, cq[kq]);
+  const i64 NE = op->NE;
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : failed)
+  for (i64 e = 0; e < NE; ++e) {
+    double re[4][3], v[4][3], phi0q[4];
+    bool mask[4];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) re[k][c] = op->rg[op->node[e * 4 + k] * 3 + c];
+    for (int kq = 0; kq < 4; ++kq)   // MapToTetrahedralQuad
+      for (int c = 0; c < 3; ++c) {
+        double x = 0.0;
+        for (int k = 0; k < 4; ++k) x += tet_sf(k, cq[kq]) * re[k][c];
+        v[kq][c] = x;
+      }
+    std::vector<double> pl(op->Nm);
+    for (int kq = 0; kq < 4; ++kq) {
+      mask[kq] = in_field(B, v[kq]);
+      phi0q[kq] = 0.0;
+      if (!mask[kq]) continue;
+      double r[3], Rp[3];
+      for (int c = 0; c < 3; ++c) r[c] = v[kq][c] - B.origin[c];
+      const double a1 = dot3(r, B.e1), a2 = dot3(r, B.e2);
+      for (int c = 0; c < 3; ++c) Rp[c] = B.origin[c] + a1 * B.e1[c] + a2 * B.e2[c];
+      for (int c = 0; c < 3; ++c) r[c] = v[kq][c] - Rp[c];
+      if (!trace_vector(op, reg, Rp, r, pl.data())) {
+        failed = 1;
+        continue;
+      }
+      double tau = 0.0;   // DOT_PRODUCT(attn, pathlen)
+      for (int m = 0; m < op->Nm; ++m) tau += attn[m] * pl[m];
+      phi0q[kq] = B.n * (1.0 * std::exp(-tau));
+    }
+    double acc[4];
+    for (int k = 0; k < 4; ++k) {
+      acc[k] = 0.0;
+      for (int kq = 0; kq < 4; ++kq)
+        if (mask[kq]) acc[k] = acc[k] + wq[kq] * u[k][kq] * phi0q[kq];
+    }
+    for (int k = 0; k < 4; ++k) {
+      double y = 0.0;
+      for (int j = 0; j < 4; ++j) y += op->IMr[k * 4 + j] * acc[j];
+      phi0ek[4 * e + k] = y;
+    }
+    if (quadmask)
+      for (int kq = 0; kq < 4; ++kq) quadmask[4 * e + kq] = mask[kq] ? 1 : 0;
+  }
+  return failed ? -1 : 0;
+}
+
+// sum_l (2l+1) P_l(x) c_l / (4 pi), P_l by the three-term recurrence (the
+// reference evaluates core's Legendre coefficients by Horner's rule).
+static double legendre_series(int L, double x, const double* c) {
+  double p0 = 1.0, p1 = x, f = 0.0;
+  for (int l = 0; l <= L; ++l) {
+    double P;
+    if (l == 0) P = 1.0;
+    else if (l == 1) P = x;
+    else {
+      P = ((2 * l - 1) * x * p1 - (l - 1) * p0) / l;
+      p0 = p1;
+      p1 = P;
+    }
+    f += (2 * l + 1) * P * c[l] / (4.0 * PI);
+  }
+  return f;
+}
+
+// FCSSource.f90:178-215 (planar): s(sdof + i NENK) += phi0ek(sdof) angfac(i, m),
+// angfac = sum_l (2l+1) P_l(k_i . k0) sigma_l(m) / (4 pi) with the operator's physics.
+void orc_beam_source(void* h, const double* axis, const double* phi0ek, double* s) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  const i64 NENK = op->NENK;
+  const int L1 = op->L + 1;
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < op->NI; ++i) {
+    const double kk0 = dot3(&op->kdir[i * 3], axis);
+    for (int m = 0; m < op->Nm; ++m) {
+      const double angfac = legendre_series(op->L, kk0, &op->sigs[m * L1]);
+      for (i64 sd : op->mat2sd[m]) s[sd + i * NENK] = s[sd + i * NENK] + phi0ek[sd] * angfac;
+    }
+  }
+}
+
+// AddAngularFluence.f90:153-166 with GenAngularDistribution (Sources.f90:257-276):
+// unc(sdof + i NENK) += phi0ek(sdof) sum_l (2l+1) P_l(k_i . k0) / (4 pi).
+void orc_beam_angular_fluence(void* h, const double* axis, const double* phi0ek, double* unc) {
+  OrcOp* op = static_cast<OrcOp*>(h);
+  const i64 NENK = op->NENK;
+  std::vector<double> one(op->L + 1, 1.0);
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < op->NI; ++i) {
+    const double angfac = legendre_series(op->L, dot3(&op->kdir[i * 3], axis), one.data());
+    for (i64 sd = 0; sd < NENK; ++sd) unc[sd + i * NENK] = unc[sd + i * NENK] + phi0ek[sd] * angfac;
+  }
 }
 
 // SpaceAngleVector%Fluence (LinearOperatorClass.f90:154-174).

@@ -56,6 +56,11 @@ def lib():
         L.orc_scatter_moment_matvec.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int, _d]
         L.orc_use_moment_kernel.argtypes = [ctypes.c_void_p, _d, _d, _i32, ctypes.c_int]
         L.orc_fluence.argtypes = [ctypes.c_void_p, _d, _d]
+        L.orc_beam_uncollided.restype = ctypes.c_int
+        L.orc_beam_uncollided.argtypes = [ctypes.c_void_p, _d, _d, _d, _d, ctypes.c_int, _d, ctypes.c_double, _d, _d,
+                                          _i32]
+        L.orc_beam_source.argtypes = [ctypes.c_void_p, _d, _d, _d]
+        L.orc_beam_angular_fluence.argtypes = [ctypes.c_void_p, _d, _d, _d]
         L.orc_norm2.restype = ctypes.c_double
         L.orc_norm2.argtypes = [_d, ctypes.c_int64]
         L.orc_num_threads.restype = ctypes.c_int
@@ -194,6 +199,51 @@ class OracleOp:
         out = np.empty(self.NENK)
         lib().orc_fluence(self.h, _p(v), _p(out))
         return out
+
+    # -- planar external beam (the reference's ExternalBeam, planar branches) ----
+    @staticmethod
+    def beam_frame(axis):
+        """An orthonormal pair spanning the plane normal to the beam axis."""
+        a = np.asarray(axis, dtype=np.float64)
+        t = np.array([1.0, 0.0, 0.0]) if abs(a[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        e1 = t - a.dot(t) * a
+        e1 /= np.linalg.norm(e1)
+        return e1, np.cross(a, e1)
+
+    def beam_uncollided(self, axis, origin, attn, cutout="none", params=None, n=1.0, frame=None):
+        """phi0ek (NENK,): the projected uncollided fluence, and the in-beam mask of
+        the quadrature points.  ``attn``: sigma_t per material."""
+        axis = np.ascontiguousarray(axis, dtype=np.float64)
+        origin = np.ascontiguousarray(origin, dtype=np.float64)
+        e1, e2 = self.beam_frame(axis) if frame is None else frame
+        e1, e2 = np.ascontiguousarray(e1, dtype=np.float64), np.ascontiguousarray(e2, dtype=np.float64)
+        prm = np.zeros(4) if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        attn = np.ascontiguousarray(np.atleast_1d(attn), dtype=np.float64)
+        assert attn.size == self.Nm
+        out = np.empty(self.NENK)
+        mask = np.empty(self.NENK, dtype=np.int32)
+        cut = {"none": 0, "circle": 1, "rectangle": 2}[cutout]
+        rc = lib().orc_beam_uncollided(self.h, _p(axis), _p(origin), _p(e1), _p(e2), cut, _p(prm), float(n),
+                                       _p(attn), _p(out), _p(mask, _i32))
+        if rc != 0:
+            raise RuntimeError("RayTracing.f90: TraceVector: While ray tracing no intersections were found.")
+        return out, mask.astype(bool)
+
+    def beam_source(self, axis, phi0ek, s):
+        """FCSSource (planar): s + phi0ek x sum_l (2l+1)/(4 pi) P_l(k.k0) sigma_l(mat)."""
+        x = self._vec(s)
+        axis = np.ascontiguousarray(axis, dtype=np.float64)
+        phi0ek = np.ascontiguousarray(phi0ek, dtype=np.float64)
+        lib().orc_beam_source(self.h, _p(axis), _p(phi0ek), _p(x))
+        return x
+
+    def beam_angular_fluence(self, axis, phi0ek):
+        """AddAngularFluence (planar): the uncollided angular fluence."""
+        unc = np.zeros(self.n)
+        axis = np.ascontiguousarray(axis, dtype=np.float64)
+        phi0ek = np.ascontiguousarray(phi0ek, dtype=np.float64)
+        lib().orc_beam_angular_fluence(self.h, _p(axis), _p(phi0ek), _p(unc))
+        return unc
 
 
 def norm2(x):

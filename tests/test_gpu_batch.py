@@ -35,7 +35,7 @@ def batch_op(um, eoff, sigt, sigs, ang):
 @pytest.mark.parametrize("block,per_cu", [(256, 4), (256, 8), (512, 2), (1024, 1), (1024, 2)])
 def test_persistent_sweep_equals_level_launches(block, per_cu):
     import torch
-    um, eoff, sigt, sigs, qmom, probs = small_batch()
+    um, eoff, sigt, sigs, probs = small_batch()
     ang = angular.level_symmetric_s8()
     op = batch_op(um, eoff, sigt, sigs, ang)
     q = torch.rand(op.n, dtype=torch.float64, device="cuda")
@@ -57,7 +57,7 @@ def test_persistent_sweep_equals_level_launches(block, per_cu):
 def test_batch_sweep_equals_each_problem_alone():
     import torch
     from lionbolt_amd.ops import Operator
-    um, eoff, sigt, sigs, qmom, probs = small_batch()
+    um, eoff, sigt, sigs, probs = small_batch()
     ang = angular.level_symmetric_s8()
     op = batch_op(um, eoff, sigt, sigs, ang)
     op.set_sweep_mode("auto")
@@ -83,10 +83,11 @@ def test_batch_sweep_equals_each_problem_alone():
 def test_batch_solve_matches_single_problem_solves():
     import torch
     from lionbolt_amd.ops import Operator
-    um, eoff, sigt, sigs, qmom, probs = small_batch()
+    um, eoff, sigt, sigs, probs = small_batch()
     ang = angular.level_symmetric_s8()
     op = batch_op(um, eoff, sigt, sigs, ang)
-    qd = torch.from_numpy(np.ascontiguousarray(qmom.reshape(-1))).cuda()
+    qd = problems.uneven_beams_sources(op, eoff, probs, sigt, sigs).reshape(-1).contiguous()
+    qmom = qd.cpu().numpy()[None, :]
     phi = torch.zeros_like(qd)
     stats = op.batch_solve_dev(qd, phi, m=30, rtol=1e-10)
     assert all(s["status"] == 0 for s in stats)

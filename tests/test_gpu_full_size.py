@@ -69,18 +69,19 @@ def test_full_size_sweep_and_boltzmann_product_match_oracle(full, full_oracle):
 
 @pytest.mark.timeout(900)
 def test_full_size_beam_flux_matches_oracle_gmres(full, full_oracle):
-    """The first group of the 24-group beam problem (its first-collision source,
-    P3, the 3x slab): GPU flux-moment GMRES(30) against the oracle's psi-space
-    GMRES(10), both to 1e-8; scalar flux (LinearOperatorClass.f90:154-174) to 1e-5."""
+    """The first group of the 24-group beam problem (its first-collision source as
+    the reference builds it, P3, the 3x slab): GPU flux-moment GMRES(30) against
+    the oracle's psi-space GMRES(10), both to 1e-8; scalar flux
+    (LinearOperatorClass.f90:154-174) to 1e-5."""
     from lionbolt_amd.ops import Operator
     m, ang = full
     G = 1
     sigt, S = problems.downscatter_xs(G=G, L=ang.L)
-    qmom, _ = problems.beam_source_moments(m, ang, S, sigt)
-    _, Mm = angular.moment_matrices(ang)
-    q = (Mm @ qmom[0]).reshape(-1)
     op = Operator(m, ang)
     op.set_physics(sigt[0], S[0, 0])
+    qmom = problems.beam_group_sources(op, m, S, sigt)[0].cpu().numpy()
+    _, Mm = angular.moment_matrices(ang)
+    q = (Mm @ qmom[0]).reshape(-1)
     a, st = op.boltzmann_matinv(q, kind="gmres", space="moment", m=30, rtol=1e-8)
     assert st["status"] == 0 and st["true_residual"] <= 1e-8
     phi_gpu = op.fluence(a)
@@ -204,7 +205,7 @@ def test_full_size_two_group_beam_solve(full):
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     # first-collision source moments of the +z beam (its uncollided flux lives in group 1)
-    qmom = torch.from_numpy(problems.beam_source_moments(m, ang, S, sigt)[0]).cuda()
+    qmom = problems.beam_group_sources(op, m, S, sigt)[0]
     phi = torch.zeros_like(qmom)
     stats = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="gmres", space="moment", m=30, rtol=1e-7)
     assert all(s["status"] == 0 for s in stats)

@@ -11,6 +11,15 @@ from oracle.oracle import OracleOp
 pytestmark = pytest.mark.gpu
 
 
+def beam_qmom(m, ang, S, sigt):
+    """The +z beam's group source moments (problems.beam_group_sources) as a host array."""
+    from lionbolt_amd.ops import Operator
+    op = Operator(m, ang)
+    q = problems.beam_group_sources(op, m, S, sigt)[0].cpu().numpy()
+    op.close()
+    return q
+
+
 def oracle_multigroup(m, ang, sigt, S, qmom, rtol):
     G = S.shape[0]
     _, Mm = angular.moment_matrices(ang)
@@ -38,7 +47,7 @@ def test_multigroup_downscatter_matches_oracle():
     sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     sigt = sigt * 10.0                   # optically thicker so groups couple noticeably
     S = S * 10.0
-    qmom, _ = problems.beam_source_moments(m, ang, S, sigt)
+    qmom = beam_qmom(m, ang, S, sigt)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     phi = torch.zeros((G, ang.n_moments, m.NENK), dtype=torch.float64, device="cuda")
@@ -66,7 +75,7 @@ def test_group_batched_solve_matches_group_by_group(batch):
     sigt, S = problems.downscatter_xs(G=G, L=ang.L)
     sigt = sigt * 10.0
     S = S * 10.0
-    qmom, _ = problems.beam_source_moments(m, ang, S, sigt)
+    qmom = beam_qmom(m, ang, S, sigt)
     q_d = torch.from_numpy(qmom).cuda()
     res = {}
     for b in (1, batch):

@@ -30,20 +30,3 @@ def test_uneven_beams_draw_is_seeded_and_in_range():
         assert p["mesh"].NE == 6 * p["n"] ** 3
     ne = np.array([p["mesh"].NE for p in a])
     assert ne.min() >= 6 * 7 ** 3 and ne.max() <= 6 * 21 ** 3 and len(set(ne)) > 8
-
-
-def test_cube_beam_path_length():
-    m = M.jittered_cube(4, a=3.0, seed=21)
-    # along +z the path length is the height above z = 0
-    phi = problems.cube_beam_uncollided(m, 3.0, [0.0, 0.0, 1.0], 0.7)
-    assert np.allclose(phi, np.exp(-0.7 * m.dof_coords()[:, 2]), rtol=0, atol=1e-15)
-    # oblique: x - l k0 lies on the surface and x - 0.999 l k0 strictly inside
-    k0 = np.array([0.3, -0.5, math.sqrt(1 - 0.34)])
-    ell = -np.log(problems.cube_beam_uncollided(m, 3.0, k0, 1.0))
-    x = m.dof_coords()
-    foot = x - ell[:, None] * k0
-    on = np.isclose(foot, 0.0, atol=1e-12) | np.isclose(foot, 3.0, atol=1e-12)
-    assert np.all(on.any(axis=1))
-    inner = x - 0.999 * ell[:, None] * k0
-    pos = ell > 1e-9
-    assert np.all((inner[pos] > -1e-12) & (inner[pos] < 3.0 + 1e-12))
