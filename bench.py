@@ -330,15 +330,24 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
     X, Y, data = learned_pc.build_training_set()
     params, tinfo = learned_pc.train(X, Y)
     del X, Y
+    t0 = time.perf_counter()
+    thr, table = learned_pc.select_threshold(params, fmt=fmt)
+    policy = {"threshold": thr if math.isfinite(thr) else "inf", "seconds": time.perf_counter() - t0,
+              "training_problems_fgmres_faster": sum(r["fgmres_s"] < r["gmres_s"] for r in table),
+              "training_problems": len(table),
+              "thickness_range": [min(r["thickness"] for r in table), max(r["thickness"] for r in table)],
+              "iterates_gmres_vs_fgmres": [[round(r["thickness"], 3), r["gmres_iterates"], r["fgmres_iterates"]]
+                                           for r in sorted(table, key=lambda r: r["thickness"])]}
     G = 24
     sigt, S = problems.scattering_dominated_xs(G=G, L=ang.L)
     op = Operator(m, ang)
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_preconditioner(params, fmt)
+    op.set_preconditioner_threshold(thr)
     settle_autotuner(op, m, ang)
     qmom, _ = _beam_qmom(op, m, sigt, S)
     res = {"training": {"rows": data["rows"], "solves": len(data["solves"]),
-                        "data_seconds": data["seconds"], **tinfo}}
+                        "data_seconds": data["seconds"], **tinfo}, "policy": policy}
     for kind in ("gmres", "fgmres"):
         phi = torch.zeros_like(qmom)
         torch.cuda.synchronize()
@@ -349,7 +358,10 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
                      "total_iterates": sum(s["iterates"] for s in st), "sweeps": sum(s["sweeps"] for s in st),
                      "max_true_residual": max(s["true_residual"] for s in st)}
         if kind == "fgmres":
-            res[kind]["fallbacks"] = [s["fallback"] for s in st]
+            res[kind]["fallbacks"] = [s["fallback"] for s in st]   # -1: network not chosen for the group
+            res[kind]["groups_with_network"] = sum(s["fallback"] != -1 for s in st)
+            res[kind]["group_thickness"] = [round(float(sigt[g].max() * np.cbrt(m.volumes().sum())), 4)
+                                            for g in range(G)]
             res[kind]["preconditioner"] = fmt
         if kind == "gmres":
             ref = phi.clone()

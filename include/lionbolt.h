@@ -102,7 +102,8 @@ typedef struct {
   double true_residual;  /* ||T^-1 q - (I - T^-1 K) psi|| / ||T^-1 q||, recomputed in FP64 */
   double seconds;        /* wall time of the solve */
   int fallback;          /* LB_SOLVER_FGMRES: Arnoldi step after which the learned preconditioner
-                            was dropped for stagnation (0: never) */
+                            was dropped for stagnation (0: never; -1: not used, the group is below
+                            the threshold of lb_op_set_preconditioner_threshold) */
 } lb_solve_stats;
 
 /* Op%Build(mesh) + Op%Build(mesh, angular): normals, geometry and the upwind
@@ -174,6 +175,13 @@ int lb_op_set_problems(lb_op* op, int P, const int64_t* elem_off);
 
 /* Sets the network; fmt LB_FMT_BF16 (MFMA inference) or LB_FMT_FP32. */
 int lb_op_set_preconditioner(lb_op* op, const float* params, int nparams, int fmt);
+
+/* The a-priori choice of LB_SOLVER_FGMRES: a group whose optical thickness
+ * (largest sigma_t over the materials times the cube root of the mesh volume)
+ * is below min_thickness is solved without the network (plain flux-moment GMRES;
+ * lb_solve_stats.fallback = -1).  Default 0 (every group uses it); +inf turns the
+ * network off.  The caller derives the threshold from the training problems. */
+int lb_op_set_preconditioner_threshold(lb_op* op, double min_thickness);
 
 /* d_z [16][NENK] = P(d_r [16][NENK]) for group g0 (single-group moment vectors). */
 int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0);

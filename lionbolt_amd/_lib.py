@@ -75,6 +75,7 @@ SIGNATURES = {
     "lb_op_set_preconditioner": (ctypes.c_int, [_vp, _f, ctypes.c_int, ctypes.c_int]),
     "lb_mlp_apply_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_mlp_apply_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_op_set_preconditioner_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
     "lb_beam_uncollided_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, _vp]),
     "lb_beam_uncollided_range_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, ctypes.c_int64, ctypes.c_int64,
                                                     _vp]),

@@ -1,6 +1,7 @@
 // C ABI of liblbhip: operator lifecycle and the host/device entry points that
 // stand in for the reference's LinearOperator procedures.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -402,6 +403,17 @@ int lb_beam_angular_fluence_dev(lb_op* op, const lb_beam_desc* beam, const doubl
       return LB_E_ARG;
     }
     return beam_angular_fluence(op, beam, d_phi0, d_unc);
+  });
+}
+
+int lb_op_set_preconditioner_threshold(lb_op* op, double min_thickness) {
+  return guarded([&]() -> int {
+    if (!op || std::isnan(min_thickness) || min_thickness < 0.0) {
+      set_error("lb_op_set_preconditioner_threshold: needs an operator and a non-negative thickness");
+      return LB_E_ARG;
+    }
+    op->mlp_min_tau = min_thickness;
+    return LB_OK;
   });
 }
 

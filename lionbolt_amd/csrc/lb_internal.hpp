@@ -207,6 +207,7 @@ struct lb_op {
   // learned right preconditioner (per cell-node MLP), see lb_mlp.hip
   bool mlp = false;
   int mlp_fmt = 1;                  // LB_FMT_BF16 (MFMA) or LB_FMT_FP32
+  double mlp_min_tau = 0.0;         // groups optically thinner than this solve without the network
   lb::DevBuf d_mlp_w;               // FP32 parameters, natural layout
   lb::DevBuf d_mlp_pack;            // BF16 weights in MFMA lane order
   lb::DevBuf d_elem_h;              // [NE] element size (6 V)^(1/3)
@@ -311,5 +312,8 @@ int moment_matvec_dev(lb_op* op, double* v, int g0);
 int moment_rhs_dev(lb_op* op, const double* q, double* b, int g0);
 int gmres_snapshots_dev(lb_op* op, const double* q, const lb_solver_opts* o, double* phi, double* snaps,
                         int max_snaps, int* nsnaps, lb_solve_stats* st);
+// Optical thickness of group g: the largest sigma_t over the materials times the
+// cube root of the mesh volume.
+double group_thickness(const lb_op* op, int g);
 
 }  // namespace lb

@@ -590,6 +590,14 @@ int fgmres(lb_op* op, int64_t n, const Apply& apply, const Precond* P, const dou
 
 }  // namespace
 
+double group_thickness(const lb_op* op, int g) {
+  double vol = 0.0;
+  for (double d : op->h_detJ) vol += d / 6.0;
+  double st = 0.0;
+  for (int m = 0; m < op->Nm; ++m) st = std::max(st, op->h_sigt[(size_t)m * op->G + g]);
+  return st * std::cbrt(vol);
+}
+
 // out <- D T^-1 M S in: the scattering-sweep-moments operator K of group g0 on
 // flux moments, out of place.
 int moment_k_dev(lb_op* op, const double* in, double* out, int g0) {
@@ -792,9 +800,11 @@ int boltzmann_matinv_dev(lb_op* op, double* s, int g0, const lb_solver_opts* o, 
       return moment_matvec_dev(op, v, g0);
     };
     Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g0); };
+    const bool use_net = group_thickness(op, g0) >= op->mlp_min_tau;
     FgmresResult r;
-    LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr, &r,
-                  true));
+    LB_TRY(fgmres(op, nphi, A, use_net ? &Pc : nullptr, rhs, phi, V, Z, R, m, opt.rtol, opt.atol, opt.maxit,
+                  opt.verbose, nullptr, &r, true));
+    if (!use_net) r.fallback = -1;
     LB_TRY(launch_copy(str, n, q, s));
     LB_TRY(launch_moments_to_discrete(op, phi, s, g0, 1, 1));
     LB_TRY(sweep_inplace(op, s, g0));
@@ -1038,7 +1048,9 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
       return moment_k_dev(op, in, out, g);
     };
     double* phi_g = phi_out + (int64_t)g * nphi;
-    if (flexible) {
+    // the network only where the training problems showed it pays (lb_op_set_preconditioner_threshold)
+    const bool use_net = flexible && group_thickness(op, g) >= op->mlp_min_tau;
+    if (use_net) {
       Precond Pc = [&](const double* v, double* z) { return mlp_apply(op, v, z, g); };
       FgmresResult r;
       LB_TRY(fgmres(op, nphi, A, &Pc, rhs, phi_g, V, Z, R, m, opt.rtol, opt.atol, opt.maxit, opt.verbose, nullptr,
@@ -1055,6 +1067,7 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
       st[g].iterates = r.iterates;
       st[g].matvecs = r.matvecs;
       st[g].final_norm = r.final_norm;
+      if (flexible) st[g].fallback = -1;   // plain GMRES chosen before the solve
     }
     st[g].sweeps = sweeps;
     st[g].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -18,9 +18,10 @@
 // slot order (host side, once).  An FP32 path (one thread per dof, weights in
 // LDS) serves as the precision reference.
 //
-// Training (FP32): minibatch Adam over (feature, target) rows with small GEMM
-// kernels (one thread per output element, sums over the batch in a fixed order,
-// so gradients are deterministic).
+// Training (FP32): minibatch Adam over (feature, target) rows; the forward and
+// backward GEMMs run on FP32 matrix cores (v_mfma_f32_16x16x4_f32), the weight
+// gradients' sums over the minibatch split 16 ways and reduced in a fixed order,
+// so gradients are deterministic.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -204,22 +205,115 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
   }
 }
 
-// ---- training kernels (FP32) ------------------------------------------------
+// ---- training GEMMs on FP32 matrix cores -----------------------------------
+//
+// C (M x N) = sum_k A(m, k) B(k, n) with v_mfma_f32_16x16x4_f32: one wave per
+// 16 x 16 tile of C, k in steps of 4 (lane l supplies A(m0 + l % 16, k0 + l / 16)
+// and B(k0 + l / 16, n0 + l % 16) and receives C(m0 + 4 (l / 16) + t, n0 + l % 16),
+// t < 4).  A(m, k) = a[ga(m) sam + gk(k) sak] with optional row gathers ga / gk
+// (the minibatch rows of the training set), B(k, n) = b[k sbk + n sbn].  With
+// splits > 1 the k range is cut into `splits` pieces whose partial tiles land in
+// `part` and are summed in split order by gemm_reduce_kernel, so the sums do not
+// depend on scheduling.  Epilogues: kBias (+ bias[n], tanh if act), kTanhGrad
+// (x (1 - h(m, n)^2), the tanh derivative of the layer below), kPlain.
+enum { kBias = 0, kTanhGrad = 1, kPlain = 2 };
 
-// out[b][j] = act?(sum_k in[row(b)][k] W[k][j] + bias[j]); row(b) = idx ? idx[b] : b.
-__global__ __launch_bounds__(256) void dense_fwd_kernel(int B, int K, int J, const float* __restrict__ in,
-                                                        const int32_t* __restrict__ idx, const float* __restrict__ W,
-                                                        const float* __restrict__ bias, int use_act,
-                                                        float* __restrict__ out) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= B * J) return;
-  const int b = t / J, j = t - b * J;
-  const int64_t row = idx ? (int64_t)idx[b] : (int64_t)b;
-  const float* x = in + row * K;
-  float a = bias[j];
-  for (int k = 0; k < K; ++k) a += x[k] * W[k * J + j];
-  out[t] = use_act ? act(a) : a;
+struct GemmArgs {
+  int M, N, K;
+  const float* a;
+  const int32_t* ga;   // gather of A's rows (m), or null
+  const int32_t* gk;   // gather of A's columns (k) as rows of the stored array, or null
+  int64_t sam, sak;
+  const float* b;
+  int64_t sbk, sbn;
+  float* c;
+  int64_t ldc;
+  int mode, act;
+  const float* bias;
+  const float* h;      // kTanhGrad: [M][ldc]
+  int splits;
+  float* part;         // [splits][M][N] when splits > 1
+};
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4v;
+
+__device__ __forceinline__ float gemm_epilogue(const GemmArgs& g, int i, int j, float v) {
+  if (g.mode == kBias) {
+    if (g.bias) v += g.bias[j];
+    if (g.act) v = tanhf(v);
+  } else if (g.mode == kTanhGrad) {
+    const float hv = g.h[(int64_t)i * g.ldc + j];
+    v *= 1.f - hv * hv;
+  }
+  return v;
 }
+
+__global__ __launch_bounds__(256) void mfma_gemm_kernel(GemmArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int tn = (g.N + 15) / 16, tm = (g.M + 15) / 16;
+  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wid >= tm * tn * g.splits) return;   // whole waves leave together
+  const int split = wid / (tm * tn), tile = wid - split * tm * tn;
+  const int m0 = (tile / tn) * 16, n0 = (tile % tn) * 16;
+  const int r = lane & 15, q = lane >> 4;
+  const int kchunk = ((g.K + g.splits - 1) / g.splits + 3) / 4 * 4;
+  const int k_begin = split * kchunk, k_end = min(g.K, k_begin + kchunk);
+  const int am = m0 + r, bn = n0 + r;
+  const bool a_ok = am < g.M, b_ok = bn < g.N;
+  const int64_t arow = a_ok ? (int64_t)(g.ga ? g.ga[am] : am) * g.sam : 0;
+  f32x4v acc = {0.f, 0.f, 0.f, 0.f};
+  int k0 = k_begin;
+  // eight k-steps' operands are loaded before their products
+  for (; k0 + 32 <= k_end; k0 += 32) {
+    float av[8], bv[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = k0 + 4 * s + q;
+      av[s] = a_ok ? g.a[arow + (int64_t)(g.gk ? g.gk[k] : k) * g.sak] : 0.f;
+      bv[s] = b_ok ? g.b[(int64_t)k * g.sbk + (int64_t)bn * g.sbn] : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s], acc, 0, 0, 0);
+  }
+  for (; k0 < k_end; k0 += 4) {
+    const int k = k0 + q;
+    const bool k_ok = k < k_end;
+    const float av = (a_ok && k_ok) ? g.a[arow + (int64_t)(g.gk ? g.gk[k] : k) * g.sak] : 0.f;
+    const float bv = (b_ok && k_ok) ? g.b[(int64_t)k * g.sbk + (int64_t)bn * g.sbn] : 0.f;
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+  }
+  const int j = n0 + r;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int i = m0 + 4 * q + t;
+    if (i < g.M && j < g.N) {
+      if (g.splits > 1)
+        g.part[((int64_t)split * g.M + i) * g.N + j] = acc[t];
+      else
+        g.c[(int64_t)i * g.ldc + j] = gemm_epilogue(g, i, j, acc[t]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)g.M * g.N) return;
+  const int i = int(t / g.N), j = int(t - (int64_t)i * g.N);
+  float v = 0.f;
+  for (int s = 0; s < g.splits; ++s) v += g.part[((int64_t)s * g.M + i) * g.N + j];
+  g.c[(int64_t)i * g.ldc + j] = gemm_epilogue(g, i, j, v);
+}
+
+// db[j] = sum_b dout[b][j], b in order.
+__global__ void bias_grad_kernel(int B, int J, const float* __restrict__ dout, float* __restrict__ db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  float a = 0.f;
+  for (int b = 0; b < B; ++b) a += dout[(int64_t)b * J + j];
+  db[j] = a;
+}
+
+// ---- training kernels (FP32) ------------------------------------------------
 
 // dy = 2 (yhat - y) / (B * kOut); part[block] = sum of squared errors / (B * kOut)
 __global__ __launch_bounds__(256) void loss_grad_kernel(int B, const float* __restrict__ yhat,
@@ -251,41 +345,6 @@ __global__ void sum_kernel(int n, const float* __restrict__ part, float* __restr
   }
 }
 
-// dW[k][j] = sum_b in[row(b)][k] dout[b][j] (threads k*J + j), db[j] = sum_b dout[b][j] (threads K*J + j).
-__global__ __launch_bounds__(256) void dense_bwd_w_kernel(int B, int K, int J, const float* __restrict__ in,
-                                                          const int32_t* __restrict__ idx,
-                                                          const float* __restrict__ dout, float* __restrict__ dW,
-                                                          float* __restrict__ db) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t < K * J) {
-    const int k = t / J, j = t - k * J;
-    float a = 0.f;
-    for (int b = 0; b < B; ++b) {
-      const int64_t row = idx ? (int64_t)idx[b] : (int64_t)b;
-      a += in[row * K + k] * dout[b * J + j];
-    }
-    dW[t] = a;
-  } else if (t < K * J + J) {
-    const int j = t - K * J;
-    float a = 0.f;
-    for (int b = 0; b < B; ++b) a += dout[b * J + j];
-    db[j] = a;
-  }
-}
-
-// din[b][k] = (sum_j dout[b][j] W[k][j]) (1 - h[b][k]^2), h = tanh output of the layer below.
-__global__ __launch_bounds__(256) void dense_bwd_x_kernel(int B, int K, int J, const float* __restrict__ dout,
-                                                          const float* __restrict__ W, const float* __restrict__ h,
-                                                          float* __restrict__ din) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= B * K) return;
-  const int b = t / K, k = t - b * K;
-  float a = 0.f;
-  for (int j = 0; j < J; ++j) a += dout[b * J + j] * W[k * J + j];
-  const float hv = h[t];
-  din[t] = a * (1.f - hv * hv);
-}
-
 __global__ __launch_bounds__(256) void adam_kernel(int n, float* __restrict__ w, const float* __restrict__ g,
                                                    float* __restrict__ m, float* __restrict__ v, float lr, float b1,
                                                    float b2, float eps, float bc1, float bc2) {
@@ -308,6 +367,106 @@ uint16_t bf16_rne(float x) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
+
+// Partial tiles of a split-k training GEMM: splits x M x N floats at most.
+constexpr int kSplits = 16;
+constexpr size_t kPartFloats = (size_t)kSplits * kH * kH;
+
+void launch_gemm(hipStream_t st, const GemmArgs& g) {
+  const int tiles = ((g.M + 15) / 16) * ((g.N + 15) / 16);
+  hipLaunchKernelGGL(mfma_gemm_kernel, dim3((unsigned)((tiles * g.splits + 3) / 4)), dim3(256), 0, st, g);
+  if (g.splits > 1)
+    hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)(((int64_t)g.M * g.N + 255) / 256)), dim3(256), 0, st, g);
+}
+
+GemmArgs gemm_args(int M, int N, int K, const float* a, const int32_t* ga, const int32_t* gk, int64_t sam,
+                   int64_t sak, const float* b, int64_t sbk, int64_t sbn, float* c, int64_t ldc, int mode, int act,
+                   const float* bias, const float* h, int splits, float* part) {
+  GemmArgs g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.a = a;
+  g.ga = ga;
+  g.gk = gk;
+  g.sam = sam;
+  g.sak = sak;
+  g.b = b;
+  g.sbk = sbk;
+  g.sbn = sbn;
+  g.c = c;
+  g.ldc = ldc;
+  g.mode = mode;
+  g.act = act;
+  g.bias = bias;
+  g.h = h;
+  g.splits = splits;
+  g.part = part;
+  return g;
+}
+
+// Activations, output errors and gradients of one minibatch step.
+struct StepBufs {
+  float *h1, *h2, *h3, *yh, *d3, *d2, *d1, *dy, *part, *lpart, *gpart;
+};
+
+// Forward pass, loss and its gradient with respect to every parameter for the
+// rows idx[0..B-1] of (X, Y): the network's GEMMs on FP32 matrix cores, the
+// weight gradients (sums over the minibatch, k = B) split 16 ways.
+void enqueue_step(hipStream_t st, int B, const float* X, const float* Y, const int32_t* idx, const float* w,
+                  float* g, float* loss, const StepBufs& t) {
+  const int sp = B >= 256 ? kSplits : 1;
+  // forward: h = tanh(in W + b), yh = h3 W4 + b4
+  launch_gemm(st, gemm_args(B, kH, kIn, X, idx, nullptr, kIn, 1, w + kW1, kH, 1, t.h1, kH, kBias, 1, w + kB1,
+                            nullptr, 1, nullptr));
+  launch_gemm(st, gemm_args(B, kH, kH, t.h1, nullptr, nullptr, kH, 1, w + kW2, kH, 1, t.h2, kH, kBias, 1, w + kB2,
+                            nullptr, 1, nullptr));
+  launch_gemm(st, gemm_args(B, kH, kH, t.h2, nullptr, nullptr, kH, 1, w + kW3, kH, 1, t.h3, kH, kBias, 1, w + kB3,
+                            nullptr, 1, nullptr));
+  launch_gemm(st, gemm_args(B, kOut, kH, t.h3, nullptr, nullptr, kH, 1, w + kW4, kOut, 1, t.yh, kOut, kBias, 0,
+                            w + kB4, nullptr, 1, nullptr));
+  const unsigned lb = (unsigned)(((int64_t)B * kOut + 255) / 256);
+  hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, t.yh, Y, idx, t.dy, t.lpart);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, t.lpart, loss);
+  // backward: dW = in^T d (k = minibatch rows), db = column sums of d, d_in = (d W^T) (1 - h^2)
+  launch_gemm(st, gemm_args(kH, kOut, B, t.h3, nullptr, nullptr, 1, kH, t.dy, kOut, 1, g + kW4, kOut, kPlain, 0,
+                            nullptr, nullptr, sp, t.part));
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kOut, t.dy, g + kB4);
+  launch_gemm(st, gemm_args(B, kH, kOut, t.dy, nullptr, nullptr, kOut, 1, w + kW4, 1, kOut, t.d3, kH, kTanhGrad, 0,
+                            nullptr, t.h3, 1, nullptr));
+  launch_gemm(st, gemm_args(kH, kH, B, t.h2, nullptr, nullptr, 1, kH, t.d3, kH, 1, g + kW3, kH, kPlain, 0, nullptr,
+                            nullptr, sp, t.part));
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d3, g + kB3);
+  launch_gemm(st, gemm_args(B, kH, kH, t.d3, nullptr, nullptr, kH, 1, w + kW3, 1, kH, t.d2, kH, kTanhGrad, 0, nullptr,
+                            t.h2, 1, nullptr));
+  launch_gemm(st, gemm_args(kH, kH, B, t.h1, nullptr, nullptr, 1, kH, t.d2, kH, 1, g + kW2, kH, kPlain, 0, nullptr,
+                            nullptr, sp, t.part));
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d2, g + kB2);
+  launch_gemm(st, gemm_args(B, kH, kH, t.d2, nullptr, nullptr, kH, 1, w + kW2, 1, kH, t.d1, kH, kTanhGrad, 0, nullptr,
+                            t.h1, 1, nullptr));
+  launch_gemm(st, gemm_args(kIn, kH, B, X, nullptr, idx, 1, kIn, t.d1, kH, 1, g + kW1, kH, kPlain, 0, nullptr,
+                            nullptr, sp, t.part));
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d1, g + kB1);
+}
+
+// Carves the step buffers out of `f` (floats): (3 kH + kOut) B activations,
+// (3 kH + kOut) B errors, the split partials and 256 loss partials.
+StepBufs carve(float* f, int B) {
+  StepBufs t;
+  t.h1 = f;
+  t.h2 = t.h1 + (size_t)B * kH;
+  t.h3 = t.h2 + (size_t)B * kH;
+  t.yh = t.h3 + (size_t)B * kH;
+  t.d3 = t.yh + (size_t)B * kOut;
+  t.d2 = t.d3 + (size_t)B * kH;
+  t.d1 = t.d2 + (size_t)B * kH;
+  t.dy = t.d1 + (size_t)B * kH;
+  t.part = t.dy + (size_t)B * kOut;
+  t.lpart = t.part + kPartFloats;
+  t.gpart = t.lpart + 256;
+  return t;
+}
+constexpr size_t step_floats(int B) { return (size_t)B * (3 * kH + kOut) * 2 + kPartFloats + 256; }
 
 // Input neuron carried by K slot q of lane group kb in step st: layer 1 reads
 // the features in order; deeper layers read the registers the layer below left.
@@ -509,8 +668,8 @@ int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* param
     hipStream_t st = nullptr;
     LB_HIP(hipStreamCreate(&st));
     const int B = batch;
-    // every buffer in one allocation: params, grads, Adam m and v, activations, indices, loss
-    const size_t nf = (size_t)4 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 256 + steps;
+    // every buffer in one allocation: params, grads, Adam m and v, step buffers, loss, indices
+    const size_t nf = (size_t)4 * kParams + step_floats(B) + steps;
     const size_t bytes = nf * sizeof(float) + (size_t)steps * B * sizeof(int32_t);
     void* mem = nullptr;
     hipError_t e = hipMalloc(&mem, bytes);
@@ -523,16 +682,8 @@ int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* param
     float* g = w + kParams;
     float* am = g + kParams;
     float* av = am + kParams;
-    float* h1 = av + kParams;
-    float* h2 = h1 + (size_t)B * kH;
-    float* h3 = h2 + (size_t)B * kH;
-    float* yh = h3 + (size_t)B * kH;
-    float* d3 = yh + (size_t)B * kOut;
-    float* d2 = d3 + (size_t)B * kH;
-    float* d1 = d2 + (size_t)B * kH;
-    float* dy = d1 + (size_t)B * kH;
-    float* part = dy + (size_t)B * kOut;
-    float* loss = part + 256;
+    const StepBufs t = carve(av + kParams, B);
+    float* loss = t.gpart;
     int32_t* idx = reinterpret_cast<int32_t*>(loss + steps);
     int rc = LB_OK;
     auto fail = [&](hipError_t err, const char* what) {
@@ -557,31 +708,7 @@ int lb_mlp_train_dev(const float* d_X, const float* d_Y, int64_t N, float* param
     float bc1 = 1.f, bc2 = 1.f;
     for (int s = 0; s < steps && rc == LB_OK; ++s) {
       const int32_t* ix = idx + (size_t)s * B;
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix,
-                         w + kW1, w + kB1, 1, h1);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
-                         w + kW2, w + kB2, 1, h2);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
-                         w + kW3, w + kB3, 1, h3);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                         w + kW4, w + kB4, 0, yh);
-      const unsigned lb = blocks((int64_t)B * kOut);
-      hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, ix, dy, part);
-      hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss + s);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                         dy, g + kW4, g + kB4);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
-                         h3, d3);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
-                         g + kW3, g + kB3);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
-                         h2, d2);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
-                         g + kW2, g + kB2);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
-                         h1, d1);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, ix, d1,
-                         g + kW1, g + kB1);
+      enqueue_step(st, B, d_X, d_Y, ix, w, g, loss + s, t);
       bc1 *= b1;
       bc2 *= b2;
       hipLaunchKernelGGL(adam_kernel, dim3(blocks(kParams)), dim3(256), 0, st, kParams, w, g, am, av, lr, b1, b2, eps,
@@ -613,20 +740,12 @@ int lb_mlp_grad_dev(const float* d_X, const float* d_Y, int B, const float* para
       return LB_E_ARG;
     }
     void* mem = nullptr;
-    const size_t nf = (size_t)2 * kParams + (size_t)B * (3 * kH + kOut) * 2 + 257;
+    const size_t nf = (size_t)2 * kParams + step_floats(B) + 1;
     LB_HIP(hipMalloc(&mem, nf * sizeof(float) + (size_t)B * sizeof(int32_t)));
     float* w = static_cast<float*>(mem);
     float* g = w + kParams;
-    float* h1 = g + kParams;
-    float* h2 = h1 + (size_t)B * kH;
-    float* h3 = h2 + (size_t)B * kH;
-    float* yh = h3 + (size_t)B * kH;
-    float* d3 = yh + (size_t)B * kOut;
-    float* d2 = d3 + (size_t)B * kH;
-    float* d1 = d2 + (size_t)B * kH;
-    float* dy = d1 + (size_t)B * kH;
-    float* part = dy + (size_t)B * kOut;
-    float* loss = part + 256;
+    const StepBufs t = carve(g + kParams, B);
+    float* loss = t.gpart;
     int32_t* idx = reinterpret_cast<int32_t*>(loss + 1);
     std::vector<int32_t> h_idx(B);
     for (int b = 0; b < B; ++b) h_idx[b] = b;
@@ -639,31 +758,7 @@ int lb_mlp_grad_dev(const float* d_X, const float* d_Y, int B, const float* para
       rc = LB_E_HIP;
     }
     if (rc == LB_OK) {
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx,
-                         w + kW1, w + kB1, 1, h1);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr,
-                         w + kW2, w + kB2, 1, h2);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr,
-                         w + kW3, w + kB3, 1, h3);
-      hipLaunchKernelGGL(dense_fwd_kernel, dim3(blocks((int64_t)B * kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                         w + kW4, w + kB4, 0, yh);
-      const unsigned lb = blocks((int64_t)B * kOut);
-      hipLaunchKernelGGL(loss_grad_kernel, dim3(lb), dim3(256), 0, st, B, yh, d_Y, idx, dy, part);
-      hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, st, (int)lb, part, loss);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kOut + kOut)), dim3(256), 0, st, B, kH, kOut, h3, nullptr,
-                         dy, g + kW4, g + kB4);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kOut, dy, w + kW4,
-                         h3, d3);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h2, nullptr, d3,
-                         g + kW3, g + kB3);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d3, w + kW3,
-                         h2, d2);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kH * kH + kH)), dim3(256), 0, st, B, kH, kH, h1, nullptr, d2,
-                         g + kW2, g + kB2);
-      hipLaunchKernelGGL(dense_bwd_x_kernel, dim3(blocks((int64_t)B * kH)), dim3(256), 0, st, B, kH, kH, d2, w + kW2,
-                         h1, d1);
-      hipLaunchKernelGGL(dense_bwd_w_kernel, dim3(blocks(kIn * kH + kH)), dim3(256), 0, st, B, kIn, kH, d_X, idx, d1,
-                         g + kW1, g + kB1);
+      enqueue_step(st, B, d_X, d_Y, idx, w, g, loss, t);
       if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
           (e = hipMemcpy(grad, g, kParams * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess ||
           (loss_out && (e = hipMemcpy(loss_out, loss, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)) {

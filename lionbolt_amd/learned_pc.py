@@ -130,6 +130,44 @@ def build_training_set(nmesh=32, seed=2000, rows_per_snap=512, L=3, log=None):
     return X, Y, {"rows": int(X.shape[0]), "solves": info, "seconds": time.perf_counter() - t0}
 
 
+def select_threshold(params, nmesh=32, seed=2000, L=3, fmt="bf16", rtol=1e-7):
+    """The a-priori choice between the network and plain GMRES, from the training
+    problems (beam sources): each is solved with flux-moment GMRES(30) and with
+    FGMRES(30) and the network, both timed on the device after a warm-up solve.
+    The threshold is the smallest optical thickness t (largest sigma_t times the
+    cube root of the mesh volume, as the library measures a group) such that
+    FGMRES was faster on every training problem at least t thick; inf (the network
+    never used) when it was slower on the thickest.  Returns (threshold, table)."""
+    import torch
+    from .ops import Operator
+    ang = problems.product_8x32() if L == 3 else angular.product(8, 32, L)
+    table = []
+    for p, pr in enumerate(problems.pc_training_problems(nmesh, seed, L=L)):
+        m = pr["mesh"]
+        op = Operator(m, ang)
+        op.set_physics_groups(pr["sigt"][None, :], pr["sigs"][None, :, :])
+        op.set_preconditioner(params, fmt)
+        S = np.zeros((1, 1) + pr["sigs"].shape)
+        S[0, 0] = pr["sigs"]
+        qm = _source_moments(op, m, ang, pr["sigt"], pr["sigs"], "beam").reshape(1, op.nmom, op.NENK)
+        secs, its = {}, {}
+        for kind in ("gmres", "gmres", "fgmres"):             # the first solve settles the sweep autotuner
+            phi = torch.zeros_like(qm)
+            st = op.multigroup_solve_dev(S, phi, qmom=qm, kind=kind, space="moment", m=30, rtol=rtol, atol=0.0)
+            secs[kind], its[kind] = st[0]["seconds"], st[0]["iterates"]
+        tau = float(pr["sigt"].max() * np.cbrt(m.volumes().sum()))
+        table.append({"problem": p, "thickness": tau, "gmres_s": secs["gmres"], "fgmres_s": secs["fgmres"],
+                      "gmres_iterates": its["gmres"], "fgmres_iterates": its["fgmres"]})
+        op.close()
+    thr = math.inf
+    for row in sorted(table, key=lambda r: -r["thickness"]):
+        if row["fgmres_s"] < row["gmres_s"]:
+            thr = row["thickness"]
+        else:
+            break
+    return thr, table
+
+
 def train(X, Y, steps=4000, batch=2048, lr=2e-3, seed=2100):
     """FP32 Adam in the library's kernels; targets are standardised by their RMS
     tau for training and tau is folded back into the output layer."""

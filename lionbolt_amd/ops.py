@@ -265,6 +265,10 @@ class Operator:
         f = {"bf16": _lib.FMT_BF16, "fp32": _lib.FMT_FP32}[fmt]
         _lib.check(_lib.lib().lb_op_set_preconditioner(self.h, p.ctypes.data_as(_lib._f), p.size, f))
 
+    def set_preconditioner_threshold(self, min_thickness):
+        """Groups optically thinner than ``min_thickness`` skip the network (plain GMRES)."""
+        _lib.check(_lib.lib().lb_op_set_preconditioner_threshold(self.h, float(min_thickness)))
+
     def mlp_apply_dev(self, r, z, g0=0):
         nphi = self.nmom * self.NENK
         r_p, z_p = _dev(r, nphi, "r"), _dev(z, nphi, "z")

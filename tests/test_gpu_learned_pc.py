@@ -146,3 +146,42 @@ def test_preconditioner_argument_checks():
         o2.set_physics(sigt, sigs)
         o2.boltzmann_matinv(q, kind="fgmres", space="moment")
     assert ei.value.code == -3
+
+
+def test_threshold_chooses_plain_gmres_per_group():
+    """lb_op_set_preconditioner_threshold: groups thinner than the threshold run
+    the plain flux-moment GMRES (fallback -1, the same flux bit for bit); an
+    infinite threshold turns the network off, zero leaves it on everywhere."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(4, a=6.0, seed=77, slab=(2.0, 3.5))
+    ang = angular.product(4, 8, 3)
+    G = 3
+    sigt = np.array([[0.05, 0.15], [0.4, 1.2], [1.5, 4.5]])
+    sigs = np.stack([np.stack([0.99 * s * 0.6 ** np.arange(4) for s in row]) for row in sigt])
+    S = np.zeros((G, G, 2, 4))
+    for g in range(G):
+        S[g, g] = sigs[g]
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, sigs)
+    op.set_preconditioner(learned_pc.init_params(5), "bf16")
+    qm = torch.zeros((G, 16, m.NENK), dtype=torch.float64, device="cuda")
+    qm[:, 0] = math.sqrt(4 * math.pi)
+    plain = torch.zeros_like(qm)
+    op.multigroup_solve_dev(S, plain, qmom=qm, kind="gmres", rtol=1e-9)
+    thick = np.array([sigt[g].max() * np.cbrt(m.volumes().sum()) for g in range(G)])
+    mid = float(0.5 * (thick[0] + thick[1]))              # between the first two groups' thicknesses
+    for thr, used in ((math.inf, [False] * G), (0.0, [True] * G), (mid, [False, True, True])):
+        op.set_preconditioner_threshold(thr)
+        phi = torch.zeros_like(qm)
+        st = op.multigroup_solve_dev(S, phi, qmom=qm, kind="fgmres", rtol=1e-9)
+        assert all(s["status"] == 0 for s in st)
+        assert [s["fallback"] != -1 for s in st] == used, thr
+        for g in range(G):
+            if not used[g]:
+                assert torch.equal(phi[g], plain[g]), (thr, g)
+            else:
+                assert float((phi[g] - plain[g]).abs().max() / plain[g].abs().max()) <= 1e-6
+    with pytest.raises(_lib.LionboltError):
+        op.set_preconditioner_threshold(-1.0)
+    op.close()
