@@ -368,6 +368,21 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
         else:
             res["flux_rel_diff"] = float((phi[:, 0] - ref[:, 0]).abs().max() / ref[:, 0].abs().max())
     res["speedup_fgmres_vs_gmres"] = res["gmres"]["seconds"] / res["fgmres"]["seconds"]
+    # one network apply on a full-size residual (RMS scaling + the MFMA kernel);
+    # algorithmic bytes per spatial dof: 16 FP64 moments in and out, plus the
+    # element's material and size shared by its 4 nodes (12 B / 4)
+    r = torch.randn(op.nmom * op.NENK, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(r)
+    op.mlp_apply_dev(r, z, 0)
+    op.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.mlp_apply_dev(r, z, 0)
+    op.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    res["network_apply"] = {"dofs": op.NENK, "ms": ms, "bytes_per_dof": 259,
+                            "GBs_algorithmic": 259 * op.NENK / (ms * 1e-3) / 1e9, "format": fmt}
     op.close()
     return res
 
@@ -426,7 +441,7 @@ def single_group_leg(m, ang, reps=10):
     psi = torch.empty_like(q)
     res = {}
     for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "direction_lds-256",
-                 "direction_lds-512", "auto"):
+                 "direction_lds-512", "direction_lds_element-256", "direction_lds_element-512", "auto"):
         if mode == "auto":
             op.set_sweep_mode("auto")
             for _ in range(32):
@@ -483,7 +498,7 @@ def run(args):
             op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         mode, block, _ = op.sweep_choice(G)
-    elif args.sweep_mode in ("direction", "direction_lds"):
+    elif args.sweep_mode in ("direction", "direction_lds", "direction_lds_element"):
         block = args.block
         op.set_direction_block(block)
     for _ in range(args.warmup):
@@ -620,7 +635,8 @@ def main():
     p.add_argument("--no-learned-pc", dest="learned_pc", action="store_false",
                    help="skip the learned-preconditioner solve")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
-    p.add_argument("--sweep-mode", default="auto", choices=["auto", "levels", "graph", "direction", "direction_lds"])
+    p.add_argument("--sweep-mode", default="auto",
+                   choices=["auto", "levels", "graph", "direction", "direction_lds", "direction_lds_element"])
     p.add_argument("--block", type=int, default=1024, choices=[256, 512, 1024],
                    help="workgroup size of the direction-resident sweep modes")
     p.add_argument("--compare-modes", action="store_true")

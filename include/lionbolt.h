@@ -51,7 +51,8 @@ enum {
   LB_SWEEP_DIRECTION = 2,
   LB_SWEEP_AUTO = 3,
   LB_SWEEP_PERSISTENT = 4,
-  LB_SWEEP_DIRECTION_LDS = 5
+  LB_SWEEP_DIRECTION_LDS = 5,
+  LB_SWEEP_DIRECTION_LDS_ELEMENT = 6
 };
 
 typedef struct lb_op lb_op;
@@ -132,7 +133,10 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
  * LB_SWEEP_DIRECTION_LDS does the same and also keeps upstream traces in LDS
  * slots fixed by a slot plan built on first use per group-batch size (ring
  * slots by Belady's rule, staging copies one level ahead, global memory for
- * the rest; see lb_trace_plan_host); LB_SWEEP_AUTO (default) times the graph
+ * the rest; see lb_trace_plan_host); LB_SWEEP_DIRECTION_LDS_ELEMENT is the LDS
+ * sweep reading per-element tables (neighbours, node permutation, material,
+ * face normals) gathered by cell and recomputing c_f (k . n_f), instead of the
+ * per-(direction, cell) records, below 8 groups per cell; LB_SWEEP_AUTO (default) times the graph
  * mode, the direction-resident mode at 256, 512 and 1024 threads and, below 8
  * groups per cell, the LDS one at 256 and 512 threads on the first sweeps of
  * each group-batch size (two sweeps each, synchronising after each) and then

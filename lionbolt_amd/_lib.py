@@ -16,6 +16,7 @@ SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
 SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT, SWEEP_DIRECTION_LDS = 0, 1, 2, 3, 4, 5
+SWEEP_DIRECTION_LDS_ELEMENT = 6
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)

@@ -278,7 +278,7 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
   return guarded([&]() -> int {
-    if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_DIRECTION_LDS) {
+    if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_DIRECTION_LDS_ELEMENT) {
       set_error("lb_op_set_sweep_mode: invalid mode");
       return LB_E_ARG;
     }
@@ -712,7 +712,8 @@ void lb_op_destroy(lb_op* op) {
   clear_graphs(op);
   for (DevBuf* b : {&op->d_geo, &op->d_nbr, &op->d_perm, &op->d_mat, &op->d_tasks, &op->d_k, &op->d_w, &op->d_D,
                     &op->d_M, &op->d_deg, &op->d_yys, &op->d_sigt, &op->d_sigs, &op->w_a, &op->w_a2, &op->w_b,
-                    &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov, &op->w_mg, &op->w_acc, &op->w_S})
+                    &op->w_c, &op->w_phi, &op->w_red, &op->w_krylov, &op->w_mg, &op->w_acc, &op->w_S,
+                    &op->w_arn})
     b->release();
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);

@@ -219,6 +219,7 @@ struct lb_op {
   float last_sweep_ms = 0.f;
   // workspaces
   lb::DevBuf w_a, w_a2, w_b, w_c, w_phi, w_red, w_krylov, w_host, w_mg, w_acc, w_S;
+  lb::DevBuf w_arn;                 // device-resident Arnoldi scalars (lb::ArnoldiDev)
   lb::DevBuf w_tq;                  // T^-1 q of the current solve, for its true residual
   lb::DevBuf w_tail;                // batched residual checks of the multigroup solve (freed after it)
 };
@@ -272,6 +273,38 @@ int dot(lb_op* op, int64_t n, const double* x, const double* y, double* out);
 int multi_dot(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* w, bool self, double* out);
 // w <- (w - sum_{i<c} h_i V_i) * scale
 int multi_axpy(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* h, double scale, double* w);
+
+// Device-resident scalars of one restarted-GMRES cycle: the Hessenberg matrix
+// H ((m + 1) x m, column-major), the Givens rotations (cs, sn), the rotated
+// right-hand side g, and the Gram-Schmidt projections of the current step.  An
+// Arnoldi step runs its Gram-Schmidt passes, the decision on the second pass, the
+// Hessenberg column, the rotations and the residual estimate on the device, and
+// the host reads the step's outcome (ArnoldiCtl) once, after the step.
+enum { kArnPass2 = 0, kArnDone1 = 1, kArnDone2 = 2, kArnInvariant = 3, kArnNaN = 4 };
+struct ArnoldiCtl {
+  double scale;   // factor of the pending orthogonalisation update
+  double nu;      // norm of the orthogonalised vector (0: invariant subspace, NaN)
+  double resn;    // residual estimate |g_{k+1}| after the step
+  double pad;
+  int state;      // kArn*
+  int pad2[7];
+};
+struct ArnoldiDev {
+  int m = 0;
+  ArnoldiCtl* ctl = nullptr;
+  double *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *hk = nullptr, *d = nullptr, *coef = nullptr,
+         *y = nullptr;
+};
+int arnoldi_reserve(lb_op* op, int m, ArnoldiDev* a);
+// g <- (beta, 0, ..., 0): the start of a cycle.
+int arnoldi_start(lb_op* op, const ArnoldiDev& a, double beta);
+// Orthonormalises V_{k+1} (stride n) against V_0 .. V_k (classical Gram-Schmidt,
+// once more when the first pass removes more than half of its squared norm),
+// writes column k of H (of I - K when shifted), applies the rotations and
+// updates g; one host synchronisation, whose result lands in *out.
+int arnoldi_step(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev& a, bool shifted, ArnoldiCtl* out);
+// sol <- sol + V y, y = R^-1 g over the first k columns; no host synchronisation.
+int arnoldi_update(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev& a, double* sol);
 int launch_fluence(lb_op* op, const double* v, double* phi);
 
 // lb_groups.hip (group-batched multigroup solve)

@@ -12,6 +12,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "lb_internal.hpp"
 
@@ -53,7 +54,23 @@ struct SweepArgs {
   const int32_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
   const int32_t* __restrict__ stage_off = nullptr;   // [dir][maxlev + 2]: staging entries per consumer level
   const int2* __restrict__ stage = nullptr;      // (LDS slot, element) copied from global memory
+  // Per-element variant of the LDS sweep: the sweep-order cell words replace the
+  // records, and the element tables are gathered by cell, c_f (k . n_f) recomputed.
+  const uint32_t* __restrict__ task = nullptr;   // [dir][pos]: cell | upstream mask << 24 | outflow mask << 28
+  const double4* __restrict__ geo = nullptr;     // [e][f]: (n_f, c_f)
+  const int4* __restrict__ enbr = nullptr;
+  const uint32_t* __restrict__ eperm = nullptr;
+  const int32_t* __restrict__ emat = nullptr;
+  const double* __restrict__ kdir = nullptr;     // [dir][3]
 };
+
+// c_f (k . n_f) of one face.  Contraction is off so that every kernel that
+// computes it rounds the same four products and two sums, whatever the compiler
+// would fuse in its surroundings: the sweep variants stay bit-identical.
+__device__ __forceinline__ double face_ck(const double4& gf, double kx, double ky, double kz) {
+#pragma clang fp contract(off)
+  return gf.w * (kx * gf.x + ky * gf.y + kz * gf.z);
+}
 
 // LDS of the direction-resident LDS sweep: trace slots laid out [slot][node][group].
 // Which cell writes which slot, and where each upstream trace is read, is fixed
@@ -108,14 +125,29 @@ struct CellPrep {
   int4 us;
 };
 
-template <bool NT, bool LDS = false>
+template <bool NT, bool LDS = false, bool PE = false>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   const int64_t k = (int64_t)i * A.NE + s;
-  const TaskRec r = load_rec<NT>(A.rec + k);
   if constexpr (LDS) {
     P.ws = A.wslot[k];
     P.us = A.uslot[k];
   }
+  if constexpr (PE) {
+    const uint32_t w = A.task[k];
+    const int e = int(w & 0xFFFFFFu);
+    const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) P.ck[f] = face_ck(A.geo[(int64_t)e * 4 + f], kx, ky, kz);
+    P.w = w;
+    P.nb = A.enbr[e];
+    P.pm = A.eperm[e];
+    P.sig = A.sigt[(int64_t)A.emat[e] * A.Gtot + A.g0 + g];
+    const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) P.b[a] = A.q[base + a * A.G];
+    return;
+  }
+  const TaskRec r = load_rec<NT>(A.rec + k);
   const double4 c = load_ck<NT>(A.ck + k);
   P.w = r.w;
   P.ck[0] = c.x;
@@ -331,11 +363,11 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   }
 }
 
-template <bool NT, bool LDS = false>
+template <bool NT, bool LDS = false, bool PE = false>
 __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g,
                                            const TraceRing& ring = TraceRing{nullptr}) {
   CellPrep P;
-  prep_cell<NT, LDS>(A, i, s, g, P);
+  prep_cell<NT, LDS, PE>(A, i, s, g, P);
   finish_cell<LDS>(A, i, P, g, ring);
 }
 
@@ -362,10 +394,7 @@ __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t N
   const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
   double v[4];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const double4 gf = geo[(int64_t)e * 4 + f];
-    v[f] = gf.w * (kx * gf.x + ky * gf.y + kz * gf.z);
-  }
+  for (int f = 0; f < 4; ++f) v[f] = face_ck(geo[(int64_t)e * 4 + f], kx, ky, kz);
   ck[k] = make_double4(v[0], v[1], v[2], v[3]);
 }
 
@@ -518,7 +547,7 @@ __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int
 // end re-read its last cell), which keeps the compiler's counter waits exact.
 // The staging copy for the next level (LDS sweep) is issued after the
 // look-ahead and written to LDS after the solves.
-template <int R, bool NT, bool LDS>
+template <int R, bool NT, bool LDS, bool PE>
 __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, int s0, int s1, int n0, int n1,
                                                      int cpr, int c, int g, bool lane_ok, const CellPrep (&P)[R],
                                                      CellPrep (&Pn)[R], const TraceRing& ring,
@@ -536,7 +565,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
     }
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS, PE>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
   const StageCopy sc = stage_load<LDS>(A, i, so, lnext);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -544,7 +573,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
     if (lane_ok && s < s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
   }
   if (lane_ok)
-    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
+    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS, PE>(A, i, s, g, ring);   // beyond R rounds
   stage_store<LDS>(ring, sc);
 }
 
@@ -553,7 +582,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
 // first of the next level; only the last chunk of a level is followed by the
 // barrier (cells of a level are independent) and issues the staging copy for
 // the next level.  Advances (l, s0, s1) and returns false after the last chunk.
-template <int R, bool NT, bool LDS>
+template <int R, bool NT, bool LDS, bool PE>
 __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                      int nl, int chunk, int cpr, int c, int g, bool lane_ok,
                                                      int& l, int& s0, int& s1, const CellPrep (&P)[R],
@@ -564,7 +593,8 @@ __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, 
   const bool more = ln < nl;
   const int n0 = !more ? s0 : (last ? off[ln] : s1);
   const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
-  level_upstream_first<R, NT, LDS>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so, last ? l + 1 : -1);
+  level_upstream_first<R, NT, LDS, PE>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so,
+                                       last ? l + 1 : -1);
   if (last) __syncthreads();
   l = ln;
   s0 = n0;
@@ -575,7 +605,7 @@ __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, 
 // Levels larger than R * cpr cells are walked in chunks of that size, so that
 // every cell's static data is fetched one chunk ahead (the short, wide
 // directions of a mesh have levels of thousands of cells).
-template <int BLOCK, int R, bool NT, bool LDS = false>
+template <int BLOCK, int R, bool NT, bool LDS = false, bool PE = false>
 __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                            int nl, TraceRing ring = TraceRing{nullptr},
                                                            const int32_t* __restrict__ so = nullptr) {
@@ -588,11 +618,11 @@ __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, i
   if (nl <= 0) return;
   int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS, PE>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
   while (true) {
-    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
+    if (!chunk_upstream_first<R, NT, LDS, PE>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
       break;
-    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
+    if (!chunk_upstream_first<R, NT, LDS, PE>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
       break;
   }
 }
@@ -632,7 +662,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
 // upstream traces come from LDS where the plan holds them, otherwise from global
 // memory.  The flux is written to both; the workgroup barrier between levels
 // orders the LDS writes and reads.
-template <int BLOCK, bool NT>
+template <int BLOCK, bool NT, bool PE = false>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
                                                                     const int32_t* __restrict__ nlev, int maxlev,
                                                                     SweepArgs A) {
@@ -643,7 +673,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
                                                             TraceRing{ring_buf}, so);
   else
-    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true>(
+    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true, PE>(
         A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf}, so);
 }
 
@@ -990,12 +1020,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// A kernel given a gate (state word, mask) runs only when bit *state of mask is
+// set: the device-side Arnoldi step below switches its optional second
+// Gram-Schmidt pass on and off this way, without a host round trip.
+__device__ __forceinline__ bool gate_closed(const int* state, int mask) {
+  return state != nullptr && ((mask >> *state) & 1) == 0;
+}
+
 // part[(slot0 + i) kOrthoBlocks + block] = partial <w, V_i> for i < c; with
 // self_slot >= 0 also part[self_slot kOrthoBlocks + block] = partial <w, w>.
 __global__ __launch_bounds__(kBlock) void multi_dot_partial_kernel(int64_t n, int c, const double* __restrict__ V,
                                                                    int64_t ld, const double* __restrict__ w,
                                                                    int slot0, int self_slot,
-                                                                   double* __restrict__ part) {
+                                                                   double* __restrict__ part,
+                                                                   const int* gate, int mask) {
+  if (gate_closed(gate, mask)) return;
   __shared__ double sh[kOrthoChunk + 1][kBlock / 64];
   double acc[kOrthoChunk + 1];
 #pragma unroll
@@ -1026,7 +1065,9 @@ __global__ __launch_bounds__(kBlock) void multi_dot_partial_kernel(int64_t n, in
 
 // out[slot] = sum over blocks of part[slot kOrthoBlocks + block]; one block per slot.
 __global__ __launch_bounds__(kBlock) void multi_dot_final_kernel(const double* __restrict__ part,
-                                                                 double* __restrict__ out) {
+                                                                 double* __restrict__ out, const int* gate,
+                                                                 int mask) {
+  if (gate_closed(gate, mask)) return;
   __shared__ double sh[kBlock];
   double acc = 0.0;
   for (int j = threadIdx.x; j < kOrthoBlocks; j += kBlock) acc += part[(int64_t)blockIdx.x * kOrthoBlocks + j];
@@ -1048,6 +1089,126 @@ __global__ __launch_bounds__(kBlock) void multi_axpy_kernel(int64_t n, int c, co
       if (i < c) a -= h.h[i] * V[i * ld + j];
     w[j] = a * scale;
   }
+}
+
+// The same update with the coefficients (and the scale, when given) in device
+// memory, written there by the kernels below.
+__global__ __launch_bounds__(kBlock) void multi_axpy_dev_kernel(int64_t n, int c, const double* __restrict__ V,
+                                                                int64_t ld, const double* __restrict__ coef,
+                                                                const double* __restrict__ scale_p,
+                                                                const int* gate, int mask, double* __restrict__ w) {
+  if (gate_closed(gate, mask)) return;
+  double h[kOrthoChunk];
+#pragma unroll
+  for (int i = 0; i < kOrthoChunk; ++i) h[i] = i < c ? coef[i] : 0.0;
+  const double scale = scale_p ? *scale_p : 1.0;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    double a = w[j];
+#pragma unroll
+    for (int i = 0; i < kOrthoChunk; ++i)
+      if (i < c) a -= h[i] * V[i * ld + j];
+    w[j] = a * scale;
+  }
+}
+
+// One wave, one lane: the scalar work of an Arnoldi step is a few hundred flops
+// in a fixed order, done serially so that its bits never vary.
+
+// After the dots d[0..c) = <w, V_i>, d[c] = <w, w> of Gram-Schmidt pass `pass`:
+// accumulate the projections in hk, stage them as the update's coefficients and
+// decide, by the test of orthonormalize() (more than half of ||w||^2 left), what
+// comes next.  Pass 1 runs only when pass 0 asked for it.
+__global__ void ortho_decide_kernel(const double* __restrict__ d, int c, int pass, double* __restrict__ hk,
+                                    double* __restrict__ coef, ArnoldiCtl* __restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  if (pass == 1 && ctl->state != kArnPass2) return;
+  double hh = 0.0;
+  for (int i = 0; i < c; ++i) hh += d[i] * d[i];
+  const double ww = d[c];
+  if (!isfinite(ww) || !isfinite(hh)) {
+    ctl->nu = __builtin_nan("");
+    ctl->scale = 1.0;
+    ctl->state = kArnNaN;
+    return;
+  }
+  for (int i = 0; i < c; ++i) {
+    hk[i] = pass == 0 ? d[i] : hk[i] + d[i];
+    coef[i] = d[i];
+  }
+  const double rem = ww - hh;
+  if (rem > 0.5 * ww) {
+    ctl->nu = sqrt(rem);
+    ctl->scale = 1.0 / ctl->nu;
+    ctl->state = pass == 0 ? kArnDone1 : kArnDone2;
+  } else if (pass == 0) {
+    ctl->scale = 1.0;
+    ctl->state = kArnPass2;
+  } else {
+    ctl->nu = 0.0;                        // w lies in span V to working precision
+    ctl->state = kArnInvariant;
+  }
+}
+
+// Givens rotation of Golub & van Loan as SPARSKIT's givens (iters.f:3225-3260).
+__device__ void givens_dev(double x, double y, double* c, double* s) {
+  if (x == 0.0 && y == 0.0) {
+    *c = 1.0;
+    *s = 0.0;
+  } else if (fabs(y) > fabs(x)) {
+    const double t = x / y;
+    *s = copysign(1.0 / sqrt(1.0 + t * t), y);
+    *c = t * *s;
+  } else {
+    const double t = y / x;
+    *c = copysign(1.0 / sqrt(1.0 + t * t), x);
+    *s = t * *c;
+  }
+}
+
+// Column k of the Hessenberg matrix from the projections hk and the norm nu
+// (shifted: of I - K when the basis was built on K), the earlier rotations
+// applied to it, its own rotation, and the rotated right-hand side g, whose
+// entry k + 1 is the residual estimate (iters.f:2059-2094).
+__global__ void arnoldi_givens_kernel(double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                                      double* __restrict__ g, const double* __restrict__ hk, int k, int ldh,
+                                      int shifted, ArnoldiCtl* __restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  if (ctl->state == kArnNaN) {
+    ctl->resn = __builtin_nan("");
+    return;
+  }
+  const double nu = ctl->nu;
+  double* col = H + (int64_t)k * ldh;
+  for (int i = 0; i <= k; ++i) col[i] = shifted ? (i == k ? 1.0 : 0.0) - hk[i] : hk[i];
+  col[k + 1] = shifted ? -nu : nu;
+  for (int i = 0; i < k; ++i) {
+    const double a = col[i], b = col[i + 1];
+    col[i] = cs[i] * a + sn[i] * b;
+    col[i + 1] = cs[i] * b - sn[i] * a;
+  }
+  givens_dev(col[k], col[k + 1], &cs[k], &sn[k]);
+  col[k] = cs[k] * col[k] + sn[k] * col[k + 1];
+  col[k + 1] = 0.0;
+  g[k + 1] = -sn[k] * g[k];
+  g[k] = cs[k] * g[k];
+  ctl->resn = fabs(g[k + 1]);
+}
+
+__global__ void arnoldi_start_kernel(double* __restrict__ g, int m, double beta) {
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = i == 0 ? beta : 0.0;
+}
+
+// y = -R^-1 g over the first k columns (back substitution), the coefficients of
+// the update x <- x - (-V y).
+__global__ void arnoldi_solve_kernel(const double* __restrict__ H, const double* __restrict__ g, int k, int ldh,
+                                     double* __restrict__ y) {
+  if (threadIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; --i) {
+    double a = g[i];
+    for (int j = i + 1; j < k; ++j) a -= H[(int64_t)j * ldh + i] * y[j];
+    y[i] = H[(int64_t)i * ldh + i] != 0.0 ? a / H[(int64_t)i * ldh + i] : 0.0;
+  }
+  for (int i = 0; i < k; ++i) y[i] = -y[i];
 }
 
 __global__ __launch_bounds__(kBlock) void fluence_kernel(const double* __restrict__ w, int NI, int64_t NENK,
@@ -1094,19 +1255,22 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
     launch_direction_nt<false>(block, ndir, st, lo, nl, maxlev, A);
 }
 
-template <bool NT>
+template <bool NT, bool PE = false>
 static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
                                    int maxlev, const SweepArgs& A, size_t bytes) {
-  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
-                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
-                                  : (const void*)&sweep_direction_lds_kernel<1024, NT>;
+  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT, PE>
+                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT, PE>
+                                  : (const void*)&sweep_direction_lds_kernel<1024, NT, PE>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT, PE>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev,
+                       A);
   else if (block == 512)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT, PE>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev,
+                       A);
   else
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT, PE>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev,
+                       A);
   return LB_OK;
 }
 
@@ -1252,7 +1416,8 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
 // One sweep issued in a fixed launch configuration.
 static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
   const int mode = cfg.mode, block = cfg.block;
-  if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
+  const bool lds = mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_DIRECTION_LDS_ELEMENT;
+  if ((mode == LB_SWEEP_DIRECTION || lds || mode == LB_SWEEP_PERSISTENT) &&
       !((block == 256 || block == 512 || block == 1024) && block / G >= 1)) {
     set_error("sweep: the workgroup size must be 256, 512 or 1024 and hold the group batch");
     return LB_E_ARG;
@@ -1289,7 +1454,11 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
               op->G, g0, G, op->NE, q, psi};
   A.vec1 = one_group_vectors(G, psi);
   const bool nt = G >= kStreamGroups;
-  if (mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS) {
+  if (mode == LB_SWEEP_DIRECTION_LDS_ELEMENT && nt) {
+    set_error("sweep: the per-element LDS sweep takes fewer than 8 groups per cell");
+    return LB_E_ARG;
+  }
+  if (mode == LB_SWEEP_DIRECTION || lds) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
       LB_HIP(hipMemcpy(op->d_lvl_off.p, op->h_lvl_off.data(), op->h_lvl_off.size() * sizeof(int32_t),
@@ -1299,7 +1468,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     }
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
-    if (mode == LB_SWEEP_DIRECTION_LDS) {
+    if (lds) {
       int rc = LB_OK;
       const TracePlan* plan = trace_plan(op, G, &rc);
       if (!plan) return rc;
@@ -1309,7 +1478,16 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       A.stage_off = plan->d_stage_off.as<const int32_t>();
       A.stage = plan->d_stage.as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
-      if (nt)
+      if (mode == LB_SWEEP_DIRECTION_LDS_ELEMENT) {
+        A.task = op->d_tasks.as<const uint32_t>();
+        A.geo = op->d_geo.as<const double4>();
+        A.enbr = op->d_nbr.as<const int4>();
+        A.eperm = op->d_perm.as<const uint32_t>();
+        A.emat = op->d_mat.as<const int32_t>();
+        A.kdir = op->d_k.as<const double>();
+        LB_TRY((launch_direction_lds_nt<false, true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A,
+                                                      bytes)));
+      } else if (nt)
         LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
       else
         LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
@@ -1648,9 +1826,9 @@ int multi_dot(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const do
   for (int i0 = 0; i0 < c || (i0 == 0 && self); i0 += kOrthoChunk) {
     const int cc = std::min(kOrthoChunk, c - i0);
     hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(kOrthoBlocks), dim3(kBlock), 0, op->stream, n, cc,
-                       V + (int64_t)i0 * ld, ld, w, i0, (self && i0 == 0) ? c : -1, part);
+                       V + (int64_t)i0 * ld, ld, w, i0, (self && i0 == 0) ? c : -1, part, nullptr, 0);
   }
-  hipLaunchKernelGGL(multi_dot_final_kernel, dim3(slots), dim3(kBlock), 0, op->stream, part, res);
+  hipLaunchKernelGGL(multi_dot_final_kernel, dim3(slots), dim3(kBlock), 0, op->stream, part, res, nullptr, 0);
   LB_HIP(hipGetLastError());
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, slots * sizeof(double), hipMemcpyDeviceToHost, op->stream));
   LB_HIP(hipStreamSynchronize(op->stream));
@@ -1669,6 +1847,112 @@ int multi_axpy(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const d
   }
   LB_HIP(hipGetLastError());
   return LB_OK;
+}
+
+namespace {
+
+// res[i] = <w, V_i> (i < c), res[c] = <w, w>, left in device memory; the
+// kernels run only when the gate is open.
+int multi_dot_enqueue(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* w, double* res,
+                      const int* gate, int mask) {
+  const int slots = c + 1;
+  LB_TRY(op->w_red.reserve((size_t)slots * kOrthoBlocks * sizeof(double)));
+  double* part = op->w_red.as<double>();
+  for (int i0 = 0; i0 < c || i0 == 0; i0 += kOrthoChunk) {
+    const int cc = std::min(kOrthoChunk, c - i0);
+    hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(kOrthoBlocks), dim3(kBlock), 0, op->stream, n, cc,
+                       V + (int64_t)i0 * ld, ld, w, i0, i0 == 0 ? c : -1, part, gate, mask);
+  }
+  hipLaunchKernelGGL(multi_dot_final_kernel, dim3(slots), dim3(kBlock), 0, op->stream, part, res, gate, mask);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+// w <- (w - sum_{i<c} coef_i V_i) * (*scale_p, or 1), coefficients in device memory.
+int multi_axpy_enqueue(lb_op* op, int64_t n, const double* V, int64_t ld, int c, const double* coef,
+                       const double* scale_p, const int* gate, int mask, double* w) {
+  for (int i0 = 0; i0 < c; i0 += kOrthoChunk) {
+    const int cc = std::min(kOrthoChunk, c - i0);
+    hipLaunchKernelGGL(multi_axpy_dev_kernel, dim3(grid_stride(n)), dim3(kBlock), 0, op->stream, n, cc,
+                       V + (int64_t)i0 * ld, ld, coef + i0, i0 + cc == c ? scale_p : nullptr, gate, mask, w);
+  }
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+}  // namespace
+
+int arnoldi_reserve(lb_op* op, int m, ArnoldiDev* a) {
+  if (m < 1) {
+    set_error("arnoldi: restart length must be positive");
+    return LB_E_ARG;
+  }
+  const size_t ldh = (size_t)m + 1;
+  const size_t doubles = ldh * m + m + m + ldh + ldh + (ldh + 1) + ldh + m;
+  LB_TRY(op->w_arn.reserve(sizeof(ArnoldiCtl) + doubles * sizeof(double)));
+  a->m = m;
+  a->ctl = op->w_arn.as<ArnoldiCtl>();
+  double* p = reinterpret_cast<double*>(a->ctl + 1);
+  a->H = p;
+  p += ldh * m;
+  a->cs = p;
+  p += m;
+  a->sn = p;
+  p += m;
+  a->g = p;
+  p += ldh;
+  a->hk = p;
+  p += ldh;
+  a->d = p;
+  p += ldh + 1;
+  a->coef = p;
+  p += ldh;
+  a->y = p;
+  if (!op->w_host.p || op->w_host.bytes < sizeof(ArnoldiCtl)) {
+    if (op->w_host.p) (void)hipHostFree(op->w_host.p);
+    op->w_host.p = nullptr;
+    const size_t want = std::max<size_t>(sizeof(ArnoldiCtl), kMaxGroupBatch * sizeof(double));
+    LB_HIP(hipHostMalloc(&op->w_host.p, want, hipHostMallocDefault));
+    op->w_host.bytes = want;
+  }
+  return LB_OK;
+}
+
+int arnoldi_start(lb_op* op, const ArnoldiDev& a, double beta) {
+  hipLaunchKernelGGL(arnoldi_start_kernel, dim3(1), dim3(64), 0, op->stream, a.g, a.m, beta);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int arnoldi_step(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev& a, bool shifted, ArnoldiCtl* out) {
+  if (k < 0 || k >= a.m) {
+    set_error("arnoldi: step outside the cycle");
+    return LB_E_ARG;
+  }
+  hipStream_t s = op->stream;
+  const int c = k + 1;
+  double* w = const_cast<double*>(V) + (int64_t)c * n;
+  const int* st = &a.ctl->state;
+  LB_TRY(multi_dot_enqueue(op, n, V, n, c, w, a.d, nullptr, 0));
+  hipLaunchKernelGGL(ortho_decide_kernel, dim3(1), dim3(64), 0, s, a.d, c, 0, a.hk, a.coef, a.ctl);
+  LB_TRY(multi_axpy_enqueue(op, n, V, n, c, a.coef, &a.ctl->scale, st, (1 << kArnDone1) | (1 << kArnPass2), w));
+  LB_TRY(multi_dot_enqueue(op, n, V, n, c, w, a.d, st, 1 << kArnPass2));
+  hipLaunchKernelGGL(ortho_decide_kernel, dim3(1), dim3(64), 0, s, a.d, c, 1, a.hk, a.coef, a.ctl);
+  LB_TRY(multi_axpy_enqueue(op, n, V, n, c, a.coef, &a.ctl->scale, st, 1 << kArnDone2, w));
+  hipLaunchKernelGGL(arnoldi_givens_kernel, dim3(1), dim3(64), 0, s, a.H, a.cs, a.sn, a.g, a.hk, k, a.m + 1,
+                     shifted ? 1 : 0, a.ctl);
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipMemcpyAsync(op->w_host.p, a.ctl, sizeof(ArnoldiCtl), hipMemcpyDeviceToHost, s));
+  LB_HIP(hipStreamSynchronize(s));
+  std::memcpy(out, op->w_host.p, sizeof(ArnoldiCtl));
+  return LB_OK;
+}
+
+int arnoldi_update(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev& a, double* sol) {
+  if (k == 0) return LB_OK;
+  hipLaunchKernelGGL(arnoldi_solve_kernel, dim3(1), dim3(64), 0, op->stream, a.H, a.g, k, a.m + 1, a.y);
+  LB_HIP(hipGetLastError());
+  return multi_axpy_enqueue(op, n, V, n, k, a.y, nullptr, nullptr, 0, sol);
 }
 
 int launch_fluence(lb_op* op, const double* v, double* phi) {

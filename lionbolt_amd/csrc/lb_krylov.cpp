@@ -269,17 +269,20 @@ using ApplyK = std::function<int(const double* in, double* out)>;
 // on w = K v_j, whose orthogonalisation does not cancel the identity part:
 // if K v_j = sum_{i<=j} h_ij v_i + nu v_{j+1}, then
 // (I - K) v_j = sum_i (delta_ij - h_ij) v_i - nu v_{j+1}, the Hessenberg matrix
-// of I - K.  Orthogonalisation by orthonormalize() above.  Iterate counting,
-// the stopping rule ||r|| <= rtol ||rhs|| + atol, Givens rotations and the
-// explicit residual at every restart as gmres() above.  When the basis stops
-// growing (nu = 0), the solution is updated and the explicit residual decides:
-// converged, another cycle, or LB_E_BREAKDOWN if a cycle made no progress.
+// of I - K.  Orthogonalisation as orthonormalize() above; the Hessenberg matrix,
+// the Givens rotations, the residual estimate and the final triangular solve
+// stay in device memory (arnoldi_step), so an Arnoldi step costs one host
+// synchronisation, at its end, for the stopping test.  Iterate counting, the
+// stopping rule ||r|| <= rtol ||rhs|| + atol and the explicit residual at every
+// restart as gmres() above.  When the basis stops growing (nu = 0), the solution
+// is updated and the explicit residual decides: converged, another cycle, or
+// LB_E_BREAKDOWN if a cycle made no progress.
 int gmres_shifted(lb_op* op, int64_t n, const ApplyK& applyK, const double* rhs, double* sol, double* V, int m,
                   double rtol, double atol, int maxit, int verbose, GmresResult* res) {
   hipStream_t s = op->stream;
   auto vec = [&](int j) { return V + (int64_t)j * n; };
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), c(m), sn(m), g(m + 1), hk(m + 1);
-  auto h = [&](int i, int k) -> double& { return H[(size_t)k * (m + 1) + i]; };
+  ArnoldiDev arn;
+  LB_TRY(arnoldi_reserve(op, m, &arn));
   int iter = 0, nmv = 0, status = 0;
   auto ret = [&](double v) {
     ++iter;
@@ -333,8 +336,7 @@ int gmres_shifted(lb_op* op, int64_t n, const ApplyK& applyK, const double* rhs,
       break;
     }
     LB_TRY(launch_scale(s, n, 1.0 / beta, v0));
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
+    LB_TRY(arnoldi_start(op, arn, beta));
     int k = 0;
     bool invariant = false;
     while (k < m) {
@@ -344,31 +346,19 @@ int gmres_shifted(lb_op* op, int64_t n, const ApplyK& applyK, const double* rhs,
       }
       LB_TRY(applyK(vec(k), vec(k + 1)));
       ++nmv;
-      double nu;
-      LB_TRY(orthonormalize(op, n, V, k + 1, vec(k + 1), hk.data(), &nu));
-      if (std::isnan(nu)) {
+      ArnoldiCtl st;
+      LB_TRY(arnoldi_step(op, n, V, k, arn, true, &st));
+      if (st.state == kArnNaN) {
         status = LB_E_NAN;
         break;
       }
-      for (int i = 0; i <= k; ++i) h(i, k) = (i == k ? 1.0 : 0.0) - hk[i];
-      h(k + 1, k) = -nu;
-      for (int i = 0; i < k; ++i) {
-        const double a = h(i, k), b = h(i + 1, k);
-        h(i, k) = c[i] * a + sn[i] * b;
-        h(i + 1, k) = c[i] * b - sn[i] * a;
-      }
-      givens(h(k, k), h(k + 1, k), &c[k], &sn[k]);
-      h(k, k) = c[k] * h(k, k) + sn[k] * h(k + 1, k);
-      h(k + 1, k) = 0.0;
-      g[k + 1] = -sn[k] * g[k];
-      g[k] = c[k] * g[k];
-      resn = std::fabs(g[k + 1]);
+      resn = st.resn;
       ++k;
       if (std::isnan(resn)) {
         status = LB_E_NAN;
         break;
       }
-      if (nu == 0.0) {
+      if (st.state == kArnInvariant) {
         invariant = true;
         break;
       }
@@ -376,14 +366,7 @@ int gmres_shifted(lb_op* op, int64_t n, const ApplyK& applyK, const double* rhs,
     }
     if (status != 0) break;
     // y = R^-1 g over the k columns, x += V y (one fused pass per 8 vectors)
-    std::vector<double> y(k, 0.0);
-    for (int i = k - 1; i >= 0; --i) {
-      double a = g[i];
-      for (int j = i + 1; j < k; ++j) a -= h(i, j) * y[j];
-      y[i] = h(i, i) != 0.0 ? a / h(i, i) : 0.0;
-    }
-    for (double& t : y) t = -t;
-    LB_TRY(multi_axpy(op, n, V, n, k, y.data(), 1.0, sol));
+    LB_TRY(arnoldi_update(op, n, V, k, arn, sol));
     if (invariant) {
       last_beta = beta;                // the explicit residual decides
       continue;

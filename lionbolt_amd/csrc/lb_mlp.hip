@@ -84,6 +84,18 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs F, float* __rest
 
 __device__ __forceinline__ float act(float x) { return tanhf(x); }
 
+// tanh x = 1 - 2 / (e^{2x} + 1) from the hardware's base-2 exponential and
+// reciprocal (e^{2x} = 2^{2x / ln 2}): within a few 1e-7 of tanhf in absolute
+// terms (near 0 the subtraction cancels, so the error is absolute, not
+// relative), far below the BF16 rounding (2^-9 of the largest activations) of
+// the values it feeds.  Saturates to +-1 for large |x| (the exponential
+// overflows to inf or underflows to 0).  The BF16 kernel only: training and the
+// FP32 path keep tanhf.
+__device__ __forceinline__ float act_bf16(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
 // FP32 reference inference: one thread per dof.
 __global__ __launch_bounds__(256) void mlp_fp32_kernel(FeatArgs F, const float* __restrict__ W, double s,
                                                        double* __restrict__ z) {
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
 #pragma unroll
     for (int st = 0; st < 2; ++st)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act_bf16(a[2 * st + (q >> 2)][q & 3]);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       a[t] = f32x4{b2[t][0], b2[t][1], b2[t][2], b2[t][3]};
@@ -184,7 +196,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
 #pragma unroll
     for (int st = 0; st < 2; ++st)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act_bf16(a[2 * st + (q >> 2)][q & 3]);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       a[t] = f32x4{b3[t][0], b3[t][1], b3[t][2], b3[t][3]};
@@ -194,7 +206,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(FeatArgs F, const __bf16*
 #pragma unroll
     for (int st = 0; st < 2; ++st)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act(a[2 * st + (q >> 2)][q & 3]);
+      for (int q = 0; q < 8; ++q) hb[st][q] = (__bf16)act_bf16(a[2 * st + (q >> 2)][q & 3]);
     f32x4 o = f32x4{b4[0], b4[1], b4[2], b4[3]};
     o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[0], hb[0], o, 0, 0, 0);
     o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[1], hb[1], o, 0, 0, 0);
@@ -304,14 +316,27 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(GemmArgs g) {
   g.c[(int64_t)i * g.ldc + j] = gemm_epilogue(g, i, j, v);
 }
 
-// db[j] = sum_b dout[b][j], b in order.
-__global__ void bias_grad_kernel(int B, int J, const float* __restrict__ dout, float* __restrict__ db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= J) return;
+// db[j] = sum_b dout[b][j] (J divides 1024): thread t sums column t % J over rows
+// t / J, t / J + S, ... (S = 1024 / J slices, rows in order), then the slices are
+// added pairwise in a fixed tree, so the sum's bits do not vary from run to run.
+constexpr int kBiasThreads = 1024;
+__global__ __launch_bounds__(kBiasThreads) void bias_grad_kernel(int B, int J, const float* __restrict__ dout,
+                                                                 float* __restrict__ db) {
+  __shared__ float sh[kBiasThreads];
+  const int t = threadIdx.x, S = kBiasThreads / J;
+  const int j = t % J, r = t / J;
   float a = 0.f;
-  for (int b = 0; b < B; ++b) a += dout[(int64_t)b * J + j];
-  db[j] = a;
+  for (int b = r; b < B; b += S) a += dout[(int64_t)b * J + j];
+  sh[t] = a;
+  __syncthreads();
+  for (int h = S / 2; h > 0; h >>= 1) {
+    if (r < h) sh[t] += sh[t + h * J];
+    __syncthreads();
+  }
+  if (r == 0) db[j] = sh[t];
 }
+static_assert(kBiasThreads % kH == 0 && kBiasThreads % kOut == 0 && (kH & (kH - 1)) == 0 && (kOut & (kOut - 1)) == 0,
+              "bias-gradient slices must be a power of two");
 
 // ---- training kernels (FP32) ------------------------------------------------
 
@@ -431,22 +456,22 @@ void enqueue_step(hipStream_t st, int B, const float* X, const float* Y, const i
   // backward: dW = in^T d (k = minibatch rows), db = column sums of d, d_in = (d W^T) (1 - h^2)
   launch_gemm(st, gemm_args(kH, kOut, B, t.h3, nullptr, nullptr, 1, kH, t.dy, kOut, 1, g + kW4, kOut, kPlain, 0,
                             nullptr, nullptr, sp, t.part));
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kOut, t.dy, g + kB4);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(kBiasThreads), 0, st, B, kOut, t.dy, g + kB4);
   launch_gemm(st, gemm_args(B, kH, kOut, t.dy, nullptr, nullptr, kOut, 1, w + kW4, 1, kOut, t.d3, kH, kTanhGrad, 0,
                             nullptr, t.h3, 1, nullptr));
   launch_gemm(st, gemm_args(kH, kH, B, t.h2, nullptr, nullptr, 1, kH, t.d3, kH, 1, g + kW3, kH, kPlain, 0, nullptr,
                             nullptr, sp, t.part));
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d3, g + kB3);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(kBiasThreads), 0, st, B, kH, t.d3, g + kB3);
   launch_gemm(st, gemm_args(B, kH, kH, t.d3, nullptr, nullptr, kH, 1, w + kW3, 1, kH, t.d2, kH, kTanhGrad, 0, nullptr,
                             t.h2, 1, nullptr));
   launch_gemm(st, gemm_args(kH, kH, B, t.h1, nullptr, nullptr, 1, kH, t.d2, kH, 1, g + kW2, kH, kPlain, 0, nullptr,
                             nullptr, sp, t.part));
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d2, g + kB2);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(kBiasThreads), 0, st, B, kH, t.d2, g + kB2);
   launch_gemm(st, gemm_args(B, kH, kH, t.d2, nullptr, nullptr, kH, 1, w + kW2, 1, kH, t.d1, kH, kTanhGrad, 0, nullptr,
                             t.h1, 1, nullptr));
   launch_gemm(st, gemm_args(kIn, kH, B, X, nullptr, idx, 1, kIn, t.d1, kH, 1, g + kW1, kH, kPlain, 0, nullptr,
                             nullptr, sp, t.part));
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, st, B, kH, t.d1, g + kB1);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(kBiasThreads), 0, st, B, kH, t.d1, g + kB1);
 }
 
 // Carves the step buffers out of `f` (floats): (3 kH + kOut) B activations,

@@ -124,7 +124,8 @@ class Operator:
 
     _SWEEP_MODES = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION,
                     "auto": _lib.SWEEP_AUTO, "persistent": _lib.SWEEP_PERSISTENT,
-                    "direction_lds": _lib.SWEEP_DIRECTION_LDS}
+                    "direction_lds": _lib.SWEEP_DIRECTION_LDS,
+                    "direction_lds_element": _lib.SWEEP_DIRECTION_LDS_ELEMENT}
 
     def set_sweep_mode(self, mode):
         _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, self._SWEEP_MODES[mode]))

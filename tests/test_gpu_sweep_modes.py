@@ -24,13 +24,16 @@ def test_direction_resident_equals_level_launches(G, block):
     op.set_direction_block(block)
     q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
     res = {}
-    for mode in ("levels", "direction", "direction", "graph", "direction_lds", "direction_lds"):
+    modes = ["levels", "direction", "direction", "graph", "direction_lds", "direction_lds"]
+    if G < 8:                                   # the per-element variant takes fewer than 8 groups per cell
+        modes += ["direction_lds_element"] * 2
+    for mode in modes:
         op.set_sweep_mode(mode)
         psi = torch.full_like(q, float("nan"))
         op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         res.setdefault(mode, []).append(psi.cpu().numpy())
-    for mode in ("direction", "graph", "direction_lds"):
+    for mode in set(modes) - {"levels"}:
         for v in res[mode]:
             assert np.array_equal(v, res["levels"][0]), mode
 
@@ -53,7 +56,7 @@ def test_lds_slots_with_staging_and_global_reads(G):
     op.set_physics_groups(sigt, np.zeros((G, 2, 1)))
     q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
     out = {}
-    for mode in ("levels", "direction_lds"):
+    for mode in ("levels", "direction_lds", "direction_lds_element"):
         op.set_sweep_mode(mode)
         for block in (256, 512):
             op.set_direction_block(block)
@@ -63,6 +66,7 @@ def test_lds_slots_with_staging_and_global_reads(G):
             out[(mode, block)] = psi.cpu().numpy()
     for block in (256, 512):
         assert np.array_equal(out[("direction_lds", block)], out[("levels", 256)])
+        assert np.array_equal(out[("direction_lds_element", block)], out[("levels", 256)])
 
 
 def test_direction_resident_in_place_and_solver():
