@@ -161,12 +161,32 @@ def solve_leg(m, ang, rtol, group_batch=1):
     s4pi = math.sqrt(4.0 * math.pi)
     flux = {"collided_sum_all_groups": float(phi[:, 0].sum()) * s4pi,
             "uncollided_sum_group1": float(umom[0].sum()) * s4pi}
+    reduced = {}
+    if group_batch == 1 and ang.NI == 256 and ang.L == 3:
+        # D psi of the Arnoldi steps on MFMA with reduced inputs (FP32 accumulation);
+        # right-hand side, psi and the true residual in FP64
+        ref = phi[:, 0].clone()
+        for fmt in ("split_bf16", "bf16"):
+            op.set_moment_precision(fmt)
+            phr = torch.zeros_like(qmom)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            sr = op.multigroup_solve_dev(S, phr, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol,
+                                         maxit=60, raise_on_fail=False)
+            torch.cuda.synchronize()
+            reduced[fmt] = {"seconds": time.perf_counter() - t1, "iterates": [x["iterates"] for x in sr],
+                            "sweeps": sum(x["sweeps"] for x in sr),
+                            "max_true_residual": max(x["true_residual"] for x in sr),
+                            "groups_not_converged": sum(x["status"] != 0 for x in sr),
+                            "scalar_flux_max_rel_diff_vs_fp64":
+                                float(((phr[:, 0] - ref).abs().amax(dim=1) / ref.abs().amax(dim=1)).max())}
+        op.set_moment_precision("fp64")
     op.close()
     max_res = max(s["true_residual"] for s in stats)
     return {"seconds": wall, "groups": G, "rtol": rtol, "autotune_s": tune_s, "sweep_choice": choice,
             "max_true_residual": max_res, "target_residual": 1e-6, "target_met": bool(max_res <= 1e-6),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
-            "group_batch": group_batch, "scalar_flux": flux,
+            "group_batch": group_batch, "scalar_flux": flux, "reduced_precision_D_psi": reduced,
             "method": ("plain GMRES(30) on flux moments per group, FP64" if group_batch == 1 else
                        f"plain GMRES(30) on flux moments, {group_batch} groups per coupled system, FP64")}
 
@@ -441,7 +461,7 @@ def single_group_leg(m, ang, reps=10):
     psi = torch.empty_like(q)
     res = {}
     for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "direction_lds-256",
-                 "direction_lds-512", "direction_lds_element-256", "direction_lds_element-512", "auto"):
+                 "direction_lds-512", "auto"):
         if mode == "auto":
             op.set_sweep_mode("auto")
             for _ in range(32):
@@ -498,7 +518,7 @@ def run(args):
             op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         mode, block, _ = op.sweep_choice(G)
-    elif args.sweep_mode in ("direction", "direction_lds", "direction_lds_element"):
+    elif args.sweep_mode in ("direction", "direction_lds"):
         block = args.block
         op.set_direction_block(block)
     for _ in range(args.warmup):
@@ -636,7 +656,7 @@ def main():
                    help="skip the learned-preconditioner solve")
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto",
-                   choices=["auto", "levels", "graph", "direction", "direction_lds", "direction_lds_element"])
+                   choices=["auto", "levels", "graph", "direction", "direction_lds"])
     p.add_argument("--block", type=int, default=1024, choices=[256, 512, 1024],
                    help="workgroup size of the direction-resident sweep modes")
     p.add_argument("--compare-modes", action="store_true")

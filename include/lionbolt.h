@@ -51,8 +51,7 @@ enum {
   LB_SWEEP_DIRECTION = 2,
   LB_SWEEP_AUTO = 3,
   LB_SWEEP_PERSISTENT = 4,
-  LB_SWEEP_DIRECTION_LDS = 5,
-  LB_SWEEP_DIRECTION_LDS_ELEMENT = 6
+  LB_SWEEP_DIRECTION_LDS = 5
 };
 
 typedef struct lb_op lb_op;
@@ -71,6 +70,10 @@ typedef struct {
 
 /* Most elements of one operator: sweep records keep a cell index in 24 bits. */
 #define LB_MAX_ELEMENTS 16777215
+
+/* Most materials of one operator (material indices 1..256): sweep records keep
+ * the material in 8 bits. */
+#define LB_MAX_MATERIALS 256
 
 /* Highest scattering order: PNSCATTERING 15, the highest the reference's
  * input decks use (validation/electrons_incident/Au_1MeV_0deg/lb.in and its siblings). */
@@ -133,10 +136,7 @@ int lb_op_set_scatter_mode(lb_op* op, int mode);
  * LB_SWEEP_DIRECTION_LDS does the same and also keeps upstream traces in LDS
  * slots fixed by a slot plan built on first use per group-batch size (ring
  * slots by Belady's rule, staging copies one level ahead, global memory for
- * the rest; see lb_trace_plan_host); LB_SWEEP_DIRECTION_LDS_ELEMENT is the LDS
- * sweep reading per-element tables (neighbours, node permutation, material,
- * face normals) gathered by cell and recomputing c_f (k . n_f), instead of the
- * per-(direction, cell) records, below 8 groups per cell; LB_SWEEP_AUTO (default) times the graph
+ * the rest; see lb_trace_plan_host); LB_SWEEP_AUTO (default) times the graph
  * mode, the direction-resident mode at 256, 512 and 1024 threads and, below 8
  * groups per cell, the LDS one at 256 and 512 threads on the first sweeps of
  * each group-batch size (two sweeps each, synchronising after each) and then
@@ -186,6 +186,15 @@ int lb_op_set_preconditioner(lb_op* op, const float* params, int nparams, int fm
  * lb_solve_stats.fallback = -1).  Default 0 (every group uses it); +inf turns the
  * network off.  The caller derives the threshold from the training problems. */
 int lb_op_set_preconditioner_threshold(lb_op* op, double min_thickness);
+
+/* Input precision of the discrete-to-moment product D psi inside the Arnoldi
+ * steps of the flux-moment GMRES (K = D T^-1 M S): LB_FMT_FP64 (default, the
+ * FP64 kernels) or an MFMA format of lb_gemm_d_dev (psi packed column-major in
+ * the format, FP32 accumulation, the moments widened to FP64).  The right-hand
+ * side, psi and the true residual stay FP64.  Reduced formats need 256
+ * directions and P3 (16 moments).  No reference counterpart (the reference
+ * computes K in FP64). */
+int lb_op_set_moment_precision(lb_op* op, int fmt);
 
 /* d_z [16][NENK] = P(d_r [16][NENK]) for group g0 (single-group moment vectors). */
 int lb_mlp_apply_dev(lb_op* op, const double* d_r, double* d_z, int g0);
@@ -392,7 +401,7 @@ int lb_op_set_group_batch(lb_op* op, int groups);
  * LB_FMT_FP32, LB_FMT_BF16, LB_FMT_SPLIT_BF16 (x = hi + lo, three products),
  * LB_FMT_FP16 (power-of-two scale per column).  Reduced operands are stored with
  * the reduced dimension contiguous per column.  stream: a hipStream_t or NULL. */
-enum { LB_FMT_FP32 = 0, LB_FMT_BF16 = 1, LB_FMT_SPLIT_BF16 = 2, LB_FMT_FP16 = 3 };
+enum { LB_FMT_FP32 = 0, LB_FMT_BF16 = 1, LB_FMT_SPLIT_BF16 = 2, LB_FMT_FP16 = 3, LB_FMT_FP64 = 4 };
 
 /* src [rows][ncols] FP64 -> dst [ncols][rows] in the format (dst_lo: low part of
  * the split format; inv_scale[ncols]: FP16 column scales 2^-e, written here). */

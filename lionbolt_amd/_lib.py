@@ -16,14 +16,13 @@ SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
 SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT, SWEEP_DIRECTION_LDS = 0, 1, 2, 3, 4, 5
-SWEEP_DIRECTION_LDS_ELEMENT = 6
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)
 _i32 = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 _f = ctypes.POINTER(ctypes.c_float)
-FMT_FP32, FMT_BF16, FMT_SPLIT_BF16, FMT_FP16 = 0, 1, 2, 3
+FMT_FP32, FMT_BF16, FMT_SPLIT_BF16, FMT_FP16, FMT_FP64 = 0, 1, 2, 3, 4
 MLP_PARAMS = 10576
 
 
@@ -77,6 +76,7 @@ SIGNATURES = {
     "lb_mlp_apply_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_mlp_apply_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_op_set_preconditioner_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "lb_op_set_moment_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lb_beam_uncollided_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, _vp]),
     "lb_beam_uncollided_range_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, ctypes.c_int64, ctypes.c_int64,
                                                     _vp]),

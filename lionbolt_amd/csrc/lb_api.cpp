@@ -138,6 +138,12 @@ int lb_op_create(const lb_mesh_desc* mesh, const lb_quad_desc* quad, lb_op** out
                 "indices); split larger meshes into problems of one batch or several operators");
       return LB_E_ARG;
     }
+    for (int64_t e = 0; e < mesh->NE; ++e)
+      if (mesh->mat[e] - mesh->index_base >= LB_MAX_MATERIALS) {
+        set_error("lb_op_create: at most 256 materials per operator (the sweep records hold 8-bit material "
+                  "indices)");
+        return LB_E_ARG;
+      }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
       (void)hipGetLastError();
@@ -278,7 +284,7 @@ int lb_op_set_scatter_mode(lb_op* op, int mode) {
 
 int lb_op_set_sweep_mode(lb_op* op, int mode) {
   return guarded([&]() -> int {
-    if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_DIRECTION_LDS_ELEMENT) {
+    if (!op || mode < LB_SWEEP_LEVELS || mode > LB_SWEEP_DIRECTION_LDS) {
       set_error("lb_op_set_sweep_mode: invalid mode");
       return LB_E_ARG;
     }
@@ -413,6 +419,21 @@ int lb_op_set_preconditioner_threshold(lb_op* op, double min_thickness) {
       return LB_E_ARG;
     }
     op->mlp_min_tau = min_thickness;
+    return LB_OK;
+  });
+}
+
+int lb_op_set_moment_precision(lb_op* op, int fmt) {
+  return guarded([&]() -> int {
+    if (!op || fmt < LB_FMT_FP32 || fmt > LB_FMT_FP64) {
+      set_error("lb_op_set_moment_precision: needs an operator and an LB_FMT_* format");
+      return LB_E_ARG;
+    }
+    if (fmt != LB_FMT_FP64 && (op->NI != 256 || op->nmom != 16)) {
+      set_error("lb_op_set_moment_precision: reduced-precision products need 256 directions and P3");
+      return LB_E_ARG;
+    }
+    op->moment_fmt = fmt;
     return LB_OK;
   });
 }
@@ -718,12 +739,12 @@ void lb_op_destroy(lb_op* op) {
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);
   op->d_rec.release();
-  op->d_ck.release();
-  for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec, &op->batch.d_ck,
+  for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec,
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
     b->release();
-  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat, &op->d_vert, &op->w_beam}) b->release();
+  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat, &op->d_vert, &op->w_beam, &op->w_mix})
+    b->release();
   for (auto& kv : op->batch.seg) {
     kv.second.d_tiles.release();
     kv.second.d_tile_off.release();

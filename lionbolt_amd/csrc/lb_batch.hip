@@ -122,7 +122,6 @@ int set_problems(lb_op* op, int P, const int64_t* eoff) {
   clear_graphs(op);
   BatchTable& B = op->batch;
   B.d_rec.release();
-  B.d_ck.release();
   for (auto& kv : B.seg) {
     kv.second.d_tiles.release();
     kv.second.d_tile_off.release();

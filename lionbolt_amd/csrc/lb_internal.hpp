@@ -139,7 +139,7 @@ struct BatchTable {
   std::vector<int32_t> items;            // 4 per (problem, direction) item: p, i, level base, level count
   std::vector<int32_t> lvl;              // absolute level offsets (sweep positions in batch order)
   std::vector<uint32_t> tasks;           // [NI][NE] task words, problem-major inside each direction
-  DevBuf d_items, d_lvl, d_tasks, d_rec, d_ck, d_active, d_counter, d_coef, d_part, d_out;
+  DevBuf d_items, d_lvl, d_tasks, d_rec, d_active, d_counter, d_coef, d_part, d_out;
   double* h_io = nullptr;                // pinned host staging for per-problem scalars
   std::map<int, SegTable> seg;           // by rows
   int ncu = 0;
@@ -187,8 +187,8 @@ struct lb_op {
   int scatter_mode = LB_SCATTER_MOMENT;
   int sweep_mode = LB_SWEEP_AUTO;
   std::map<int, lb::SweepTune> tune;  // autotuner state per group-batch size
-  lb::DevBuf d_rec;                 // per (direction, sweep position): task word, material, neighbours
-  lb::DevBuf d_ck;                  // per (direction, sweep position): c_f (k . n_f) of the four faces
+  lb::DevBuf d_rec;                 // per (direction, sweep position): task word, permutation and material,
+                                    // neighbours, c_f (k . n_f) of three faces (48 B)
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
   std::map<int, lb::TracePlan> plans;  // LDS sweep slot plans per group-batch size (built on first use)
   int lds_max = 0;                  // LDS bytes a workgroup may allocate (queried once)
@@ -208,6 +208,8 @@ struct lb_op {
   bool mlp = false;
   int mlp_fmt = 1;                  // LB_FMT_BF16 (MFMA) or LB_FMT_FP32
   double mlp_min_tau = 0.0;         // groups optically thinner than this solve without the network
+  int moment_fmt = 4;               // LB_FMT_FP64, or the MFMA format of D psi in the Arnoldi steps
+  lb::DevBuf w_mix;                 // reduced-precision D psi: packed psi and FP32 moments
   lb::DevBuf d_mlp_w;               // FP32 parameters, natural layout
   lb::DevBuf d_mlp_pack;            // BF16 weights in MFMA lane order
   lb::DevBuf d_elem_h;              // [NE] element size (6 V)^(1/3)
@@ -306,6 +308,9 @@ int arnoldi_step(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev&
 // sol <- sol + V y, y = R^-1 g over the first k columns; no host synchronisation.
 int arnoldi_update(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev& a, double* sol);
 int launch_fluence(lb_op* op, const double* v, double* phi);
+// lb_mfma.hip: phi [nmom][NENK] = D psi of one group with the operator's reduced
+// input format (moment_fmt), FP32 accumulation, result widened to FP64.
+int launch_moments_reduced(lb_op* op, const double* psi, double* phi);
 
 // lb_groups.hip (group-batched multigroup solve)
 int launch_batch_source(lb_op* op, const double* phi_all, const double* S, const double* qmom, int g0, int B,

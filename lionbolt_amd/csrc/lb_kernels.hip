@@ -27,22 +27,23 @@ __device__ __forceinline__ bool in_face(int f, int a) {
   return a != 3 - f;
 }
 
-// Per-(direction, sweep position) record, in sweep order so that the lanes of one
-// level read consecutive records.
+// Per-(direction, sweep position) record, 48 bytes, in sweep order so that the
+// lanes of one level read consecutive records.  ck holds c_f (k . n_f) of faces
+// 0-2: every term of the local matrix and of the upstream coupling of a linear
+// tet is a multiple of one of the four (closed forms of
+// FEAInnerProducts.f90:311-407), and the fourth is minus the sum of the three,
+// since sum_f c_f n_f = (10 / detJ) sum_f A_f n_f = 0 over a closed tetrahedron.
 struct TaskRec {
   uint32_t w;      // cell | upstream mask << 24 | outflow mask << 28
-  int32_t mat;
-  uint32_t perm;   // neighbour local node of each face node, 2 bits each
-  uint32_t pad;
-  int4 nbr;        // neighbour across each local face, -1 on the boundary
+  uint32_t pm;     // neighbour local node of each face node, 2 bits each (24 bits) | material << 24
+  int32_t nb[4];   // neighbour across each local face, -1 on the boundary
+  double ck[3];    // c_f (k . n_f) of faces 0, 1, 2
 };
+static_assert(sizeof(TaskRec) == 48, "three 16-byte loads per record");
 
-// Arguments shared by the sweep kernels.  ck[dir][pos] holds c_f (k . n_f) for the
-// four faces: every term of the local matrix and of the upstream coupling of a
-// linear tet is a multiple of it (closed forms of FEAInnerProducts.f90:311-407).
+// Arguments shared by the sweep kernels.
 struct SweepArgs {
   const TaskRec* __restrict__ rec;
-  const double4* __restrict__ ck;
   const double* __restrict__ sigt;
   int Gtot, g0, G;
   int64_t NE;
@@ -50,23 +51,14 @@ struct SweepArgs {
   double* psi;
   int vec1 = 0;      // one group and a 16-byte-aligned psi: 16-byte flux stores
   // LDS sweep only: the slot plan of this group-batch size (lb_trace_plan.cpp)
-  const int4* __restrict__ uslot = nullptr;      // [dir][pos]: LDS slot of each upstream trace, -1 global
-  const int32_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
+  const short4* __restrict__ uslot = nullptr;    // [dir][pos]: LDS slot of each upstream trace, -1 global
+  const int16_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
   const int32_t* __restrict__ stage_off = nullptr;   // [dir][maxlev + 2]: staging entries per consumer level
   const int2* __restrict__ stage = nullptr;      // (LDS slot, element) copied from global memory
-  // Per-element variant of the LDS sweep: the sweep-order cell words replace the
-  // records, and the element tables are gathered by cell, c_f (k . n_f) recomputed.
-  const uint32_t* __restrict__ task = nullptr;   // [dir][pos]: cell | upstream mask << 24 | outflow mask << 28
-  const double4* __restrict__ geo = nullptr;     // [e][f]: (n_f, c_f)
-  const int4* __restrict__ enbr = nullptr;
-  const uint32_t* __restrict__ eperm = nullptr;
-  const int32_t* __restrict__ emat = nullptr;
-  const double* __restrict__ kdir = nullptr;     // [dir][3]
 };
 
-// c_f (k . n_f) of one face.  Contraction is off so that every kernel that
-// computes it rounds the same four products and two sums, whatever the compiler
-// would fuse in its surroundings: the sweep variants stay bit-identical.
+// c_f (k . n_f) of one face, with contraction off: the same four products and
+// two sums whatever the compiler would fuse in its surroundings.
 __device__ __forceinline__ double face_ck(const double4& gf, double kx, double ky, double kz) {
 #pragma clang fp contract(off)
   return gf.w * (kx * gf.x + ky * gf.y + kz * gf.z);
@@ -89,27 +81,8 @@ constexpr int kStreamGroups = 8;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <bool NT>
-__device__ __forceinline__ TaskRec load_rec(const TaskRec* p) {
-  if (!NT) return *p;
-  const u32x4* q = reinterpret_cast<const u32x4*>(p);
-  const u32x4 a = __builtin_nontemporal_load(q);
-  const u32x4 b = __builtin_nontemporal_load(q + 1);
-  TaskRec r;
-  r.w = a.x;
-  r.mat = (int32_t)a.y;
-  r.perm = a.z;
-  r.pad = a.w;
-  r.nbr = make_int4((int)b.x, (int)b.y, (int)b.z, (int)b.w);
-  return r;
-}
-
-template <bool NT>
-__device__ __forceinline__ double4 load_ck(const double4* p) {
-  if (!NT) return *p;
-  const double* d = reinterpret_cast<const double*>(p);
-  return make_double4(__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1),
-                      __builtin_nontemporal_load(d + 2), __builtin_nontemporal_load(d + 3));
+__device__ __forceinline__ double dbl_of(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // The part of a cell's solve that does not depend on upstream results.  Kernels
@@ -120,44 +93,39 @@ struct CellPrep {
   double sig;
   double b[4];
   int4 nb;
-  uint32_t pm;
+  uint32_t pm;  // node permutation (24 bits)
   int ws;     // LDS sweep only: slot this cell writes and slots of its upstream traces
   int4 us;
 };
 
-template <bool NT, bool LDS = false, bool PE = false>
+template <bool NT, bool LDS = false>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   const int64_t k = (int64_t)i * A.NE + s;
   if constexpr (LDS) {
     P.ws = A.wslot[k];
-    P.us = A.uslot[k];
+    const short4 us = A.uslot[k];
+    P.us = make_int4(us.x, us.y, us.z, us.w);
   }
-  if constexpr (PE) {
-    const uint32_t w = A.task[k];
-    const int e = int(w & 0xFFFFFFu);
-    const double kx = A.kdir[3 * i], ky = A.kdir[3 * i + 1], kz = A.kdir[3 * i + 2];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) P.ck[f] = face_ck(A.geo[(int64_t)e * 4 + f], kx, ky, kz);
-    P.w = w;
-    P.nb = A.enbr[e];
-    P.pm = A.eperm[e];
-    P.sig = A.sigt[(int64_t)A.emat[e] * A.Gtot + A.g0 + g];
-    const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) P.b[a] = A.q[base + a * A.G];
-    return;
+  const u32x4* q = reinterpret_cast<const u32x4*>(A.rec + k);
+  u32x4 a, b, c;
+  if constexpr (NT) {
+    a = __builtin_nontemporal_load(q);
+    b = __builtin_nontemporal_load(q + 1);
+    c = __builtin_nontemporal_load(q + 2);
+  } else {
+    a = q[0];
+    b = q[1];
+    c = q[2];
   }
-  const TaskRec r = load_rec<NT>(A.rec + k);
-  const double4 c = load_ck<NT>(A.ck + k);
-  P.w = r.w;
-  P.ck[0] = c.x;
-  P.ck[1] = c.y;
-  P.ck[2] = c.z;
-  P.ck[3] = c.w;
-  P.nb = r.nbr;
-  P.pm = r.perm;
-  P.sig = A.sigt[(int64_t)r.mat * A.Gtot + A.g0 + g];
-  const int e = int(r.w & 0xFFFFFFu);
+  P.w = a.x;
+  P.pm = a.y & 0xFFFFFFu;
+  P.nb = make_int4((int)a.z, (int)a.w, (int)b.x, (int)b.y);
+  P.ck[0] = dbl_of(b.z, b.w);
+  P.ck[1] = dbl_of(c.x, c.y);
+  P.ck[2] = dbl_of(c.z, c.w);
+  P.ck[3] = -((P.ck[0] + P.ck[1]) + P.ck[2]);
+  P.sig = A.sigt[(int64_t)(a.y >> 24) * A.Gtot + A.g0 + g];
+  const int e = int(P.w & 0xFFFFFFu);
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 #pragma unroll
   for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
@@ -363,22 +331,23 @@ __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const Cel
   }
 }
 
-template <bool NT, bool LDS = false, bool PE = false>
+template <bool NT, bool LDS = false>
 __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g,
                                            const TraceRing& ring = TraceRing{nullptr}) {
   CellPrep P;
-  prep_cell<NT, LDS, PE>(A, i, s, g, P);
+  prep_cell<NT, LDS>(A, i, s, g, P);
   finish_cell<LDS>(A, i, P, g, ring);
 }
 
-// Records and c_f (k . n_f) for every (direction, sweep position).
+// The record of every (direction, sweep position); materials fit 8 bits
+// (LB_MAX_MATERIALS, checked at creation).
 __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t NE, const uint32_t* __restrict__ tasks,
                                                                const double4* __restrict__ geo,
                                                                const int4* __restrict__ nbr,
                                                                const uint32_t* __restrict__ perm,
                                                                const int32_t* __restrict__ mat,
                                                                const double* __restrict__ kdir,
-                                                               TaskRec* __restrict__ rec, double4* __restrict__ ck) {
+                                                               TaskRec* __restrict__ rec) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= (int64_t)NI * NE) return;
   const int i = int(k / NE);
@@ -386,16 +355,16 @@ __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t N
   const int e = int(w & 0xFFFFFFu);
   TaskRec r;
   r.w = w;
-  r.mat = mat[e];
-  r.perm = perm[e];
-  r.pad = 0;
-  r.nbr = nbr[e];
-  rec[k] = r;
+  r.pm = (perm[e] & 0xFFFFFFu) | ((uint32_t)mat[e] << 24);
+  const int4 nb = nbr[e];
+  r.nb[0] = nb.x;
+  r.nb[1] = nb.y;
+  r.nb[2] = nb.z;
+  r.nb[3] = nb.w;
   const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
-  double v[4];
 #pragma unroll
-  for (int f = 0; f < 4; ++f) v[f] = face_ck(geo[(int64_t)e * 4 + f], kx, ky, kz);
-  ck[k] = make_double4(v[0], v[1], v[2], v[3]);
+  for (int f = 0; f < 3; ++f) r.ck[f] = face_ck(geo[(int64_t)e * 4 + f], kx, ky, kz);
+  rec[k] = r;
 }
 
 template <bool NT>
@@ -547,7 +516,7 @@ __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int
 // end re-read its last cell), which keeps the compiler's counter waits exact.
 // The staging copy for the next level (LDS sweep) is issued after the
 // look-ahead and written to LDS after the solves.
-template <int R, bool NT, bool LDS, bool PE>
+template <int R, bool NT, bool LDS>
 __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, int s0, int s1, int n0, int n1,
                                                      int cpr, int c, int g, bool lane_ok, const CellPrep (&P)[R],
                                                      CellPrep (&Pn)[R], const TraceRing& ring,
@@ -565,7 +534,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
     }
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS, PE>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
   const StageCopy sc = stage_load<LDS>(A, i, so, lnext);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -573,7 +542,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
     if (lane_ok && s < s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
   }
   if (lane_ok)
-    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS, PE>(A, i, s, g, ring);   // beyond R rounds
+    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
   stage_store<LDS>(ring, sc);
 }
 
@@ -582,7 +551,7 @@ __device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, 
 // first of the next level; only the last chunk of a level is followed by the
 // barrier (cells of a level are independent) and issues the staging copy for
 // the next level.  Advances (l, s0, s1) and returns false after the last chunk.
-template <int R, bool NT, bool LDS, bool PE>
+template <int R, bool NT, bool LDS>
 __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                      int nl, int chunk, int cpr, int c, int g, bool lane_ok,
                                                      int& l, int& s0, int& s1, const CellPrep (&P)[R],
@@ -593,8 +562,7 @@ __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, 
   const bool more = ln < nl;
   const int n0 = !more ? s0 : (last ? off[ln] : s1);
   const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
-  level_upstream_first<R, NT, LDS, PE>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so,
-                                       last ? l + 1 : -1);
+  level_upstream_first<R, NT, LDS>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so, last ? l + 1 : -1);
   if (last) __syncthreads();
   l = ln;
   s0 = n0;
@@ -605,7 +573,7 @@ __device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, 
 // Levels larger than R * cpr cells are walked in chunks of that size, so that
 // every cell's static data is fetched one chunk ahead (the short, wide
 // directions of a mesh have levels of thousands of cells).
-template <int BLOCK, int R, bool NT, bool LDS = false, bool PE = false>
+template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                            int nl, TraceRing ring = TraceRing{nullptr},
                                                            const int32_t* __restrict__ so = nullptr) {
@@ -618,11 +586,11 @@ __device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, i
   if (nl <= 0) return;
   int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS, PE>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
   while (true) {
-    if (!chunk_upstream_first<R, NT, LDS, PE>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
+    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
       break;
-    if (!chunk_upstream_first<R, NT, LDS, PE>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
+    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
       break;
   }
 }
@@ -662,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
 // upstream traces come from LDS where the plan holds them, otherwise from global
 // memory.  The flux is written to both; the workgroup barrier between levels
 // orders the LDS writes and reads.
-template <int BLOCK, bool NT, bool PE = false>
+template <int BLOCK, bool NT>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
                                                                     const int32_t* __restrict__ nlev, int maxlev,
                                                                     SweepArgs A) {
@@ -673,7 +641,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
                                                             TraceRing{ring_buf}, so);
   else
-    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true, PE>(
+    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true>(
         A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf}, so);
 }
 
@@ -1255,22 +1223,19 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
     launch_direction_nt<false>(block, ndir, st, lo, nl, maxlev, A);
 }
 
-template <bool NT, bool PE = false>
+template <bool NT>
 static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
                                    int maxlev, const SweepArgs& A, size_t bytes) {
-  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT, PE>
-                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT, PE>
-                                  : (const void*)&sweep_direction_lds_kernel<1024, NT, PE>;
+  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
+                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
+                                  : (const void*)&sweep_direction_lds_kernel<1024, NT>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT, PE>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A);
   else if (block == 512)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT, PE>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A);
   else
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT, PE>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A);
   return LB_OK;
 }
 
@@ -1310,7 +1275,7 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
                               p.stats)) != LB_OK)
     return nullptr;
   const int slots = p.ring + 2 * p.stage;
-  bool ok = true;
+  bool ok = slots <= 32767;   // slots travel as 16-bit indices
   for (int32_t v : uslot) ok &= v >= -1 && v < slots;
   for (int32_t v : wslot) ok &= v >= -1 && v < p.ring;
   for (size_t k = 0; k < ent.size(); k += 2) ok &= ent[k] >= p.ring && ent[k] < slots && ent[k + 1] >= 0 && ent[k + 1] < NE;
@@ -1332,8 +1297,9 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
         for (DevBuf* b : {&t->d_uslot, &t->d_wslot, &t->d_stage_off, &t->d_stage}) b->release();
     }
   } guard{&p};
-  if ((*rc = upload(op, p.d_uslot, uslot.data(), uslot.size() * sizeof(int32_t))) != LB_OK) return nullptr;
-  if ((*rc = upload(op, p.d_wslot, wslot.data(), wslot.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  const std::vector<int16_t> us16(uslot.begin(), uslot.end()), ws16(wslot.begin(), wslot.end());
+  if ((*rc = upload(op, p.d_uslot, us16.data(), us16.size() * sizeof(int16_t))) != LB_OK) return nullptr;
+  if ((*rc = upload(op, p.d_wslot, ws16.data(), ws16.size() * sizeof(int16_t))) != LB_OK) return nullptr;
   if ((*rc = upload(op, p.d_stage_off, soff.data(), soff.size() * sizeof(int32_t))) != LB_OK) return nullptr;
   if ((*rc = upload(op, p.d_stage, ent.data(), ent.size() * sizeof(int32_t))) != LB_OK) return nullptr;
   guard.t = nullptr;
@@ -1359,16 +1325,15 @@ static void launch_persistent(int block, bool nt, dim3 grid, hipStream_t st, con
     launch_persistent_nt<false>(block, grid, st, items, nitems, lvl, act, ctr, A);
 }
 
-// Records and c_f (k . n_f) of every (direction, sweep position) for a task order.
-static int ensure_records(lb_op* op, DevBuf& rec, DevBuf& ck, const DevBuf& tasks) {
+// The records of every (direction, sweep position) for a task order.
+static int ensure_records(lb_op* op, DevBuf& rec, const DevBuf& tasks) {
   if (rec.p) return LB_OK;
   const int64_t nrec = op->NI * op->NE;
   LB_TRY(rec.reserve(nrec * sizeof(TaskRec)));
-  LB_TRY(ck.reserve(nrec * sizeof(double4)));
   hipLaunchKernelGGL(build_records_kernel, dim3(grid_for(nrec)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
                      tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
                      op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
-                     rec.as<TaskRec>(), ck.as<double4>());
+                     rec.as<TaskRec>());
   LB_HIP(hipGetLastError());
   return LB_OK;
 }
@@ -1416,8 +1381,7 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
 // One sweep issued in a fixed launch configuration.
 static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
   const int mode = cfg.mode, block = cfg.block;
-  const bool lds = mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_DIRECTION_LDS_ELEMENT;
-  if ((mode == LB_SWEEP_DIRECTION || lds || mode == LB_SWEEP_PERSISTENT) &&
+  if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
       !((block == 256 || block == 512 || block == 1024) && block / G >= 1)) {
     set_error("sweep: the workgroup size must be 256, 512 or 1024 and hold the group batch");
     return LB_E_ARG;
@@ -1432,8 +1396,8 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       return LB_E_ARG;
     }
     BatchTable& B = op->batch;
-    LB_TRY(ensure_records(op, B.d_rec, B.d_ck, B.d_tasks));
-    SweepArgs A{B.d_rec.as<const TaskRec>(), B.d_ck.as<const double4>(), op->d_sigt.as<const double>(),
+    LB_TRY(ensure_records(op, B.d_rec, B.d_tasks));
+    SweepArgs A{B.d_rec.as<const TaskRec>(), op->d_sigt.as<const double>(),
                 op->G, g0, G, op->NE, q, psi};
     A.vec1 = one_group_vectors(G, psi);
     const bool nt = G >= kStreamGroups;
@@ -1449,16 +1413,12 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     });
   }
 
-  LB_TRY(ensure_records(op, op->d_rec, op->d_ck, op->d_tasks));
-  SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_ck.as<const double4>(), op->d_sigt.as<const double>(),
+  LB_TRY(ensure_records(op, op->d_rec, op->d_tasks));
+  SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_sigt.as<const double>(),
               op->G, g0, G, op->NE, q, psi};
   A.vec1 = one_group_vectors(G, psi);
   const bool nt = G >= kStreamGroups;
-  if (mode == LB_SWEEP_DIRECTION_LDS_ELEMENT && nt) {
-    set_error("sweep: the per-element LDS sweep takes fewer than 8 groups per cell");
-    return LB_E_ARG;
-  }
-  if (mode == LB_SWEEP_DIRECTION || lds) {
+  if (mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS) {
     if (!op->d_lvl_off.p) {
       LB_TRY(op->d_lvl_off.reserve(op->h_lvl_off.size() * sizeof(int32_t)));
       LB_HIP(hipMemcpy(op->d_lvl_off.p, op->h_lvl_off.data(), op->h_lvl_off.size() * sizeof(int32_t),
@@ -1468,26 +1428,17 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     }
     const int32_t* lo = op->d_lvl_off.as<const int32_t>();
     const int32_t* nl = op->d_nlev.as<const int32_t>();
-    if (lds) {
+    if (mode == LB_SWEEP_DIRECTION_LDS) {
       int rc = LB_OK;
       const TracePlan* plan = trace_plan(op, G, &rc);
       if (!plan) return rc;
       const size_t bytes = (size_t)(plan->ring + 2 * plan->stage) * 4 * G * sizeof(double);
-      A.uslot = plan->d_uslot.as<const int4>();
-      A.wslot = plan->d_wslot.as<const int32_t>();
+      A.uslot = plan->d_uslot.as<const short4>();
+      A.wslot = plan->d_wslot.as<const int16_t>();
       A.stage_off = plan->d_stage_off.as<const int32_t>();
       A.stage = plan->d_stage.as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
-      if (mode == LB_SWEEP_DIRECTION_LDS_ELEMENT) {
-        A.task = op->d_tasks.as<const uint32_t>();
-        A.geo = op->d_geo.as<const double4>();
-        A.enbr = op->d_nbr.as<const int4>();
-        A.eperm = op->d_perm.as<const uint32_t>();
-        A.emat = op->d_mat.as<const int32_t>();
-        A.kdir = op->d_k.as<const double>();
-        LB_TRY((launch_direction_lds_nt<false, true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A,
-                                                      bytes)));
-      } else if (nt)
+      if (nt)
         LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
       else
         LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));

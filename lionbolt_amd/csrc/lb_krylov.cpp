@@ -589,6 +589,7 @@ int moment_k_dev(lb_op* op, const double* in, double* out, int g0) {
   double* psi_t = op->w_a2.as<double>();
   LB_TRY(launch_moments_to_discrete(op, in, psi_t, g0, 1, 0));
   LB_TRY(sweep_inplace(op, psi_t, g0));
+  if (op->moment_fmt != LB_FMT_FP64) return launch_moments_reduced(op, psi_t, out);
   return launch_moments(op, psi_t, out, 1, op->NENK);
 }
 
@@ -1056,10 +1057,14 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
     st[g].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (st[g].status != 0) rc_all = st[g].status;
   }
-  // psi and the true residual, kTailGroups groups per sweep
-  const int B0 = std::min(kTailGroups, G);
+  // psi and the true residual, kTailGroups groups per sweep, or fewer when the
+  // device cannot hold that many groups' buffers (8 groups at 384k tets x 256
+  // directions: 55 GB)
+  int B0 = std::min(kTailGroups, G), rc_tail;
+  while ((rc_tail = op->w_tail.reserve((size_t)(2 * n + 3 * nphi) * B0 * sizeof(double))) == LB_E_NOMEM && B0 > 1)
+    B0 = (B0 + 1) / 2;
+  LB_TRY(rc_tail);
   const int64_t nb = n * B0, pb = nphi * B0;
-  LB_TRY(op->w_tail.reserve((size_t)(2 * nb + 3 * pb) * sizeof(double)));
   double* psi_b = op->w_tail.as<double>();
   double* r_b = psi_b + nb;
   double* Phi = r_b + nb;

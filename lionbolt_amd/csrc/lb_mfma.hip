@@ -19,6 +19,7 @@
 // takes one contiguous quarter of the reduced dimension.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -299,11 +300,52 @@ int grid_cols(int64_t ncols) {
   return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
 }
 
+__global__ __launch_bounds__(256) void widen_kernel(int64_t n, const float* __restrict__ in, double* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) out[j] = in[j];
+}
+
 }  // namespace
 
 }  // namespace lb
 
 using namespace lb;
+
+extern "C" {
+int lb_pack_dev(int fmt, int rows, int64_t ncols, const double* src, void* dst, void* dst_lo, float* inv_scale,
+                void* stream);
+int lb_gemm_d_dev(int fmt, int nmom, int NI, const double* D, int64_t ncols, const void* psi, const void* psi_lo,
+                  const float* inv_scale, float* phi, void* stream);
+}
+
+namespace lb {
+
+// D psi inside the flux-moment GMRES with reduced-precision inputs: psi
+// [NI][NENK] is packed column-major in the format (lb_pack_dev), multiplied on
+// MFMA with FP32 accumulation (lb_gemm_d_dev) and the FP32 moments are widened
+// into phi [nmom][NENK].
+int launch_moments_reduced(lb_op* op, const double* psi, double* phi) {
+  const int fmt = op->moment_fmt, NI = (int)op->NI, nmom = (int)op->nmom;
+  const int64_t ncols = op->NENK;
+  const size_t elem = fmt == FMT_FP32 ? 4 : 2;
+  const size_t hi = (size_t)NI * ncols * elem, lo = fmt == FMT_SPLIT_BF16 ? (size_t)NI * ncols * 2 : 0;
+  const size_t sc = fmt == FMT_FP16 ? (size_t)ncols * 4 : 0, out = (size_t)nmom * ncols * 4;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  LB_TRY(op->w_mix.reserve(al(hi) + al(lo) + al(sc) + al(out)));
+  char* base = op->w_mix.as<char>();
+  void* d_hi = base;
+  void* d_lo = lo ? base + al(hi) : nullptr;
+  float* d_sc = sc ? reinterpret_cast<float*>(base + al(hi) + al(lo)) : nullptr;
+  float* d_out = reinterpret_cast<float*>(base + al(hi) + al(lo) + al(sc));
+  LB_TRY(lb_pack_dev(fmt, NI, ncols, psi, d_hi, d_lo, d_sc, op->stream));
+  LB_TRY(lb_gemm_d_dev(fmt, nmom, NI, op->d_D.as<const double>(), ncols, d_hi, d_lo, d_sc, d_out, op->stream));
+  const int64_t n = (int64_t)nmom * ncols;
+  hipLaunchKernelGGL(widen_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)), dim3(256), 0,
+                     op->stream, n, d_out, phi);
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+}  // namespace lb
 
 extern "C" {
 

@@ -124,8 +124,7 @@ class Operator:
 
     _SWEEP_MODES = {"levels": _lib.SWEEP_LEVELS, "graph": _lib.SWEEP_GRAPH, "direction": _lib.SWEEP_DIRECTION,
                     "auto": _lib.SWEEP_AUTO, "persistent": _lib.SWEEP_PERSISTENT,
-                    "direction_lds": _lib.SWEEP_DIRECTION_LDS,
-                    "direction_lds_element": _lib.SWEEP_DIRECTION_LDS_ELEMENT}
+                    "direction_lds": _lib.SWEEP_DIRECTION_LDS}
 
     def set_sweep_mode(self, mode):
         _lib.check(_lib.lib().lb_op_set_sweep_mode(self.h, self._SWEEP_MODES[mode]))
@@ -269,6 +268,15 @@ class Operator:
     def set_preconditioner_threshold(self, min_thickness):
         """Groups optically thinner than ``min_thickness`` skip the network (plain GMRES)."""
         _lib.check(_lib.lib().lb_op_set_preconditioner_threshold(self.h, float(min_thickness)))
+
+    _MOMENT_FMTS = {"fp64": _lib.FMT_FP64, "fp32": _lib.FMT_FP32, "bf16": _lib.FMT_BF16,
+                    "split_bf16": _lib.FMT_SPLIT_BF16, "fp16": _lib.FMT_FP16}
+
+    def set_moment_precision(self, fmt="fp64"):
+        """Input precision of D psi inside the flux-moment GMRES's Arnoldi steps
+        (MFMA formats with FP32 accumulation; the right-hand side, psi and the true
+        residual stay FP64)."""
+        _lib.check(_lib.lib().lb_op_set_moment_precision(self.h, self._MOMENT_FMTS[fmt]))
 
     def mlp_apply_dev(self, r, z, g0=0):
         nphi = self.nmom * self.NENK

@@ -52,8 +52,8 @@ def test_create_without_device_fails_loudly():
 
 def test_size_limits_are_rejected_before_any_work():
     """Argument checks run before the device is touched: more elements than the
-    24-bit cell index of the sweep records, a scattering order above 15, an
-    empty mesh."""
+    24-bit cell index of the sweep records, more materials than their 8-bit
+    material index, a scattering order above 15, an empty mesh."""
     import ctypes
     m = M.jittered_cube(1)
     ang = angular.snprep(1)
@@ -71,6 +71,12 @@ def test_size_limits_are_rejected_before_any_work():
                        _lib.i64ptr(keep[3]), 0)
     qd = _lib.QuadDesc(ang.NI, _lib.dptr(keep[4]), _lib.dptr(keep[5]), 1)
     assert L.lb_schedule_host(ctypes.byref(md), ctypes.byref(qd), None, None, None) == -1
+    mat = keep[3].copy()
+    mat[-1] = 256                                        # the 257th material (0-based indices)
+    md = _lib.MeshDesc(m.NE, m.NK, _lib.dptr(keep[0]), _lib.i64ptr(keep[1]), _lib.i64ptr(keep[2]),
+                       _lib.i64ptr(mat), 0)
+    assert L.lb_op_create(ctypes.byref(md), ctypes.byref(qd), ctypes.byref(h)) == -1
+    assert b"256 materials" in L.lb_last_error()
 
 
 @pytest.mark.parametrize("case", ["small_s8", "small_snprep3", "cube48k_s8", "grazing_s8"])

@@ -25,8 +25,6 @@ def test_direction_resident_equals_level_launches(G, block):
     q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
     res = {}
     modes = ["levels", "direction", "direction", "graph", "direction_lds", "direction_lds"]
-    if G < 8:                                   # the per-element variant takes fewer than 8 groups per cell
-        modes += ["direction_lds_element"] * 2
     for mode in modes:
         op.set_sweep_mode(mode)
         psi = torch.full_like(q, float("nan"))
@@ -56,7 +54,7 @@ def test_lds_slots_with_staging_and_global_reads(G):
     op.set_physics_groups(sigt, np.zeros((G, 2, 1)))
     q = torch.rand(ang.NI * m.NE * 4 * G, dtype=torch.float64, device="cuda")
     out = {}
-    for mode in ("levels", "direction_lds", "direction_lds_element"):
+    for mode in ("levels", "direction_lds"):
         op.set_sweep_mode(mode)
         for block in (256, 512):
             op.set_direction_block(block)
@@ -66,7 +64,6 @@ def test_lds_slots_with_staging_and_global_reads(G):
             out[(mode, block)] = psi.cpu().numpy()
     for block in (256, 512):
         assert np.array_equal(out[("direction_lds", block)], out[("levels", 256)])
-        assert np.array_equal(out[("direction_lds_element", block)], out[("levels", 256)])
 
 
 def test_direction_resident_in_place_and_solver():
