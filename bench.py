@@ -376,7 +376,8 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
         torch.cuda.synchronize()
         res[kind] = {"seconds": time.perf_counter() - t0, "iterates": [s["iterates"] for s in st],
                      "total_iterates": sum(s["iterates"] for s in st), "sweeps": sum(s["sweeps"] for s in st),
-                     "max_true_residual": max(s["true_residual"] for s in st)}
+                     "max_true_residual": max(s["true_residual"] for s in st),
+                     "group_seconds": [round(s["seconds"], 4) for s in st]}
         if kind == "fgmres":
             res[kind]["fallbacks"] = [s["fallback"] for s in st]   # -1: network not chosen for the group
             res[kind]["groups_with_network"] = sum(s["fallback"] != -1 for s in st)
@@ -388,6 +389,21 @@ def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
         else:
             res["flux_rel_diff"] = float((phi[:, 0] - ref[:, 0]).abs().max() / ref[:, 0].abs().max())
     res["speedup_fgmres_vs_gmres"] = res["gmres"]["seconds"] / res["fgmres"]["seconds"]
+    # the network in every group (threshold 0, runtime fallback only): what the
+    # a-priori choice avoids, group by group
+    op.set_preconditioner_threshold(0.0)
+    phi = torch.zeros_like(qmom)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="fgmres", space="moment", m=30, rtol=rtol, atol=0.0)
+    torch.cuda.synchronize()
+    res["fgmres_network_in_every_group"] = {
+        "seconds": time.perf_counter() - t0, "iterates": [s["iterates"] for s in st],
+        "total_iterates": sum(s["iterates"] for s in st), "fallbacks": [s["fallback"] for s in st],
+        "group_seconds": [round(s["seconds"], 4) for s in st],
+        "gmres_group_seconds": res["gmres"].get("group_seconds"),
+        "max_true_residual": max(s["true_residual"] for s in st)}
+    op.set_preconditioner_threshold(thr)
     # one network apply on a full-size residual (RMS scaling + the MFMA kernel);
     # algorithmic bytes per spatial dof: 16 FP64 moments in and out, plus the
     # element's material and size shared by its 4 nodes (12 B / 4)
