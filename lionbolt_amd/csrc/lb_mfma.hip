@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "lb_internal.hpp"
 
@@ -95,95 +96,94 @@ __global__ __launch_bounds__(256) void d_gemm_kernel(int nmom, const double* __r
   const bool row_ok = row < nmom;
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t ntile = (ncols + 15) / 16;
-  if (FMT == FMT_FP32) {
-    float a[Q];
+  // A operand of the format, converted once per wave
+  constexpr int NA = FMT == FMT_FP32 ? Q : Q / 8;
+  typedef typename std::conditional<FMT == FMT_FP32, float,
+                                    typename std::conditional<FMT == FMT_FP16, f16x8, bf16x8>::type>::type AT;
+  AT a[NA], alo[FMT == FMT_SPLIT_BF16 ? NA : 1];
+  if constexpr (FMT == FMT_FP32) {
 #pragma unroll
     for (int s = 0; s < Q; ++s) a[s] = row_ok ? (float)D[row * NI + kb * Q + s] : 0.f;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntile; t += waves) {
-      const int64_t col = t * 16 + j;
-      const bool ok = col < ncols;
-      const float* p = static_cast<const float*>(psi) + col * NI + kb * Q;
-      f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s4 = 0; s4 < Q; s4 += 4) {
-        const float4 v = ok ? *reinterpret_cast<const float4*>(p + s4) : make_float4(0.f, 0.f, 0.f, 0.f);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4 + 0], v.x, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4 + 1], v.y, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4 + 2], v.z, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s4 + 3], v.w, c, 0, 0, 0);
-      }
-      if (ok)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (4 * kb + r < nmom) phi[(int64_t)(4 * kb + r) * ncols + col] = c[r];
-    }
-  } else if (FMT == FMT_FP16) {
-    f16x8 a[Q / 8];
-#pragma unroll
-    for (int s = 0; s < Q / 8; ++s)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) a[s][e] = row_ok ? (_Float16)D[row * NI + kb * Q + 8 * s + e] : (_Float16)0.f;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntile; t += waves) {
-      const int64_t col = t * 16 + j;
-      const bool ok = col < ncols;
-      const f16x8* p = reinterpret_cast<const f16x8*>(static_cast<const _Float16*>(psi) + col * NI + kb * Q);
-      f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < Q / 8; ++s) {
-        f16x8 v = {};
-        if (ok) v = p[s];
-        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[s], v, c, 0, 0, 0);
-      }
-      if (ok) {
-        const float sc = inv_scale[col];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (4 * kb + r < nmom) phi[(int64_t)(4 * kb + r) * ncols + col] = c[r] * sc;
-      }
-    }
   } else {
-    bf16x8 a[Q / 8], alo[Q / 8];
 #pragma unroll
     for (int s = 0; s < Q / 8; ++s)
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const double d = row_ok ? D[row * NI + kb * Q + 8 * s + e] : 0.0;
-        const __bf16 hi = to_bf16((float)d);
-        a[s][e] = hi;
-        alo[s][e] = to_bf16((float)(d - (double)(float)hi));
-      }
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntile; t += waves) {
-      const int64_t col = t * 16 + j;
-      const bool ok = col < ncols;
-      const int64_t o = col * NI + kb * Q;
-      const bf16x8* p = reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(psi) + o);
-      f32x4 c = {0.f, 0.f, 0.f, 0.f};
-      if (FMT == FMT_BF16) {
-#pragma unroll
-        for (int s = 0; s < Q / 8; ++s) {
-          bf16x8 v = {};
-          if (ok) v = p[s];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], v, c, 0, 0, 0);
+        if constexpr (FMT == FMT_FP16) {
+          a[s][e] = (_Float16)d;
+        } else {
+          const __bf16 hi = to_bf16((float)d);
+          a[s][e] = hi;
+          if constexpr (FMT == FMT_SPLIT_BF16) alo[s][e] = to_bf16((float)(d - (double)(float)hi));
         }
+      }
+  }
+  // B operand of one tile: this lane's Q directions of column t * 16 + j (and
+  // the low halves for split BF16), loaded one tile ahead of its products so
+  // that a wave keeps two tiles' loads in flight.
+  constexpr int NB = FMT == FMT_FP32 ? Q / 4 : Q / 8;
+  typedef typename std::conditional<FMT == FMT_FP32, float4, AT>::type BT;
+  struct Tile {
+    BT v[NB];
+    BT vl[FMT == FMT_SPLIT_BF16 ? NB : 1];
+  };
+  auto load = [&](int64_t t, Tile& T) {
+    const int64_t col = t * 16 + j;
+    const bool ok = col < ncols;
+    const int64_t o = col * NI + kb * Q;
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      if constexpr (FMT == FMT_FP32) {
+        T.v[s] = ok ? reinterpret_cast<const float4*>(static_cast<const float*>(psi) + o)[s]
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+      } else if constexpr (FMT == FMT_FP16) {
+        T.v[s] = ok ? reinterpret_cast<const f16x8*>(static_cast<const _Float16*>(psi) + o)[s] : f16x8{};
       } else {
-        const bf16x8* pl = reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(psi_lo) + o);
-#pragma unroll
-        for (int s = 0; s < Q / 8; ++s) {
-          bf16x8 v = {}, vl = {};
-          if (ok) {
-            v = p[s];
-            vl = pl[s];
-          }
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[s], v, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], vl, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], v, c, 0, 0, 0);
-        }
+        T.v[s] = ok ? reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(psi) + o)[s] : bf16x8{};
+        if constexpr (FMT == FMT_SPLIT_BF16)
+          T.vl[s] = ok ? reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(psi_lo) + o)[s] : bf16x8{};
       }
-      if (ok)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (4 * kb + r < nmom) phi[(int64_t)(4 * kb + r) * ncols + col] = c[r];
     }
+  };
+  auto product = [&](const Tile& T) {
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      if constexpr (FMT == FMT_FP32) {
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * s + 0], T.v[s].x, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * s + 1], T.v[s].y, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * s + 2], T.v[s].z, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[4 * s + 3], T.v[s].w, c, 0, 0, 0);
+      } else if constexpr (FMT == FMT_FP16) {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[s], T.v[s], c, 0, 0, 0);
+      } else if constexpr (FMT == FMT_BF16) {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], T.v[s], c, 0, 0, 0);
+      } else {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[s], T.v[s], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], T.vl[s], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], T.v[s], c, 0, 0, 0);
+      }
+    }
+    return c;
+  };
+  int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  Tile cur;
+  if (t < ntile) load(t, cur);
+  while (t < ntile) {
+    const int64_t tn = t + waves;
+    Tile nxt;
+    if (tn < ntile) load(tn, nxt);
+    const f32x4 c = product(cur);
+    const int64_t col = t * 16 + j;
+    if (col < ncols) {
+      const float sc = FMT == FMT_FP16 ? inv_scale[col] : 1.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * kb + r < nmom) phi[(int64_t)(4 * kb + r) * ncols + col] = FMT == FMT_FP16 ? c[r] * sc : c[r];
+    }
+    cur = nxt;
+    t = tn;
   }
 }
 
@@ -294,10 +294,17 @@ __global__ __launch_bounds__(256) void fp16_scale_kernel(int rows, int64_t ncols
   inv_scale[c] = ldexpf(1.0f, -e);
 }
 
-int grid_cols(int64_t ncols) {
+// Grid of the two products: one wave per 16-column tile up to `cap`
+// workgroups, grid-stride beyond.  Each wave converts its operand of D or M
+// once.  The M product (one 16-direction tile of M per wave, 16 B loads of phi)
+// runs best with 2048 workgroups (8 per CU), each wave walking many tiles
+// (BF16 at 6.1 M columns: 3.0 -> 4.0 TB/s); the D product, whose tile loop
+// waits on its own loads, needs the extra waves of up to 65,536 workgroups to
+// keep bytes in flight (2,048 workgroups: 2.6 -> 1.4 TB/s at 36.9 M columns).
+int grid_cols(int64_t ncols, int64_t cap) {
   const int64_t tiles = (ncols + 15) / 16;
   const int64_t blocks = (tiles + 3) / 4;
-  return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
+  return (int)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap);
 }
 
 __global__ __launch_bounds__(256) void widen_kernel(int64_t n, const float* __restrict__ in, double* __restrict__ out) {
@@ -382,7 +389,7 @@ int lb_gemm_d_dev(int fmt, int nmom, int NI, const double* D, int64_t ncols, con
       set_error("lb_gemm_d_dev: needs NI = 256, nmom <= 16 and the operands of the format");
       return LB_E_ARG;
     }
-    const int g = grid_cols(ncols);
+    const int g = grid_cols(ncols, 8192);
     switch (fmt) {
       case FMT_FP32: hipLaunchKernelGGL((d_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
       case FMT_BF16: hipLaunchKernelGGL((d_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, nmom, D, ncols, psi, psi_lo, inv_scale, phi); break;
@@ -403,7 +410,7 @@ int lb_gemm_m_dev(int fmt, int NI, int nmom, const double* M, int64_t ncols, con
       set_error("lb_gemm_m_dev: needs NI = 256, nmom = 16 and the operands of the format");
       return LB_E_ARG;
     }
-    const int g = grid_cols(ncols);
+    const int g = grid_cols(ncols, (fmt == FMT_BF16 || fmt == FMT_SPLIT_BF16) ? 4096 : 65536);
     switch (fmt) {
       case FMT_FP32: hipLaunchKernelGGL((m_gemm_kernel<FMT_FP32, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
       case FMT_BF16: hipLaunchKernelGGL((m_gemm_kernel<FMT_BF16, 256>), dim3(g), dim3(256), 0, s, M, ncols, phi, phi_lo, inv_scale, psi); break;
