@@ -503,6 +503,11 @@ def single_group_leg(m, ang, reps=10):
     return res
 
 
+def _note(msg):
+    """A progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run(args):
     import torch
     from lionbolt_amd import problems
@@ -610,14 +615,18 @@ def run(args):
         out["roofline"]["traffic_unit"] = "bytes per launch (PMC, " + tr["source"] + ")"
         out["roofline"]["traffic_bytes_per_unit"] = per_cag
     out.update(out_extra)
+    _note(f"headline sweep: {ms_step:.2f} ms per step, {value:.4g} cell-angle-groups/s")
     if other_modes:
         out["other_sweep_modes"] = other_modes
     if args.single:
         out["single_group_sweep"] = single_group_leg(m, ang)
+        _note("single-group sweep modes done")
     if args.high_order:
         out["high_order_operator"] = high_order_leg()
+        _note("scattering orders 3-15 done")
     if args.solve:
         runs = [solve_leg(m, ang, args.solve_rtol, b) for b in args.group_batches]
+        _note("24-group solve done")
         out["solve"] = min(runs, key=lambda r: r["seconds"])
         out["solve_by_group_batch"] = {str(r["group_batch"]): {k: r[k] for k in ("seconds", "sweeps",
                                                                                 "max_true_residual")}
@@ -625,19 +634,25 @@ def run(args):
     ref = None
     if args.ref_method:
         out["reference_method_gpu"] = reference_method_leg(m, ang, 1e-6)
+        _note("reference method done")
         ref = {"sweeps": out["reference_method_gpu"]["sweeps"],
                "plain_sweeps": out["solve"]["sweeps"] if "solve" in out else None}
     if args.cube48k:
         out["cube48k_gpu"] = cube48k_leg()
+        _note("48,000-tet solve done")
     if args.batch:
         out["uneven_beams_batch"] = batch_leg()
+        _note("64-problem batch done")
     if args.learned_pc:
         out["learned_pc"] = learned_pc_leg(m, ang)
+        _note("learned preconditioner done")
     if args.gemm_study:
         from lionbolt_amd import gemm_study
         out["moment_gemm_study"] = gemm_study.run(ncols=m.NENK, groups=G)
     if args.cpu_baseline:
+        _note("CPU baseline (oracle on the host cores) ...")
         out["cpu_baseline"] = cpu_baseline(args.n, ref)
+        _note("CPU baseline done")
         for key in ("solve_projected", "solve_projected_moment_form"):
             proj = out["cpu_baseline"].get(key)
             if proj and "solve" in out:

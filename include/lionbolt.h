@@ -295,6 +295,18 @@ int lb_beam_moments_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi
 /* AddAngularFluence.f90:153-166: d_unc [NI][NENK] += phi0 g_i.  Synchronous. */
 int lb_beam_angular_fluence_dev(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_unc);
 
+/* ExternalBeam%Source => FCSSource (FCSSource.f90:3-215, planar beam) on a
+ * host vector in the reference layout s[sdof + i NENK] (n = NENK NI): s += the
+ * first-collision source, phi0 attenuated by the operator's sigma_t (the
+ * single-group physics, as FluenceQuadrature(OpK%XS%t)) and weighted by
+ * sum_l (2l+1)/(4 pi) P_l(k_i . axis) sigma_l(material).  Synchronous. */
+int lb_beam_source(lb_op* op, const lb_beam_desc* beam, double* s, int64_t n);
+
+/* ExternalBeam%AddAngularFluence (AddAngularFluence.f90:3-166, planar) on a
+ * host vector: unc += phi0 g_i, phi0 attenuated by attn[material].
+ * Synchronous. */
+int lb_beam_add_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* attn, double* unc, int64_t n);
+
 /* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
 int lb_op_set_direction_block(lb_op* op, int threads);
 

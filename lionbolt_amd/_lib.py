@@ -82,6 +82,8 @@ SIGNATURES = {
                                                     _vp]),
     "lb_beam_moments_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _vp, _vp, _vp]),
     "lb_beam_angular_fluence_dev": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _vp, _vp]),
+    "lb_beam_source": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, ctypes.c_int64]),
+    "lb_beam_add_angular_fluence": (ctypes.c_int, [_vp, ctypes.POINTER(BeamDesc), _d, _d, ctypes.c_int64]),
     "lb_mlp_features_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_double)]),
     "lb_mlp_train_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_uint64, _f]),

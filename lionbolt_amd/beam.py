@@ -101,6 +101,24 @@ class ExternalBeam:
         fcs, _ = self.moments(op, phi0)
         op.moments_to_discrete_dev(fcs, s, g, 1, accumulate=True)
 
+    def source_host(self, op, s):
+        """FCSSource on a host vector (reference layout, NENK * NI): s + the
+        first-collision source with the operator's single-group sigma_t and
+        sigma_l (``lb_beam_source``); returns the new vector."""
+        x = op._vec(s)
+        d = self.fldgeo.desc(self.n)
+        _lib.check(_lib.lib().lb_beam_source(op.h, ctypes.byref(d), _lib.dptr(x), x.size))
+        return x
+
+    def add_angular_fluence_host(self, op, attn, unc):
+        """AddAngularFluence on a host vector: unc + the uncollided angular fluence
+        for attenuation ``attn`` (per material) (``lb_beam_add_angular_fluence``)."""
+        x = op._vec(unc)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(attn, dtype=np.float64), (op.Nm,)))
+        d = self.fldgeo.desc(self.n)
+        _lib.check(_lib.lib().lb_beam_add_angular_fluence(op.h, ctypes.byref(d), _lib.dptr(a), _lib.dptr(x), x.size))
+        return x
+
     def AddAngularFluence(self, op, phi0, unc):
         """unc [NI][NENK] (cuda) += phi0 sum_l (2l+1)/(4 pi) P_l(k_i . axis)."""
         p, u = _dev(phi0, op.NENK, "phi0"), _dev(unc, op.n, "unc")

@@ -412,6 +412,37 @@ int lb_beam_angular_fluence_dev(lb_op* op, const lb_beam_desc* beam, const doubl
   });
 }
 
+int lb_beam_source(lb_op* op, const lb_beam_desc* beam, double* s, int64_t n) {
+  return guarded([&]() -> int {
+    LB_TRY(check_ready(op, "ExternalBeam%Source"));
+    LB_TRY(check_n(op, n, "ExternalBeam%Source"));
+    if (!s) {
+      set_error("ExternalBeam%Source: null source vector");
+      return LB_E_ARG;
+    }
+    LB_TRY(to_device(op, s, n));
+    LB_TRY(beam_source(op, beam, 0, op->w_a.as<double>()));
+    return to_host(op, s, n);
+  });
+}
+
+int lb_beam_add_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* attn, double* unc, int64_t n) {
+  return guarded([&]() -> int {
+    if (!op) {
+      set_error("ExternalBeam%AddAngularFluence: null operator");
+      return LB_E_ARG;
+    }
+    LB_TRY(check_n(op, n, "ExternalBeam%AddAngularFluence"));
+    if (!unc || !attn) {
+      set_error("ExternalBeam%AddAngularFluence: null argument");
+      return LB_E_ARG;
+    }
+    LB_TRY(to_device(op, unc, n));
+    LB_TRY(beam_add_angular_fluence(op, beam, attn, op->w_a.as<double>()));
+    return to_host(op, unc, n);
+  });
+}
+
 int lb_op_set_preconditioner_threshold(lb_op* op, double min_thickness) {
   return guarded([&]() -> int {
     if (!op || std::isnan(min_thickness) || min_thickness < 0.0) {
@@ -743,7 +774,8 @@ void lb_op_destroy(lb_op* op) {
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
     b->release();
-  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat, &op->d_vert, &op->w_beam, &op->w_mix})
+  for (DevBuf* b : {&op->d_mlp_w, &op->d_mlp_pack, &op->d_elem_h, &op->w_feat, &op->d_vert, &op->w_beam, &op->w_mix,
+                    &op->w_phi0})
     b->release();
   for (auto& kv : op->batch.seg) {
     kv.second.d_tiles.release();

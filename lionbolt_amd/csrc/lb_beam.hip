@@ -150,6 +150,19 @@ __global__ __launch_bounds__(256) void beam_angular_kernel(int64_t NI, int64_t N
   }
 }
 
+// s[i][sd] += fac[i][mat(sd)] phi0[sd]: the first-collision source of
+// FCSSource.f90:178-215, fac the angular factor of each (direction, material).
+__global__ __launch_bounds__(256) void beam_source_kernel(int64_t NI, int64_t NENK, int Nm,
+                                                          const double* __restrict__ fac,
+                                                          const int32_t* __restrict__ mat,
+                                                          const double* __restrict__ phi0, double* __restrict__ s) {
+  const int64_t n = NI * NENK;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+    const int64_t i = j / NENK, sd = j - i * NENK;
+    s[j] += fac[i * Nm + mat[sd >> 2]] * phi0[sd];
+  }
+}
+
 __global__ void zero_int_kernel(int* x) {
   if (threadIdx.x == 0) x[0] = 0;
 }
@@ -302,6 +315,56 @@ int beam_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* d_ph
   LB_HIP(hipGetLastError());
   LB_HIP(hipStreamSynchronize(op->stream));
   return LB_OK;
+}
+
+int beam_source(lb_op* op, const lb_beam_desc* beam, int g0, double* d_s) {
+  LB_TRY(check_beam(op, beam));
+  if (g0 < 0 || g0 >= op->G) {
+    set_error("lb_beam_source: group outside the operator's physics");
+    return LB_E_ARG;
+  }
+  // attenuation sigma_t(mat, g0), as FluenceQuadrature(OpK%XS%t) (FCSSource.f90:47)
+  std::vector<double> attn(op->Nm);
+  for (int m = 0; m < op->Nm; ++m) attn[m] = op->h_sigt[(size_t)m * op->G + g0];
+  LB_TRY(op->w_phi0.reserve((size_t)op->NENK * sizeof(double)));
+  double* d_phi0 = op->w_phi0.as<double>();
+  LB_TRY(beam_uncollided(op, beam, attn.data(), 0, op->NE, d_phi0));
+  // fac(i, m) = sum_l (2l+1)/(4 pi) P_l(k_i . axis) sigma_l(m, g0) (FCSSource.f90:184-211)
+  std::vector<double> fac((size_t)op->NI * op->Nm, 0.0);
+  for (int64_t i = 0; i < op->NI; ++i) {
+    const double* k = &op->h_k[i * 3];
+    const double x = k[0] * beam->axis[0] + k[1] * beam->axis[1] + k[2] * beam->axis[2];
+    double p0 = 1.0, p1 = x;
+    for (int l = 0; l <= op->L; ++l) {
+      double P;
+      if (l == 0) P = 1.0;
+      else if (l == 1) P = x;
+      else {
+        P = ((2 * l - 1) * x * p1 - (l - 1) * p0) / l;
+        p0 = p1;
+        p1 = P;
+      }
+      for (int m = 0; m < op->Nm; ++m)
+        fac[(size_t)i * op->Nm + m] +=
+            (2 * l + 1) * P / (4.0 * M_PI) * op->h_sigs[((size_t)m * op->G + g0) * (op->L + 1) + l];
+    }
+  }
+  LB_TRY(op->w_beam.reserve((size_t)(op->Nm + 8 + fac.size()) * sizeof(double)));
+  double* d_fac = op->w_beam.as<double>() + op->Nm + 8;
+  LB_HIP(hipMemcpy(d_fac, fac.data(), fac.size() * sizeof(double), hipMemcpyHostToDevice));
+  const int64_t n = op->NI * op->NENK;
+  hipLaunchKernelGGL(beam_source_kernel, dim3(stride_grid(n)), dim3(256), 0, op->stream, op->NI, op->NENK, op->Nm,
+                     d_fac, op->d_mat.as<const int32_t>(), d_phi0, d_s);
+  LB_HIP(hipGetLastError());
+  LB_HIP(hipStreamSynchronize(op->stream));
+  return LB_OK;
+}
+
+int beam_add_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* attn, double* d_unc) {
+  LB_TRY(op->w_phi0.reserve((size_t)op->NENK * sizeof(double)));
+  double* d_phi0 = op->w_phi0.as<double>();
+  LB_TRY(beam_uncollided(op, beam, attn, 0, op->NE, d_phi0));
+  return beam_angular_fluence(op, beam, d_phi0, d_unc);
 }
 
 }  // namespace lb

@@ -204,6 +204,7 @@ struct lb_op {
   std::vector<double> h_vert;       // [NE][4][3] vertex coordinates in local node order
   lb::DevBuf d_vert;                // the same on the device (uploaded on the first beam)
   lb::DevBuf w_beam;                // beam scratch: attenuation, failure counter, moment coefficients
+  lb::DevBuf w_phi0;                // projected uncollided fluence of the host beam entry points
   // learned right preconditioner (per cell-node MLP), see lb_mlp.hip
   bool mlp = false;
   int mlp_fmt = 1;                  // LB_FMT_BF16 (MFMA) or LB_FMT_FP32
@@ -331,6 +332,10 @@ int mlp_features(lb_op* op, const double* r, int g0, float* feat, double* scale)
 int beam_uncollided(lb_op* op, const lb_beam_desc* beam, const double* attn, int64_t e0, int64_t e1, double* d_phi0);
 int beam_moments(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_fcs, double* d_unc);
 int beam_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* d_phi0, double* d_unc);
+// d_s [NI][NENK] += the first-collision source of group g0 (FCSSource.f90:134-215)
+int beam_source(lb_op* op, const lb_beam_desc* beam, int g0, double* d_s);
+// d_unc [NI][NENK] += the uncollided angular fluence for attenuation attn[Nm]
+int beam_add_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* attn, double* d_unc);
 
 // lb_batch.hip
 int set_problems(lb_op* op, int P, const int64_t* eoff);

@@ -35,6 +35,18 @@ Module LionboltHIP
         Integer (c_int) :: verbose
     End Type
 
+    Integer (c_int), Parameter, Public :: LB_CUTOUT_NONE = 0, LB_CUTOUT_CIRCLE = 1, LB_CUTOUT_RECTANGLE = 2
+
+    ! FieldGeometry + ExternalBeam of a planar beam (src/sources/Sources.f90:13-80):
+    ! unit axis, field origin, the field plane's axes e1, e2 (the z-y-z rotation of
+    ! the axis for a rectangle), the cutout and its parameters, the population n.
+    Type, Bind(C), Public :: lb_beam_desc
+        Real (c_double) :: axis (3), origin (3), e1 (3), e2 (3)
+        Integer (c_int) :: cutout
+        Real (c_double) :: params (4)
+        Real (c_double) :: n
+    End Type
+
     Type, Bind(C), Public :: lb_solve_stats
         Integer (c_int) :: status, iterates, matvecs, sweeps
         Real (c_double) :: final_norm, true_residual, seconds
@@ -92,6 +104,22 @@ Module LionboltHIP
             Real (c_double), Intent (In)  :: v (*)
             Real (c_double), Intent (Out) :: phi (*)
         End Function
+        Integer (c_int) Function lb_beam_source (op, beam, s, n) Bind(C, name='lb_beam_source')
+            import :: c_int, c_ptr, c_double, c_int64_t, lb_beam_desc
+            Type (c_ptr),        Value          :: op
+            Type (lb_beam_desc), Intent (In)    :: beam
+            Real (c_double),     Intent (InOut) :: s (*)
+            Integer (c_int64_t), Value          :: n
+        End Function
+        Integer (c_int) Function lb_beam_add_angular_fluence (op, beam, attn, unc, n) &
+                Bind(C, name='lb_beam_add_angular_fluence')
+            import :: c_int, c_ptr, c_double, c_int64_t, lb_beam_desc
+            Type (c_ptr),        Value          :: op
+            Type (lb_beam_desc), Intent (In)    :: beam
+            Real (c_double),     Intent (In)    :: attn (*)
+            Real (c_double),     Intent (InOut) :: unc (*)
+            Integer (c_int64_t), Value          :: n
+        End Function
         Subroutine lb_op_destroy (op) Bind(C, name='lb_op_destroy')
             import :: c_ptr
             Type (c_ptr), Value :: op
@@ -119,6 +147,8 @@ Module LionboltHIP
         Procedure :: BoltzmannMatVec
         Procedure :: BoltzmannMatInv
         Procedure :: Fluence
+        Procedure :: BeamSource
+        Procedure :: BeamAddAngularFluence
         Procedure :: Destroy
     End Type
 
@@ -245,6 +275,29 @@ Contains
         rc = lb_fluence(Op%h, v, phi)
         if (rc /= 0) call lb_stop('SpaceAngleVector%Fluence', rc)
     End Function
+
+    ! ExternalBeam%Source => FCSSource (planar): s <- s + the first-collision source,
+    ! attenuated by the operator's sigma_t and weighted by its sigma_l, as the
+    ! reference does with OpK
+    Subroutine BeamSource (Op, beam, s)
+        Class (HIPOperator), Intent (InOut) :: Op
+        Type (lb_beam_desc), Intent (In)    :: beam
+        Real (c_double),     Intent (InOut) :: s (:)
+        Integer (c_int) :: rc
+        rc = lb_beam_source(Op%h, beam, s, INT(SIZE(s), c_int64_t))
+        if (rc /= 0) call lb_stop('ExternalBeam%Source', rc)
+    End Subroutine
+
+    ! ExternalBeam%AddAngularFluence (planar): unc <- unc + the uncollided angular fluence
+    Subroutine BeamAddAngularFluence (Op, beam, attn, unc)
+        Class (HIPOperator), Intent (InOut) :: Op
+        Type (lb_beam_desc), Intent (In)    :: beam
+        Real (c_double),     Intent (In)    :: attn (:)
+        Real (c_double),     Intent (InOut) :: unc (:)
+        Integer (c_int) :: rc
+        rc = lb_beam_add_angular_fluence(Op%h, beam, attn, unc, INT(SIZE(unc), c_int64_t))
+        if (rc /= 0) call lb_stop('ExternalBeam%AddAngularFluence', rc)
+    End Subroutine
 
     Subroutine Destroy (Op)
         Class (HIPOperator), Intent (InOut) :: Op

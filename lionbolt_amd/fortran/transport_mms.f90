@@ -6,7 +6,11 @@
 !  (real64); node(4,NE), neighbors(4,NE), mat(NE) (int64, 1-based, boundary 0);
 !  sigma (real64), c (real64: scattering ratio of the Boltzmann solve).
 !  output: T^-1 s for the MMS source (NENK*NI real64), then the scalar flux of
-!  BoltzmannOp%MatInv of a unit source (NENK real64).  Prints the RMS MMS error
+!  BoltzmannOp%MatInv of a unit source (NENK real64), then the first-collision
+!  source of a planar beam along (theta, phi) = (0.41, 2.2), a direction no ray
+!  follows along a mesh edge (ExternalBeam%Source with that operator's physics,
+!  NENK*NI real64), and its uncollided angular fluence for attenuation 0.5
+!  (ExternalBeam%AddAngularFluence, NENK*NI real64).  Prints the RMS MMS error
 !  and the error of the delta-down kernel in the shape of
 !  validation/operators/deltadownMMS.f90 (XSdd = 1, 10 on two materials, s = 1).
 Program transport_mms
@@ -16,6 +20,8 @@ Program transport_mms
     Type (HIPOperator)               :: OpT, OpL, OpK
     Integer (c_int64_t)              :: NE, NK, NI, L, e, i, kk, sdof, n
     Real (c_double),     Allocatable :: rg (:,:), kv (:,:), w (:), s (:), strue (:), phi (:), sigs (:,:)
+    Real (c_double),     Allocatable :: sb (:), ub (:)
+    Type (lb_beam_desc)              :: beam
     Integer (c_int64_t), Allocatable :: node (:,:), nbr (:,:), mat (:), mat2 (:)
     Real (c_double)                  :: sigma, c, r (3), err
     Real (c_double), Parameter       :: A = 10.0_c_double
@@ -63,6 +69,25 @@ Program transport_mms
     write (*, '(A,I0,A,ES12.4)') 'BoltzmannOp%MatInv iterates: ', OpL%stats%iterates, &
         '  true residual: ', OpL%stats%true_residual
 
+    ! A planar beam through the shim: its first-collision source with OpL's
+    ! physics, and its uncollided angular fluence for attenuation 0.5
+    beam%axis = [SIN(0.41_c_double) * COS(2.2_c_double), SIN(0.41_c_double) * SIN(2.2_c_double), COS(0.41_c_double)]
+    beam%origin = -3.0_c_double * beam%axis
+    beam%e1 = [1.0_c_double, 0.0_c_double, 0.0_c_double] - beam%axis(1) * beam%axis
+    beam%e1 = beam%e1 / NORM2(beam%e1)
+    beam%e2 = [beam%axis(2) * beam%e1(3) - beam%axis(3) * beam%e1(2), &
+               beam%axis(3) * beam%e1(1) - beam%axis(1) * beam%e1(3), &
+               beam%axis(1) * beam%e1(2) - beam%axis(2) * beam%e1(1)]
+    beam%cutout = LB_CUTOUT_NONE
+    beam%params = 0.0_c_double
+    beam%n = 1.0_c_double
+    ALLOCATE(sb(n), ub(n))
+    sb = 0.0_c_double
+    call OpL%BeamSource(beam, sb)
+    ub = 0.0_c_double
+    call OpL%BeamAddAngularFluence(beam, [0.5_c_double], ub)
+    write (*, '(A,ES12.4)') 'ExternalBeam%Source sum: ', SUM(sb)
+
     ! deltadownMMS: two materials split at z = 0, K = XSdd(mat) I, s = 1
     ALLOCATE(mat2(NE))
     do e = 1, NE
@@ -85,7 +110,7 @@ Program transport_mms
     call OpK%Destroy()
 
     open (newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
-    write (u) s, phi
+    write (u) s, phi, sb, ub
     close (u)
     call OpT%Destroy()
     call OpL%Destroy()

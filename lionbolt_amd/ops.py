@@ -80,6 +80,7 @@ class Operator:
         ne, ni, nm, ml = (ctypes.c_int64() for _ in range(4))
         _lib.check(L.lb_op_info(h, ctypes.byref(ne), ctypes.byref(ni), ctypes.byref(nm), ctypes.byref(ml)))
         self.maxlev = ml.value
+        self.Nm = max(1, int(np.max(mesh.mat)))    # materials 1..Nm, as the library counts them
         self.G = 0
 
     def close(self):

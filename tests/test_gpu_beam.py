@@ -73,6 +73,30 @@ def test_first_collision_source_and_uncollided_fluence_match_oracle():
     op.close()
 
 
+@pytest.mark.parametrize("cutout", ["none", "circle"])
+def test_host_entry_points_match_oracle(cutout):
+    """lb_beam_source and lb_beam_add_angular_fluence, the host-vector entries a
+    Fortran ExternalBeam%Source / %AddAngularFluence would call: they add to what
+    the vector holds, with the operator's sigma_t as the attenuation of the source."""
+    op, orc, m, ang, beam = case(cutout)
+    sigt = np.array([0.4, 1.3])
+    sigs = np.array([[0.3, 0.15, 0.08, 0.02], [1.0, 0.5, 0.2, 0.05]])
+    op.set_physics(sigt, sigs)
+    orc.set_physics(sigt, sigs)
+    fg = beam.fldgeo
+    ref0, _ = orc.beam_uncollided(fg.axis, fg.origin, sigt, cutout=cutout, params=fg.cutoutparams, n=beam.n,
+                                  frame=(fg.e1, fg.e2))
+    base = np.random.default_rng(3).random(op.n)
+    got = beam.source_host(op, base)
+    assert rel(got, orc.beam_source(fg.axis, ref0, base)) <= 1e-12
+    attn = np.array([0.25, 0.9])
+    ref_a, _ = orc.beam_uncollided(fg.axis, fg.origin, attn, cutout=cutout, params=fg.cutoutparams, n=beam.n,
+                                   frame=(fg.e1, fg.e2))
+    unc = beam.add_angular_fluence_host(op, attn, base)
+    assert rel(unc - base, orc.beam_angular_fluence(fg.axis, ref_a)) <= 1e-12
+    op.close()
+
+
 def test_beam_solve_total_flux_matches_oracle():
     """One group, P3, the slab as a second material: the first-collision source
     solved by flux-moment GMRES on the GPU and psi-space GMRES in the oracle, the

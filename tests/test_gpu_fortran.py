@@ -44,7 +44,8 @@ def test_fortran_driver_matches_oracle(tmp_path):
     assert len(dd) == 1 and float(dd[0].split(":")[1]) == 0.0
     out = np.fromfile(fout, dtype=np.float64)
     n = 4 * m.NE * ang.NI
-    psi_f, phi_f = out[:n], out[n:]
+    psi_f, phi_f = out[:n], out[n:n + m.NENK]
+    sb_f, ub_f = out[n + m.NENK:2 * n + m.NENK], out[2 * n + m.NENK:]
     # oracle: T^-1 of the same MMS source
     orc = OracleOp(m, ang)
     orc.set_physics([sigma], [[0.0] * (ang.L + 1)])
@@ -57,3 +58,13 @@ def test_fortran_driver_matches_oracle(tmp_path):
     x, st = orc.boltzmann_matinv(np.ones(n))
     phi_ref = orc.fluence(x)
     assert np.abs(phi_f - phi_ref).max() <= 1e-6 * np.abs(phi_ref).max()
+    # ExternalBeam%Source / %AddAngularFluence through the shim (the beam the
+    # driver builds: axis at theta = 0.41, phi = 2.2, no cutout)
+    axis = np.array([np.sin(0.41) * np.cos(2.2), np.sin(0.41) * np.sin(2.2), np.cos(0.41)])
+    phi0, _ = orc.beam_uncollided(axis, -3.0 * axis, [1.0])
+    sb_ref = orc.beam_source(axis, phi0, np.zeros(n))
+    assert np.abs(sb_ref).max() > 0.0
+    assert np.abs(sb_f - sb_ref).max() <= 1e-12 * np.abs(sb_ref).max()
+    phi0h, _ = orc.beam_uncollided(axis, -3.0 * axis, [0.5])
+    ub_ref = orc.beam_angular_fluence(axis, phi0h)
+    assert np.abs(ub_f - ub_ref).max() <= 1e-12 * np.abs(ub_ref).max()
