@@ -339,7 +339,7 @@ def batch_leg(reps=10, rtol=1e-6):
     return res
 
 
-def learned_pc_leg(m, ang, rtol=1e-7, fmt="bf16"):
+def learned_pc_leg(m, ang, rtol=1e-6, fmt="bf16"):
     """The 24-group P3 problem with within-group c = 0.99, solved group by
     group with plain flux-moment GMRES(30) and with flexible GMRES(30) right-
     preconditioned by the per-cell-node network, trained first on the solver's own

@@ -130,7 +130,7 @@ def build_training_set(nmesh=32, seed=2000, rows_per_snap=512, L=3, log=None):
     return X, Y, {"rows": int(X.shape[0]), "solves": info, "seconds": time.perf_counter() - t0}
 
 
-def select_threshold(params, nmesh=32, seed=2000, L=3, fmt="bf16", rtol=1e-7):
+def select_threshold(params, nmesh=32, seed=2000, L=3, fmt="bf16", rtol=1e-6):
     """The a-priori choice between the network and plain GMRES, from the training
     problems (beam sources): each is solved with flux-moment GMRES(30) and with
     FGMRES(30) and the network, both timed on the device after a warm-up solve.
