@@ -1079,6 +1079,62 @@ __global__ __launch_bounds__(kBlock) void multi_axpy_dev_kernel(int64_t n, int c
   }
 }
 
+// The update of multi_axpy_dev_kernel for at most kOrthoChunk vectors fused with
+// the dots of the next Gram-Schmidt pass (when the state asks for one): each
+// element of the updated w meets the V_i it was just updated with, so the pass
+// reads V and w once instead of twice.  Runs on the fixed grid of
+// multi_dot_partial_kernel (kOrthoBlocks) with the same element order per
+// thread, so the partial sums, and with them every bit of the dots, are those
+// of the separate kernels.
+__global__ __launch_bounds__(kBlock) void multi_axpy_dot_kernel(int64_t n, int c, const double* __restrict__ V,
+                                                                int64_t ld, const double* __restrict__ coef,
+                                                                const double* __restrict__ scale_p, const int* state,
+                                                                int mask_axpy, int mask_dot, double* __restrict__ w,
+                                                                double* __restrict__ part) {
+  if (gate_closed(state, mask_axpy)) return;
+  const bool dots = !gate_closed(state, mask_dot);
+  __shared__ double sh[kOrthoChunk + 1][kBlock / 64];
+  double h[kOrthoChunk], acc[kOrthoChunk + 1];
+#pragma unroll
+  for (int i = 0; i < kOrthoChunk; ++i) h[i] = i < c ? coef[i] : 0.0;
+#pragma unroll
+  for (int i = 0; i <= kOrthoChunk; ++i) acc[i] = 0.0;
+  const double scale = *scale_p;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    double v[kOrthoChunk];
+    double a = w[j];
+#pragma unroll
+    for (int i = 0; i < kOrthoChunk; ++i) {
+      v[i] = i < c ? V[i * ld + j] : 0.0;
+      if (i < c) a -= h[i] * v[i];
+    }
+    a *= scale;
+    w[j] = a;
+    if (dots) {
+#pragma unroll
+      for (int i = 0; i < kOrthoChunk; ++i)
+        if (i < c) acc[i] += a * v[i];
+      acc[kOrthoChunk] += a * a;
+    }
+  }
+  if (!dots) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i <= kOrthoChunk; ++i) {
+    const double s = wave_sum(acc[i]);
+    if (lane == 0) sh[i][wave] = s;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t <= c) {
+    const int i = t < c ? t : kOrthoChunk;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) s += sh[i][q];
+    part[(int64_t)t * kOrthoBlocks + blockIdx.x] = s;
+  }
+}
+
 // One wave, one lane: the scalar work of an Arnoldi step is a few hundred flops
 // in a fixed order, done serially so that its bits never vary.
 
@@ -1886,8 +1942,16 @@ int arnoldi_step(lb_op* op, int64_t n, const double* V, int k, const ArnoldiDev&
   const int* st = &a.ctl->state;
   LB_TRY(multi_dot_enqueue(op, n, V, n, c, w, a.d, nullptr, 0));
   hipLaunchKernelGGL(ortho_decide_kernel, dim3(1), dim3(64), 0, s, a.d, c, 0, a.hk, a.coef, a.ctl);
-  LB_TRY(multi_axpy_enqueue(op, n, V, n, c, a.coef, &a.ctl->scale, st, (1 << kArnDone1) | (1 << kArnPass2), w));
-  LB_TRY(multi_dot_enqueue(op, n, V, n, c, w, a.d, st, 1 << kArnPass2));
+  if (c <= kOrthoChunk) {   // the update and the second pass's dots in one read of V and w
+    hipLaunchKernelGGL(multi_axpy_dot_kernel, dim3(kOrthoBlocks), dim3(kBlock), 0, s, n, c, V, n, a.coef,
+                       &a.ctl->scale, st, (1 << kArnDone1) | (1 << kArnPass2), 1 << kArnPass2, w,
+                       op->w_red.as<double>());
+    hipLaunchKernelGGL(multi_dot_final_kernel, dim3(c + 1), dim3(kBlock), 0, s, op->w_red.as<const double>(), a.d,
+                       st, 1 << kArnPass2);
+  } else {
+    LB_TRY(multi_axpy_enqueue(op, n, V, n, c, a.coef, &a.ctl->scale, st, (1 << kArnDone1) | (1 << kArnPass2), w));
+    LB_TRY(multi_dot_enqueue(op, n, V, n, c, w, a.d, st, 1 << kArnPass2));
+  }
   hipLaunchKernelGGL(ortho_decide_kernel, dim3(1), dim3(64), 0, s, a.d, c, 1, a.hk, a.coef, a.ctl);
   LB_TRY(multi_axpy_enqueue(op, n, V, n, c, a.coef, &a.ctl->scale, st, 1 << kArnDone2, w));
   hipLaunchKernelGGL(arnoldi_givens_kernel, dim3(1), dim3(64), 0, s, a.H, a.cs, a.sn, a.g, a.hk, k, a.m + 1,
