@@ -1079,11 +1079,14 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
     LB_TRY(launch_batch_transfer(op, S, c0, B, Phi, tmp));             // in-chunk g' <= g, within-group included
     LB_TRY(launch_axpby(str, pB, 1.0, tmp, 1.0, T, T));
     LB_TRY(launch_moments_to_discrete(op, T, psi_b, c0, B, 0, false));
-    LB_TRY(launch_sweep_direction(op, psi_b, psi_b, c0, B, 1024));    // psi = T^-1 (M S phi + q)
+    // psi = T^-1 (M S phi + q); the direction-resident kernel at 512 threads, which
+    // sweeps 8 groups of 384k tets x 256 directions in 20.8 ms against 22.4 ms at
+    // 1024 (graph-replayed level launches: 20.1 ms, but a capture per buffer pair)
+    LB_TRY(launch_sweep_direction(op, psi_b, psi_b, c0, B, 512));
     LB_TRY(launch_moments(op, psi_b, tmp, B, op->NENK * B));           // D psi
     LB_TRY(launch_axpby(str, pB, 1.0, tmp, -1.0, Phi, tmp));            // D psi - phi
     LB_TRY(launch_moments_to_discrete(op, tmp, r_b, c0, B, 0, true));  // M S_g (D psi - phi)
-    LB_TRY(launch_sweep_direction(op, r_b, r_b, c0, B, 1024));
+    LB_TRY(launch_sweep_direction(op, r_b, r_b, c0, B, 512));
     double rr[kTailGroups];
     LB_TRY(group_dot(op, nB, B, r_b, r_b, rr));
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
