@@ -98,37 +98,58 @@ struct CellPrep {
   int4 us;
 };
 
-template <bool NT, bool LDS = false>
-__device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
-  const int64_t k = (int64_t)i * A.NE + s;
-  if constexpr (LDS) {
-    P.ws = A.wslot[k];
-    const short4 us = A.uslot[k];
-    P.us = make_int4(us.x, us.y, us.z, us.w);
-  }
-  const u32x4* q = reinterpret_cast<const u32x4*>(A.rec + k);
+// A cell's record as loaded (and, in the LDS sweep, its slot-plan entries);
+// decode_rec turns it into the CellPrep and issues the loads that depend on it.
+struct RecRaw {
   u32x4 a, b, c;
+  int ws;
+  short4 us;
+};
+
+template <bool NT, bool LDS>
+__device__ __forceinline__ void load_rec(const SweepArgs& A, int i, int s, RecRaw& R) {
+  const int64_t k = (int64_t)i * A.NE + s;
+  const u32x4* q = reinterpret_cast<const u32x4*>(A.rec + k);
   if constexpr (NT) {
-    a = __builtin_nontemporal_load(q);
-    b = __builtin_nontemporal_load(q + 1);
-    c = __builtin_nontemporal_load(q + 2);
+    R.a = __builtin_nontemporal_load(q);
+    R.b = __builtin_nontemporal_load(q + 1);
+    R.c = __builtin_nontemporal_load(q + 2);
   } else {
-    a = q[0];
-    b = q[1];
-    c = q[2];
+    R.a = q[0];
+    R.b = q[1];
+    R.c = q[2];
   }
-  P.w = a.x;
-  P.pm = a.y & 0xFFFFFFu;
-  P.nb = make_int4((int)a.z, (int)a.w, (int)b.x, (int)b.y);
-  P.ck[0] = dbl_of(b.z, b.w);
-  P.ck[1] = dbl_of(c.x, c.y);
-  P.ck[2] = dbl_of(c.z, c.w);
+  if constexpr (LDS) {
+    R.ws = A.wslot[k];
+    R.us = A.uslot[k];
+  }
+}
+
+template <bool NT, bool LDS>
+__device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int g, const RecRaw& R, CellPrep& P) {
+  if constexpr (LDS) {
+    P.ws = R.ws;
+    P.us = make_int4(R.us.x, R.us.y, R.us.z, R.us.w);
+  }
+  P.w = R.a.x;
+  P.pm = R.a.y & 0xFFFFFFu;
+  P.nb = make_int4((int)R.a.z, (int)R.a.w, (int)R.b.x, (int)R.b.y);
+  P.ck[0] = dbl_of(R.b.z, R.b.w);
+  P.ck[1] = dbl_of(R.c.x, R.c.y);
+  P.ck[2] = dbl_of(R.c.z, R.c.w);
   P.ck[3] = -((P.ck[0] + P.ck[1]) + P.ck[2]);
-  P.sig = A.sigt[(int64_t)(a.y >> 24) * A.Gtot + A.g0 + g];
+  P.sig = A.sigt[(int64_t)(R.a.y >> 24) * A.Gtot + A.g0 + g];
   const int e = int(P.w & 0xFFFFFFu);
   const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
 #pragma unroll
   for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
+}
+
+template <bool NT, bool LDS = false>
+__device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
+  RecRaw R;
+  load_rec<NT, LDS>(A, i, s, R);
+  decode_rec<NT, LDS>(A, i, g, R, P);
 }
 
 // Where a cell reads its upstream traces and what it writes besides the flux.
@@ -508,98 +529,119 @@ __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int
   }
 }
 
-// One level in upstream-first order: the trace loads of the level's cells are
-// issued before the look-ahead loads of the next level, so that waiting for the
-// traces (the vector-memory counter retires in order) does not also wait for
-// the look-ahead, which then completes while the cells are solved.  The
-// look-ahead issues the same loads on every path (lanes past the next level's
-// end re-read its last cell), which keeps the compiler's counter waits exact.
-// The staging copy for the next level (LDS sweep) is issued after the
-// look-ahead and written to LDS after the solves.
-template <int R, bool NT, bool LDS>
-__device__ __forceinline__ void level_upstream_first(const SweepArgs& A, int i, int s0, int s1, int n0, int n1,
-                                                     int cpr, int c, int g, bool lane_ok, const CellPrep (&P)[R],
-                                                     CellPrep (&Pn)[R], const TraceRing& ring,
-                                                     const int32_t* __restrict__ so, int lnext) {
+// Cells [s0, s1) of level l: one chunk of a direction's walk.
+struct Chunk {
+  int l, s0, s1;
+};
+
+// The chunk after c (the rest of its level, or the first chunk of the next
+// level; levels are never empty); c itself, with more = false, after the last.
+__device__ __forceinline__ Chunk next_chunk(const int32_t* __restrict__ off, int nl, int chunk, const Chunk& c,
+                                            bool& more) {
+  const bool last = c.s1 == off[c.l + 1];
+  const int ln = last ? c.l + 1 : c.l;
+  more = ln < nl;
+  if (!more) return c;
+  const int n0 = last ? off[ln] : c.s1;
+  return Chunk{ln, n0, min(n0 + chunk, off[ln + 1])};
+}
+
+// One chunk of the two-stage walk.  On entry P holds the current chunk (records
+// decoded, cross sections and sources loaded one chunk ago) and raw the records
+// of the next chunk (loaded one chunk ago).  The chunk's upstream traces are read
+// first, LDS slots before global memory, so that the LDS reads do not wait on
+// the vector-memory counter for loads into the same registers.  Then raw is
+// decoded into Pn and the next chunk's cross sections and sources are issued, the
+// records of the chunk after it go into raw, the staging copy for the next level
+// is issued, and only then are the current cells solved: every load a solve
+// waits for was issued a chunk earlier, and no load address waits on a load of
+// the same chunk.  The look-ahead issues the same loads on every path (lanes past
+// a chunk's end re-read its last cell).
+template <int R, bool LDS>
+__device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
+                                                int chunk, int cpr, int c, int g, bool lane_ok, Chunk& cur, Chunk& nx,
+                                                bool& has_nx, const CellPrep (&P)[R], CellPrep (&Pn)[R],
+                                                const RecRaw (&raw)[R], RecRaw (&raw_next)[R], const TraceRing& ring,
+                                                const int32_t* __restrict__ so) {
   constexpr int kSolve = LDS ? kTraceGlobalRing : kTraceGlobal;
   double u[R][4][3];
+  unsigned hit = 0;
+  if constexpr (LDS) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int s = s0 + c + r * cpr;
-    if (lane_ok && s < s1) {
-      if (LDS && ring_hit(P[r]))
+    for (int r = 0; r < R; ++r) {
+      const int s = cur.s0 + c + r * cpr;
+      if (lane_ok && s < cur.s1 && ring_hit(P[r])) {
+        hit |= 1u << r;
         load_upstream<kTraceRing>(A, i, P[r], g, ring, u[r]);
-      else
-        load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+      }
     }
+    // opaque to the optimiser: the global reads stay a branch of their own after the LDS reads
+    asm volatile("" : "+v"(hit));
   }
-#pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(n0 + c + r * cpr, n1 - 1), g, Pn[r]);
-  const StageCopy sc = stage_load<LDS>(A, i, so, lnext);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int s = s0 + c + r * cpr;
-    if (lane_ok && s < s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+    const int s = cur.s0 + c + r * cpr;
+    if (lane_ok && s < cur.s1 && !(hit >> r & 1u)) load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
   }
-  if (lane_ok)
-    for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
+#pragma unroll
+  for (int r = 0; r < R; ++r) decode_rec<false, LDS>(A, i, g, raw[r], Pn[r]);
+  bool more2 = false;
+  const Chunk nn = has_nx ? next_chunk(off, nl, chunk, nx, more2) : nx;
+#pragma unroll
+  for (int r = 0; r < R; ++r) load_rec<false, LDS>(A, i, min(nn.s0 + c + r * cpr, nn.s1 - 1), raw_next[r]);
+  const bool last = cur.s1 == off[cur.l + 1];
+  const StageCopy sc = stage_load<LDS>(A, i, so, last ? cur.l + 1 : -1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = cur.s0 + c + r * cpr;
+    if (lane_ok && s < cur.s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+  }
   stage_store<LDS>(ring, sc);
-}
-
-// One chunk of the upstream-first walk: cells [s0, s1) of level l, at most
-// R * cpr of them.  Its look-ahead is the next chunk, of the same level or the
-// first of the next level; only the last chunk of a level is followed by the
-// barrier (cells of a level are independent) and issues the staging copy for
-// the next level.  Advances (l, s0, s1) and returns false after the last chunk.
-template <int R, bool NT, bool LDS>
-__device__ __forceinline__ bool chunk_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
-                                                     int nl, int chunk, int cpr, int c, int g, bool lane_ok,
-                                                     int& l, int& s0, int& s1, const CellPrep (&P)[R],
-                                                     CellPrep (&Pn)[R], const TraceRing& ring,
-                                                     const int32_t* __restrict__ so) {
-  const bool last = s1 == off[l + 1];
-  const int ln = last ? l + 1 : l;
-  const bool more = ln < nl;
-  const int n0 = !more ? s0 : (last ? off[ln] : s1);
-  const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
-  level_upstream_first<R, NT, LDS>(A, i, s0, s1, n0, n1, cpr, c, g, lane_ok, P, Pn, ring, so, last ? l + 1 : -1);
   if (last) __syncthreads();
-  l = ln;
-  s0 = n0;
-  s1 = n1;
-  return more;
+  if (!has_nx) return false;
+  cur = nx;
+  nx = nn;
+  has_nx = more2;
+  return true;
 }
 
-// Levels larger than R * cpr cells are walked in chunks of that size, so that
-// every cell's static data is fetched one chunk ahead (the short, wide
-// directions of a mesh have levels of thousands of cells).
-template <int BLOCK, int R, bool NT, bool LDS = false>
-__device__ __forceinline__ void walk_levels_upstream_first(const SweepArgs& A, int i, const int32_t* __restrict__ off,
-                                                           int nl, TraceRing ring = TraceRing{nullptr},
-                                                           const int32_t* __restrict__ so = nullptr) {
+// Direction walk with a two-stage look-ahead (few groups per cell): records two
+// chunks ahead of their solve, cross sections and sources one chunk ahead.
+// Measured on 384k tets x 256 directions against a one-stage look-ahead (whose
+// source loads waited on the record loads of the same chunk): one group, LDS
+// kernel at 512 threads 5.07 -> 4.70 ms per sweep inside the solve, at 256
+// threads 6.23 -> 5.62 ms; plain direction kernel at 256 threads 6.79 -> 6.43
+// ms; two groups at 512 threads 8.58 -> 8.13 ms.
+template <int BLOCK, int R, bool LDS = false>
+__device__ __forceinline__ void walk_levels_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off,
+                                                      int nl, TraceRing ring = TraceRing{nullptr},
+                                                      const int32_t* __restrict__ so = nullptr) {
   const int cpr = BLOCK / A.G;
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
   const int chunk = R * cpr;
-  CellPrep Pa[R], Pb[R];
   if (nl <= 0) return;
-  int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
+  CellPrep Pa[R], Pb[R];
+  RecRaw ra[R], rb[R];
+  Chunk cur{0, off[0], min(off[0] + chunk, off[1])};
+  bool has_nx = false;
+  Chunk nx = next_chunk(off, nl, chunk, cur, has_nx);
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(s0 + c + r * cpr, s1 - 1), g, Pa[r]);
+  for (int r = 0; r < R; ++r) prep_cell<false, LDS>(A, i, min(cur.s0 + c + r * cpr, cur.s1 - 1), g, Pa[r]);
+#pragma unroll
+  for (int r = 0; r < R; ++r) load_rec<false, LDS>(A, i, min(nx.s0 + c + r * cpr, nx.s1 - 1), ra[r]);
   while (true) {
-    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so))
+    if (!chunk_two_stage<R, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pa, Pb, ra, rb, ring, so))
       break;
-    if (!chunk_upstream_first<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so))
+    if (!chunk_two_stage<R, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pb, Pa, rb, ra, ring, so))
       break;
   }
 }
 
 // Rounds of static data held per lane for a workgroup size (register budget:
 // a 1024-lane workgroup leaves 128 VGPRs per lane; a 256-lane one runs one wave
-// per SIMD and holds four rounds in 256 VGPRs without spills, so levels up to
-// 1,024 cells stay on the look-ahead path: 7.55 -> 7.24 ms for one group on
-// 384k tets x 256 directions).
+// per SIMD and holds four rounds in 256 VGPRs without spills).
 template <int BLOCK>
 struct LevelRounds {
   static constexpr int value = BLOCK >= 1024 ? 1 : (BLOCK >= 512 ? 2 : 4);
@@ -616,13 +658,12 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
                                                                 const int32_t* __restrict__ nlev, int maxlev,
                                                                 SweepArgs A) {
   const int i = blockIdx.x;
-  // upstream-first order pays with few groups per cell (measured: 2 groups 10.5 -> 9.7 ms on 384k tets x 256
-  // directions) and costs with many (24 groups at 1024 threads 52.4 -> 55.3 ms)
+  // many groups per cell: one step per level (reading a level's traces before its look-ahead cost 24 groups
+  // at 1024 threads 52.4 -> 55.3 ms on 384k tets x 256 directions)
   if constexpr (NT)
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
   else
-    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1),
-                                                                     nlev[i]);
+    walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
 }
 
 // Direction-resident sweep with LDS trace slots (dynamic shared memory of
@@ -641,8 +682,8 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
                                                             TraceRing{ring_buf}, so);
   else
-    walk_levels_upstream_first<BLOCK, LevelRounds<BLOCK>::value, NT, true>(
-        A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf}, so);
+    walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
+                                                                  TraceRing{ring_buf}, so);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
