@@ -457,75 +457,39 @@ __device__ __forceinline__ void stage_store(const TraceRing& ring, const StageCo
   }
 }
 
-// One chunk of the plain walk: cells [s0, s1) of level l, at most R * cpr of
-// them, with the next chunk (of the same level or the first of the next one)
-// loaded into the second register buffer before they are solved.  Only the last
-// chunk of a level is followed by the barrier and issues the staging copy for
-// the next level (LDS sweep).  Advances (l, s0, s1); false after the last chunk.
-template <int R, bool NT, bool LDS>
-__device__ __forceinline__ bool chunk_plain(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
-                                            int chunk, int cpr, int c, int g, bool lane_ok, int& l, int& s0, int& s1,
-                                            const CellPrep (&P)[R], CellPrep (&Pn)[R], const TraceRing& ring,
-                                            const int32_t* __restrict__ so) {
-  const bool last = s1 == off[l + 1];
-  const int ln = last ? l + 1 : l;
-  const bool more = ln < nl;
-  const int n0 = !more ? s0 : (last ? off[ln] : s1);
-  const int n1 = !more ? s1 : min(n0 + chunk, off[ln + 1]);
-  if (more) prep_level<R, NT, LDS>(A, i, n0, n1, cpr, c, g, lane_ok, Pn);
-  const StageCopy sc = stage_load<LDS>(A, i, so, last ? l + 1 : -1);
-  finish_level<R, NT, LDS>(A, i, s0, s1, cpr, c, g, lane_ok, P, ring);
-  stage_store<LDS>(ring, sc);
-  if (last) __syncthreads();
-  l = ln;
-  s0 = n0;
-  s1 = n1;
-  return more;
-}
-
 // Walks the levels of one direction (positions off[0..nl]) with the whole
-// workgroup: BLOCK / G cells per round, R rounds per chunk (with few groups per
-// cell), a workgroup barrier between levels.  The static data of the next chunk (records, c_f k.n, cross
-// section, source) is loaded into the second of two register buffers before
-// the current chunk is solved, so its latency overlaps the current chunk's
-// upstream loads and solves; only the upstream-trace loads of a level wait for
-// the barrier.  so: the direction's staging offsets (LDS sweep only).
+// workgroup when each cell carries many groups (kStreamGroups or more): BLOCK / G
+// cells per round, one step per level, a workgroup barrier between levels.  The
+// static data of the next level's first R rounds (records, c_f k.n, cross
+// section, source) is loaded into the second of two register buffers before the
+// current level is solved; rounds beyond R are loaded and solved in place
+// (measured: 24 groups at 1024 threads 56.4 ms per sweep, against 80.7 ms in
+// chunks of R rounds, which spill more).  so: the direction's staging offsets
+// (LDS sweep only).  Few groups per cell take walk_levels_two_stage.
 template <int BLOCK, int R, bool NT, bool LDS = false>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
                                             TraceRing ring = TraceRing{nullptr},
                                             const int32_t* __restrict__ so = nullptr) {
+  static_assert(NT, "few groups per cell take walk_levels_two_stage");
   const int cpr = BLOCK / A.G;   // cells per round
   const int c = threadIdx.x / A.G;
   const int g = threadIdx.x - c * A.G;
   const bool lane_ok = c < cpr;
   CellPrep Pa[R], Pb[R];
   if (nl <= 0) return;
-  if constexpr (NT) {
-    // Many groups per cell: one step per level, its first R rounds fetched ahead
-    // and the rest solved in place (measured: 24 groups at 1024 threads 56.4 ms
-    // per sweep, against 80.7 ms in chunks of R rounds, which spill more).
-    prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
-    for (int l = 0; l < nl; l += 2) {
-      if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
-      const StageCopy sa = stage_load<LDS>(A, i, so, l + 1);
-      finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
-      stage_store<LDS>(ring, sa);
-      __syncthreads();
-      if (l + 1 >= nl) break;
-      if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
-      const StageCopy sb = stage_load<LDS>(A, i, so, l + 2);
-      finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
-      stage_store<LDS>(ring, sb);
-      __syncthreads();
-    }
-  } else {
-    const int chunk = R * cpr;
-    int l = 0, s0 = off[0], s1 = min(off[0] + chunk, off[1]);
-    prep_level<R, NT, LDS>(A, i, s0, s1, cpr, c, g, lane_ok, Pa);
-    while (true) {
-      if (!chunk_plain<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pa, Pb, ring, so)) break;
-      if (!chunk_plain<R, NT, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, l, s0, s1, Pb, Pa, ring, so)) break;
-    }
+  prep_level<R, NT, LDS>(A, i, off[0], off[1], cpr, c, g, lane_ok, Pa);
+  for (int l = 0; l < nl; l += 2) {
+    if (l + 1 < nl) prep_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb);
+    const StageCopy sa = stage_load<LDS>(A, i, so, l + 1);
+    finish_level<R, NT, LDS>(A, i, off[l], off[l + 1], cpr, c, g, lane_ok, Pa, ring);
+    stage_store<LDS>(ring, sa);
+    __syncthreads();
+    if (l + 1 >= nl) break;
+    if (l + 2 < nl) prep_level<R, NT, LDS>(A, i, off[l + 2], off[l + 3], cpr, c, g, lane_ok, Pa);
+    const StageCopy sb = stage_load<LDS>(A, i, so, l + 2);
+    finish_level<R, NT, LDS>(A, i, off[l + 1], off[l + 2], cpr, c, g, lane_ok, Pb, ring);
+    stage_store<LDS>(ring, sb);
+    __syncthreads();
   }
 }
 
@@ -706,7 +670,12 @@ __global__ __launch_bounds__(BLOCK) void sweep_persistent_kernel(const int4* __r
     if (it >= nitems) break;
     const int4 d = items[it];
     if (active[d.x] == 0) continue;
-    walk_levels<BLOCK, 1, NT>(A, d.y, lvl + d.z, d.w);   // one round: several workgroups share a CU
+    // one round: several workgroups share a CU (two-stage walk, few groups: 10.96-11.01 ms per sweep of the
+    // 64-problem batch against 10.93-10.94 ms with the one-stage walk it replaced, same box)
+    if constexpr (NT)
+      walk_levels<BLOCK, 1, NT>(A, d.y, lvl + d.z, d.w);
+    else
+      walk_levels_two_stage<BLOCK, 1>(A, d.y, lvl + d.z, d.w);
   }
 }
 
