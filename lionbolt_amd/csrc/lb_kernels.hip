@@ -521,7 +521,7 @@ __device__ __forceinline__ Chunk next_chunk(const int32_t* __restrict__ off, int
 // waits for was issued a chunk earlier, and no load address waits on a load of
 // the same chunk.  The look-ahead issues the same loads on every path (lanes past
 // a chunk's end re-read its last cell).
-template <int R, bool LDS>
+template <int R, bool LDS, bool NT = false>
 __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
                                                 int chunk, int cpr, int c, int g, bool lane_ok, Chunk& cur, Chunk& nx,
                                                 bool& has_nx, const CellPrep (&P)[R], CellPrep (&Pn)[R],
@@ -548,11 +548,11 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
     if (lane_ok && s < cur.s1 && !(hit >> r & 1u)) load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) decode_rec<false, LDS>(A, i, g, raw[r], Pn[r]);
+  for (int r = 0; r < R; ++r) decode_rec<NT, LDS>(A, i, g, raw[r], Pn[r]);
   bool more2 = false;
   const Chunk nn = has_nx ? next_chunk(off, nl, chunk, nx, more2) : nx;
 #pragma unroll
-  for (int r = 0; r < R; ++r) load_rec<false, LDS>(A, i, min(nn.s0 + c + r * cpr, nn.s1 - 1), raw_next[r]);
+  for (int r = 0; r < R; ++r) load_rec<NT, LDS>(A, i, min(nn.s0 + c + r * cpr, nn.s1 - 1), raw_next[r]);
   const bool last = cur.s1 == off[cur.l + 1];
   const StageCopy sc = stage_load<LDS>(A, i, so, last ? cur.l + 1 : -1);
 #pragma unroll
@@ -576,7 +576,7 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
 // kernel at 512 threads 5.07 -> 4.70 ms per sweep inside the solve, at 256
 // threads 6.23 -> 5.62 ms; plain direction kernel at 256 threads 6.79 -> 6.43
 // ms; two groups at 512 threads 8.58 -> 8.13 ms.
-template <int BLOCK, int R, bool LDS = false>
+template <int BLOCK, int R, bool LDS = false, bool NT = false>
 __device__ __forceinline__ void walk_levels_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                       int nl, TraceRing ring = TraceRing{nullptr},
                                                       const int32_t* __restrict__ so = nullptr) {
@@ -592,13 +592,15 @@ __device__ __forceinline__ void walk_levels_two_stage(const SweepArgs& A, int i,
   bool has_nx = false;
   Chunk nx = next_chunk(off, nl, chunk, cur, has_nx);
 #pragma unroll
-  for (int r = 0; r < R; ++r) prep_cell<false, LDS>(A, i, min(cur.s0 + c + r * cpr, cur.s1 - 1), g, Pa[r]);
+  for (int r = 0; r < R; ++r) prep_cell<NT, LDS>(A, i, min(cur.s0 + c + r * cpr, cur.s1 - 1), g, Pa[r]);
 #pragma unroll
-  for (int r = 0; r < R; ++r) load_rec<false, LDS>(A, i, min(nx.s0 + c + r * cpr, nx.s1 - 1), ra[r]);
+  for (int r = 0; r < R; ++r) load_rec<NT, LDS>(A, i, min(nx.s0 + c + r * cpr, nx.s1 - 1), ra[r]);
   while (true) {
-    if (!chunk_two_stage<R, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pa, Pb, ra, rb, ring, so))
+    if (!chunk_two_stage<R, LDS, NT>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pa, Pb, ra, rb, ring,
+                                     so))
       break;
-    if (!chunk_two_stage<R, LDS>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pb, Pa, rb, ra, ring, so))
+    if (!chunk_two_stage<R, LDS, NT>(A, i, off, nl, chunk, cpr, c, g, lane_ok, cur, nx, has_nx, Pb, Pa, rb, ra, ring,
+                                     so))
       break;
   }
 }
@@ -622,12 +624,11 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
                                                                 const int32_t* __restrict__ nlev, int maxlev,
                                                                 SweepArgs A) {
   const int i = blockIdx.x;
-  // many groups per cell: one step per level (reading a level's traces before its look-ahead cost 24 groups
-  // at 1024 threads 52.4 -> 55.3 ms on 384k tets x 256 directions)
-  if constexpr (NT)
-    walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
-  else
-    walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i]);
+  // the two-stage walk at any group count (8 groups, the multigroup solve's tail, at 512 threads: 19.6 -> 17.3 ms
+  // per sweep of 384k tets x 256 directions against one step per level; 24 groups 57.4 -> 54.8 ms at 512
+  // threads, 53.7 -> 55.7 ms at 1024, where level launches stay faster)
+  walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, false, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1),
+                                                                    nlev[i]);
 }
 
 // Direction-resident sweep with LDS trace slots (dynamic shared memory of
