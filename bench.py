@@ -688,7 +688,7 @@ def main():
     p.add_argument("--calib", action="store_true", help="run a known-size copy first (PMC scaling)")
     p.add_argument("--sweep-mode", default="auto",
                    choices=["auto", "levels", "graph", "direction", "direction_lds"])
-    p.add_argument("--block", type=int, default=1024, choices=[256, 512, 1024],
+    p.add_argument("--block", type=int, default=1024, choices=[256, 512, 768, 1024],
                    help="workgroup size of the direction-resident sweep modes")
     p.add_argument("--compare-modes", action="store_true")
     p.add_argument("--gemm-study", action="store_true", help="scattering-moment products on MFMA, all formats")

@@ -307,7 +307,8 @@ int lb_beam_source(lb_op* op, const lb_beam_desc* beam, double* s, int64_t n);
  * Synchronous. */
 int lb_beam_add_angular_fluence(lb_op* op, const lb_beam_desc* beam, const double* attn, double* unc, int64_t n);
 
-/* Workgroup size of the direction-resident sweep: 256, 512 or 1024 (default). */
+/* Workgroup size of the direction-resident sweeps: 256, 512, 768 or 1024
+ * (default); 768 is not a persistent-sweep size. */
 int lb_op_set_direction_block(lb_op* op, int threads);
 
 /* TransportOp%MatInv (TransportOperator.f90:72-151): s <- T^-1 s in place. */

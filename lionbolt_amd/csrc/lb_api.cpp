@@ -559,8 +559,8 @@ int lb_batch_solve_dev(lb_op* op, const double* d_qmom, const lb_solver_opts* op
 
 int lb_op_set_direction_block(lb_op* op, int threads) {
   return guarded([&]() -> int {
-    if (!op || (threads != 256 && threads != 512 && threads != 1024)) {
-      set_error("lb_op_set_direction_block: threads must be 256, 512 or 1024");
+    if (!op || (threads != 256 && threads != 512 && threads != 768 && threads != 1024)) {
+      set_error("lb_op_set_direction_block: threads must be 256, 512, 768 or 1024");
       return LB_E_ARG;
     }
     op->dir_block = threads;

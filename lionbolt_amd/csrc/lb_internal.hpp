@@ -92,11 +92,13 @@ struct SweepGraph {
 struct SweepConfig {
   int mode, block, per_cu;
 };
-// (The LDS sweep at 1024 threads spills registers and is not timed.)
-constexpr int kTuneConfigs = 6;
+// (The LDS sweep at 1024 threads spills registers and is not timed; at 768 it
+// runs three waves per SIMD on one round of look-ahead.)
+constexpr int kTuneConfigs = 7;
 constexpr SweepConfig kTuneConfig[kTuneConfigs] = {
-    {LB_SWEEP_GRAPH, 0, 0},        {LB_SWEEP_DIRECTION, 256, 0},     {LB_SWEEP_DIRECTION, 512, 0},
-    {LB_SWEEP_DIRECTION, 1024, 0}, {LB_SWEEP_DIRECTION_LDS, 256, 0}, {LB_SWEEP_DIRECTION_LDS, 512, 0}};
+    {LB_SWEEP_GRAPH, 0, 0},           {LB_SWEEP_DIRECTION, 256, 0},     {LB_SWEEP_DIRECTION, 512, 0},
+    {LB_SWEEP_DIRECTION, 1024, 0},    {LB_SWEEP_DIRECTION_LDS, 256, 0}, {LB_SWEEP_DIRECTION_LDS, 512, 0},
+    {LB_SWEEP_DIRECTION_LDS, 768, 0}};
 constexpr int kBatchTuneConfigs = 8;
 constexpr SweepConfig kBatchTuneConfig[kBatchTuneConfigs] = {
     {LB_SWEEP_GRAPH, 0, 0},         {LB_SWEEP_PERSISTENT, 256, 4},  {LB_SWEEP_PERSISTENT, 256, 8},

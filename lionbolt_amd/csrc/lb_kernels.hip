@@ -616,11 +616,15 @@ __device__ __forceinline__ void walk_levels_two_stage(const SweepArgs& A, int i,
 }
 
 // Rounds of static data held per lane for a workgroup size (register budget:
-// a 1024-lane workgroup leaves 128 VGPRs per lane; a 256-lane one runs one wave
-// per SIMD and holds four rounds in 256 VGPRs without spills).
+// a 1024-lane workgroup leaves 128 VGPRs per lane; a 768-lane one 168, where one
+// round of the two-stage walk fits in 146 and three waves share each SIMD; a
+// 256-lane one runs one wave per SIMD and holds four rounds in 256 VGPRs
+// without spills).  The LDS sweep of one group at 768 threads against 512
+// (two rounds, two waves per SIMD): the 24-group solve 1.241 s against 1.319-
+// 1.334 s on one box.
 template <int BLOCK>
 struct LevelRounds {
-  static constexpr int value = BLOCK >= 1024 ? 1 : (BLOCK >= 512 ? 2 : 4);
+  static constexpr int value = BLOCK >= 768 ? 1 : (BLOCK >= 512 ? 2 : 4);
 };
 
 // Direction-resident sweep: one workgroup owns one direction (of one batch of
@@ -1288,6 +1292,8 @@ static void launch_direction_nt(int block, unsigned ndir, hipStream_t st, const 
     hipLaunchKernelGGL((sweep_direction_kernel<256, NT>), dim3(ndir), dim3(256), 0, st, lo, nl, maxlev, A);
   else if (block == 512)
     hipLaunchKernelGGL((sweep_direction_kernel<512, NT>), dim3(ndir), dim3(512), 0, st, lo, nl, maxlev, A);
+  else if (block == 768)
+    hipLaunchKernelGGL((sweep_direction_kernel<768, NT>), dim3(ndir), dim3(768), 0, st, lo, nl, maxlev, A);
   else
     hipLaunchKernelGGL((sweep_direction_kernel<1024, NT>), dim3(ndir), dim3(1024), 0, st, lo, nl, maxlev, A);
 }
@@ -1305,12 +1311,15 @@ static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, con
                                    int maxlev, const SweepArgs& A, size_t bytes) {
   const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
                    : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
+                   : block == 768 ? (const void*)&sweep_direction_lds_kernel<768, NT>
                                   : (const void*)&sweep_direction_lds_kernel<1024, NT>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
     hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A);
   else if (block == 512)
     hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A);
+  else if (block == 768)
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<768, NT>), dim3(ndir), dim3(768), bytes, st, lo, nl, maxlev, A);
   else
     hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A);
   return LB_OK;
@@ -1459,8 +1468,10 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
 static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
   const int mode = cfg.mode, block = cfg.block;
   if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
-      !((block == 256 || block == 512 || block == 1024) && block / G >= 1)) {
-    set_error("sweep: the workgroup size must be 256, 512 or 1024 and hold the group batch");
+      !((block == 256 || block == 512 || block == 1024 || (block == 768 && mode != LB_SWEEP_PERSISTENT)) &&
+        block / G >= 1)) {
+    set_error("sweep: the workgroup size must be 256, 512, 768 (direction-resident only) or 1024 and hold the "
+              "group batch");
     return LB_E_ARG;
   }
   if (mode == LB_SWEEP_PERSISTENT) {

@@ -12,7 +12,8 @@ from lionbolt_amd import angular, mesh as M
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("G,block", [(1, 256), (1, 512), (1, 1024), (2, 512), (5, 512), (24, 1024), (24, 256)])
+@pytest.mark.parametrize("G,block", [(1, 256), (1, 512), (1, 768), (1, 1024), (2, 512), (3, 768), (5, 512),
+                                     (24, 1024), (24, 768), (24, 256)])
 def test_direction_resident_equals_level_launches(G, block):
     import torch
     from lionbolt_amd.ops import Operator
@@ -56,13 +57,13 @@ def test_lds_slots_with_staging_and_global_reads(G):
     out = {}
     for mode in ("levels", "direction_lds"):
         op.set_sweep_mode(mode)
-        for block in (256, 512):
+        for block in (256, 512, 768):
             op.set_direction_block(block)
             psi = torch.full_like(q, float("nan"))
             op.transport_matinv_dev(q, psi, 0, G)
             op.synchronize()
             out[(mode, block)] = psi.cpu().numpy()
-    for block in (256, 512):
+    for block in (256, 512, 768):
         assert np.array_equal(out[("direction_lds", block)], out[("levels", 256)])
 
 
