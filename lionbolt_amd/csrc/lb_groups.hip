@@ -134,6 +134,51 @@ __global__ __launch_bounds__(kBlock) void group_dot_partial_kernel(int64_t n, in
   if ((int)threadIdx.x < B) part[(int64_t)blockIdx.x * kMaxGroupBatch + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// The same for B = 8 and 16-byte aligned vectors: each lane takes whole rows of
+// B values (one per group), four 16-byte loads per operand and row, so the
+// group of every value is its position in the row.
+template <int B>
+__global__ __launch_bounds__(kBlock) void group_dot_rows_kernel(int64_t rows, const double* __restrict__ x,
+                                                                const double* __restrict__ y, int same,
+                                                                double* __restrict__ part) {
+  __shared__ double sh[B][kBlock];
+  double acc[B];
+#pragma unroll
+  for (int g = 0; g < B; ++g) acc[g] = 0.0;
+  const dbl2* xv = reinterpret_cast<const dbl2*>(x);
+  const dbl2* yv = reinterpret_cast<const dbl2*>(y);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < rows; r += stride) {
+    dbl2 a[B / 2], b[B / 2];
+#pragma unroll
+    for (int h = 0; h < B / 2; ++h) a[h] = __builtin_nontemporal_load(xv + r * (B / 2) + h);
+    if (same) {
+#pragma unroll
+      for (int h = 0; h < B / 2; ++h) b[h] = a[h];
+    } else {
+#pragma unroll
+      for (int h = 0; h < B / 2; ++h) b[h] = __builtin_nontemporal_load(yv + r * (B / 2) + h);
+    }
+#pragma unroll
+    for (int h = 0; h < B / 2; ++h) {
+      acc[2 * h] += a[h].x * b[h].x;
+      acc[2 * h + 1] += a[h].y * b[h].y;
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < B; ++g) sh[g][threadIdx.x] = acc[g];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int g = 0; g < B; ++g) sh[g][threadIdx.x] += sh[g][threadIdx.x + s];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < B) part[(int64_t)blockIdx.x * kMaxGroupBatch + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 __global__ void group_dot_final_kernel(int nblocks, int B, const double* __restrict__ part, double* __restrict__ out) {
   const int g = threadIdx.x;
   if (g >= B) return;
@@ -209,7 +254,14 @@ int group_dot(lb_op* op, int64_t n, int B, const double* x, const double* y, dou
   }
   double* part = op->w_red.as<double>();
   double* res = part + (size_t)kGroupDotBlocks * kMaxGroupBatch;
-  hipLaunchKernelGGL(group_dot_partial_kernel, dim3(kGroupDotBlocks), dim3(kBlock), 0, op->stream, n, B, x, y, part);
+  const bool rows8 = B == 8 && n % 8 == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(y) & 15) == 0;
+  if (rows8)
+    hipLaunchKernelGGL(group_dot_rows_kernel<8>, dim3(kGroupDotBlocks), dim3(kBlock), 0, op->stream, n / 8, x, y,
+                       x == y ? 1 : 0, part);
+  else
+    hipLaunchKernelGGL(group_dot_partial_kernel, dim3(kGroupDotBlocks), dim3(kBlock), 0, op->stream, n, B, x, y,
+                       part);
   hipLaunchKernelGGL(group_dot_final_kernel, dim3(1), dim3(64), 0, op->stream, kGroupDotBlocks, B, part, res);
   LB_HIP(hipGetLastError());
   LB_HIP(hipMemcpyAsync(op->w_host.p, res, B * sizeof(double), hipMemcpyDeviceToHost, op->stream));
