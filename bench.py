@@ -130,11 +130,28 @@ def settle_autotuner(op, m, ang, G=1):
         if op.sweep_choice(G)[0] != "auto":
             break
         op.transport_matinv_dev(q, psi, 0, G)
+    if G == 1:
+        # the within-group solves' sweeps, which store the flux (and maybe read the source)
+        # in sweep order, are timed apart
+        for _ in range(32):
+            if op.sweep_order_choice()[0] != "auto":
+                break
+            op.transport_matinv_order_dev(q, psi, 0)
+        psi.copy_(q)
+        for _ in range(32):
+            if op.sweep_order_choice(True)[0] != "auto":
+                break
+            op.transport_matinv_order_dev(psi, psi, 0, source_in_order=True)
     op.synchronize()
     dt = time.perf_counter() - t0
     del q, psi
     torch.cuda.empty_cache()
     return dt
+
+
+# Layouts of the within-group angular flux (Operator.set_flux_order); the library's default first.
+FLUX_LAYOUTS = ("sweep", "sweep_source", "natural")
+DEFAULT_FLUX_LAYOUT = FLUX_LAYOUTS[0]
 
 
 def solve_leg(m, ang, rtol, group_batch=1):
@@ -157,6 +174,22 @@ def solve_leg(m, ang, rtol, group_batch=1):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     choice = list(op.sweep_choice(group_batch))
+    order_choice = {"flux": list(op.sweep_order_choice()), "source_and_flux": list(op.sweep_order_choice(True))}
+    layouts = {}
+    if group_batch == 1:
+        # the same solve with the other layouts of the within-group angular flux (bit-identical moments)
+        for order in FLUX_LAYOUTS:
+            if order == DEFAULT_FLUX_LAYOUT:
+                continue
+            op.set_flux_order(order)
+            phn = torch.zeros_like(qmom)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            op.multigroup_solve_dev(S, phn, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
+            torch.cuda.synchronize()
+            layouts[order] = {"seconds": time.perf_counter() - t1, "moments_bit_identical": bool(torch.equal(phn, phi))}
+            del phn
+        op.set_flux_order(DEFAULT_FLUX_LAYOUT)
     # scalar flux = sqrt(4 pi) x the l = 0 moment; group 1 adds its uncollided flux (Main.f90:443)
     s4pi = math.sqrt(4.0 * math.pi)
     flux = {"collided_sum_all_groups": float(phi[:, 0].sum()) * s4pi,
@@ -184,6 +217,8 @@ def solve_leg(m, ang, rtol, group_batch=1):
     op.close()
     max_res = max(s["true_residual"] for s in stats)
     return {"seconds": wall, "groups": G, "rtol": rtol, "autotune_s": tune_s, "sweep_choice": choice,
+            "sweep_order_choice": order_choice,
+            "flux_layout": DEFAULT_FLUX_LAYOUT if group_batch == 1 else "natural", "other_flux_layouts": layouts,
             "max_true_residual": max_res, "target_residual": 1e-6, "target_met": bool(max_res <= 1e-6),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
             "group_batch": group_batch, "scalar_flux": flux, "reduced_precision_D_psi": reduced,
@@ -499,6 +534,30 @@ def single_group_leg(m, ang, reps=10):
         res[mode] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": m.NE * ang.NI / (ms * 1e-3)}
         if mode == "auto":
             res[mode]["choice"] = list(op.sweep_choice(1))
+    # in place, as the natural-layout solve sweeps; and with the flux stored in sweep
+    # order (out of place), as the flux-moment solve sweeps
+    op.set_direction_block(768)
+    op.set_sweep_mode("direction_lds")
+    for name, run in (("direction_lds-768-in-place", lambda: op.transport_matinv_dev(psi, psi, 0, 1)),
+                      ("direction_lds-768-sweep-order", lambda: op.transport_matinv_order_dev(q, psi, 0)),
+                      ("direction_lds-768-sweep-order-in-place",
+                       lambda: op.transport_matinv_order_dev(psi, psi, 0, source_in_order=True)),
+                      ("direction_lds-512-sweep-order", None), ("direction-512-sweep-order", None),
+                      ("levels-sweep-order", None)):
+        if run is None:
+            mode = name.rsplit("-sweep-order", 1)[0]
+            if "-" in mode:
+                op.set_direction_block(int(mode.split("-")[1]))
+            op.set_sweep_mode(mode.split("-")[0])
+            run = lambda: op.transport_matinv_order_dev(q, psi, 0)   # noqa: E731
+        run()
+        op.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        op.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / reps
+        res[name] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": m.NE * ang.NI / (ms * 1e-3)}
     op.close()
     return res
 

@@ -16,6 +16,7 @@ SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
 SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT, SWEEP_DIRECTION_LDS = 0, 1, 2, 3, 4, 5
+FLUX_NATURAL, FLUX_SWEEP_ORDER, FLUX_SWEEP_ORDER_SOURCE = 0, 1, 2
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)
@@ -109,6 +110,13 @@ SIGNATURES = {
     "lb_transport_matinv_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "lb_moments_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_moments_to_discrete_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "lb_op_set_flux_order": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lb_op_sweep_order_choice": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_int)]),
+    "lb_transport_matinv_order_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "lb_moments_to_discrete_order_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "lb_moments_order_dev": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lb_flux_to_natural_dev": (ctypes.c_int, [_vp, _vp, _vp]),
     "lb_multigroup_solve_dev": (ctypes.c_int, [_vp, _vp, _d, ctypes.POINTER(SolverOpts), _vp,
                                                ctypes.POINTER(SolveStats)]),
     "lb_pack_dev": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),

@@ -710,6 +710,84 @@ int lb_moments_dev(lb_op* op, const double* d_psi, double* d_phi, int G) {
   });
 }
 
+int lb_op_set_flux_order(lb_op* op, int order) {
+  return guarded([&]() -> int {
+    if (!op || order < LB_FLUX_NATURAL || order > LB_FLUX_SWEEP_ORDER_SOURCE) {
+      set_error("lb_op_set_flux_order: order must be LB_FLUX_NATURAL, LB_FLUX_SWEEP_ORDER or "
+                "LB_FLUX_SWEEP_ORDER_SOURCE");
+      return LB_E_ARG;
+    }
+    op->flux_order = order;
+    return LB_OK;
+  });
+}
+
+int lb_op_sweep_order_choice(lb_op* op, int source_in_order, int* mode, int* threads) {
+  return guarded([&]() -> int {
+    if (!op || !mode || !threads) {
+      set_error("lb_op_sweep_order_choice: invalid argument");
+      return LB_E_ARG;
+    }
+    auto it = op->tune_pos.find(source_in_order ? kOrderBoth : kOrderFlux);
+    if (it == op->tune_pos.end() || it->second.choice < 0) {
+      *mode = LB_SWEEP_AUTO;
+      *threads = 0;
+    } else {
+      *mode = kTuneConfig[it->second.choice].mode;
+      *threads = kTuneConfig[it->second.choice].block;
+    }
+    return LB_OK;
+  });
+}
+
+int lb_transport_matinv_order_dev(lb_op* op, const double* d_q, double* d_psi_so, int g0, int source_in_order) {
+  return guarded([&]() -> int {
+    LB_TRY(check_ready(op, "lb_transport_matinv_order_dev"));
+    if (g0 < 0 || g0 >= op->G || !d_q || !d_psi_so) {
+      set_error("lb_transport_matinv_order_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    const int64_t n = op->NI * op->NENK;
+    const bool same = source_in_order && d_q == d_psi_so;
+    if (!same && d_q < d_psi_so + n && d_psi_so < d_q + n) {
+      set_error("lb_transport_matinv_order_dev: the source and the sweep-order flux overlap");
+      return LB_E_ARG;
+    }
+    return launch_sweep_pos(op, d_q, d_psi_so, g0, source_in_order != 0);
+  });
+}
+
+int lb_moments_to_discrete_order_dev(lb_op* op, const double* d_phi, double* d_psi_so, int g0) {
+  return guarded([&]() -> int {
+    LB_TRY(check_ready(op, "lb_moments_to_discrete_order_dev"));
+    if (g0 < 0 || g0 >= op->G || !d_phi || !d_psi_so) {
+      set_error("lb_moments_to_discrete_order_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    return launch_moments_to_order(op, d_phi, d_psi_so, g0);
+  });
+}
+
+int lb_moments_order_dev(lb_op* op, const double* d_psi_so, double* d_phi) {
+  return guarded([&]() -> int {
+    if (!op || !d_psi_so || !d_phi) {
+      set_error("lb_moments_order_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    return launch_moments_pos(op, d_psi_so, d_phi);
+  });
+}
+
+int lb_flux_to_natural_dev(lb_op* op, const double* d_psi_so, double* d_psi) {
+  return guarded([&]() -> int {
+    if (!op || !d_psi_so || !d_psi || d_psi_so == d_psi) {
+      set_error("lb_flux_to_natural_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    return launch_flux_to_natural(op, d_psi_so, d_psi);
+  });
+}
+
 int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, int g0, int G, int accumulate) {
   return guarded([&]() -> int {
     LB_TRY(check_ready(op, "lb_moments_to_discrete_dev"));
@@ -770,6 +848,9 @@ void lb_op_destroy(lb_op* op) {
   for (auto& kv : op->tables) kv.second.d_entries.release();
   if (op->w_host.p) (void)hipHostFree(op->w_host.p);
   op->d_rec.release();
+  op->d_rec_pos.release();
+  op->d_inv.release();
+  op->w_so.release();
   for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec,
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
@@ -786,7 +867,8 @@ void lb_op_destroy(lb_op* op) {
   op->d_lvl_off.release();
   op->d_nlev.release();
   for (auto& kv : op->plans)
-    for (DevBuf* b : {&kv.second.d_uslot, &kv.second.d_wslot, &kv.second.d_stage_off, &kv.second.d_stage})
+    for (DevBuf* b : {&kv.second.d_uslot, &kv.second.d_wslot, &kv.second.d_stage_off, &kv.second.d_stage,
+                      &kv.second.d_stage_pos})
       b->release();
   op->d_xsdd.release();
   op->w_phis.release();

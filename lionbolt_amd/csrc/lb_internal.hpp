@@ -81,7 +81,7 @@ constexpr int kFaceNodes[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 struct SweepGraph {
   const double* q;
   double* psi;
-  int g0, G, mode, block, per_cu;
+  int g0, G, mode, block, per_cu, order;
   hipGraphExec_t exec;
 };
 
@@ -92,6 +92,9 @@ struct SweepGraph {
 struct SweepConfig {
   int mode, block, per_cu;
 };
+// Layout of a one-group sweep's vectors: both natural; the flux in sweep order
+// from a natural source; source and flux in sweep order.
+constexpr int kOrderNatural = 0, kOrderFlux = 1, kOrderBoth = 2;
 // (The LDS sweep at 1024 threads spills registers and is not timed; at 768 it
 // runs three waves per SIMD on one round of look-ahead.)
 constexpr int kTuneConfigs = 7;
@@ -120,6 +123,7 @@ struct TracePlan {
   DevBuf d_wslot;            // int32 [NI][NE]: slot the cell writes, -1 = none
   DevBuf d_stage_off;        // int32 [NI][maxlev + 2]: first staging entry of each consumer level
   DevBuf d_stage;            // int2 entries: (LDS slot, element whose flux is copied)
+  DevBuf d_stage_pos;        // the same entries with the element's sweep position (sweep-order flux)
 };
 
 // Most energy groups solved together by the group-batched multigroup solve.
@@ -189,6 +193,10 @@ struct lb_op {
   int scatter_mode = LB_SCATTER_MOMENT;
   int sweep_mode = LB_SWEEP_AUTO;
   std::map<int, lb::SweepTune> tune;  // autotuner state per group-batch size
+  std::map<int, lb::SweepTune> tune_pos;  // the same for one-group sweeps in sweep order, by lb::kOrder*
+  int flux_order = LB_FLUX_SWEEP_ORDER;   // layout of the within-group solves' angular flux
+  lb::DevBuf d_inv;                 // int32 [NI][NE]: sweep position of each element per direction
+  lb::DevBuf d_rec_pos;             // d_rec with the neighbours' sweep positions in place of their elements
   lb::DevBuf d_rec;                 // per (direction, sweep position): task word, permutation and material,
                                     // neighbours, c_f (k . n_f) of three faces (48 B)
   lb::DevBuf d_lvl_off, d_nlev;     // level offsets per direction, for the direction-resident sweep
@@ -227,6 +235,7 @@ struct lb_op {
   lb::DevBuf w_arn;                 // device-resident Arnoldi scalars (lb::ArnoldiDev)
   lb::DevBuf w_tq;                  // T^-1 q of the current solve, for its true residual
   lb::DevBuf w_tail;                // batched residual checks of the multigroup solve (freed after it)
+  lb::DevBuf w_so;                  // one group's angular flux in sweep order (within-group solves)
 };
 
 namespace lb {
@@ -254,6 +263,21 @@ LevelTable* level_table(lb_op* op, int G, int* rc);
 
 // lb_kernels.hip (launchers; all enqueue on op->stream)
 int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G);
+// One group's sweep with the flux written in sweep order: psi_so[i][pos] holds the
+// flux of the cell at position pos of direction i's device sweep order.  The
+// source q is in the natural layout (then it must not alias psi_so) or, with
+// source_in_order, in sweep order too (in place allowed).
+int launch_sweep_pos(lb_op* op, const double* q, double* psi_so, int g0, bool source_in_order = false);
+// phi = D psi_so for one group's sweep-order flux (bit-identical to launch_moments
+// on the same flux in natural layout).
+int launch_moments_pos(lb_op* op, const double* psi_so, double* phi);
+// One group's M S phi (S = sigma_l(mat, g0) when scaled) written in sweep order.
+int launch_moments_to_order(lb_op* op, const double* phi, double* psi_so, int g0, bool scaled = true);
+// Sweep-order flux of one group to the natural layout.
+int launch_flux_to_natural(lb_op* op, const double* psi_so, double* psi);
+// Whether the within-group solves keep their angular flux in sweep order (the
+// operator's flux order, FP64 moment products of at most 16 moments, no batch).
+bool sweep_order_flux(const lb_op* op);
 // One sweep with the direction-resident kernel at a fixed workgroup size (no autotuning).
 int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int G, int block);
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols);

@@ -50,6 +50,9 @@ struct SweepArgs {
   const double* q;   // may alias psi (in-place sweep)
   double* psi;
   int vec1 = 0;      // one group and a 16-byte-aligned psi: 16-byte flux stores
+  int pos_out = 0;   // flux stored at the cell's sweep position instead of its element (one group); the
+                     // records then carry the neighbours' positions and the staging entries positions
+  int pos_in = 0;    // source read at the cell's sweep position (with pos_out; q may then alias psi)
   // LDS sweep only: the slot plan of this group-batch size (lb_trace_plan.cpp)
   const short4* __restrict__ uslot = nullptr;    // [dir][pos]: LDS slot of each upstream trace, -1 global
   const int16_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
@@ -126,7 +129,7 @@ __device__ __forceinline__ void load_rec(const SweepArgs& A, int i, int s, RecRa
 }
 
 template <bool NT, bool LDS>
-__device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int g, const RecRaw& R, CellPrep& P) {
+__device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int s, int g, const RecRaw& R, CellPrep& P) {
   if constexpr (LDS) {
     P.ws = R.ws;
     P.us = make_int4(R.us.x, R.us.y, R.us.z, R.us.w);
@@ -139,8 +142,8 @@ __device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int g, con
   P.ck[2] = dbl_of(R.c.z, R.c.w);
   P.ck[3] = -((P.ck[0] + P.ck[1]) + P.ck[2]);
   P.sig = A.sigt[(int64_t)(R.a.y >> 24) * A.Gtot + A.g0 + g];
-  const int e = int(P.w & 0xFFFFFFu);
-  const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
+  const int src = A.pos_in ? s : int(P.w & 0xFFFFFFu);   // where the source is
+  const int64_t base = ((int64_t)i * A.NE + src) * 4 * A.G + g;
 #pragma unroll
   for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
 }
@@ -149,7 +152,7 @@ template <bool NT, bool LDS = false>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   RecRaw R;
   load_rec<NT, LDS>(A, i, s, R);
-  decode_rec<NT, LDS>(A, i, g, R, P);
+  decode_rec<NT, LDS>(A, i, s, g, R, P);
 }
 
 // Where a cell reads its upstream traces and what it writes besides the flux.
@@ -204,16 +207,16 @@ __device__ __forceinline__ double pivot_recip(double x) {
 }
 
 template <int SRC>
-__device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, const CellPrep& P, int g,
+__device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                                     const TraceRing& ring, const double (&u)[4][3]) {
   const uint32_t w = P.w;
-  const int e = int(w & 0xFFFFFFu);
+  const int dst = A.pos_out ? s : int(w & 0xFFFFFFu);   // where the flux goes
   const unsigned up = (w >> 24) & 0xFu;
   const unsigned out = w >> 28;
   double b[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) b[a] = P.b[a];
-  const int64_t base = ((int64_t)i * A.NE + e) * 4 * A.G + g;
+  const int64_t base = ((int64_t)i * A.NE + dst) * 4 * A.G + g;
 
   // Upstream faces (TransportAlgebra.f90:82-94): s_e += -(k.n) IIM_f psi_up with
   // IIM_f psi_up = c_f (u_a + U/5) on the face nodes and -4/5 c_f U on the opposite
@@ -331,11 +334,11 @@ __device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, c
 }
 
 template <int SRC>
-__device__ __forceinline__ void finish_cell_body(const SweepArgs& A, int i, const CellPrep& P, int g,
+__device__ __forceinline__ void finish_cell_body(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                                  const TraceRing& ring) {
   double u[4][3];
   load_upstream<SRC>(A, i, P, g, ring, u);
-  solve_with_upstream<SRC>(A, i, P, g, ring, u);
+  solve_with_upstream<SRC>(A, i, s, P, g, ring, u);
 }
 
 __device__ __forceinline__ bool ring_hit(const CellPrep& P) {
@@ -350,15 +353,15 @@ __device__ __forceinline__ bool ring_hit(const CellPrep& P) {
 // the vector-memory counter, which would also wait for the look-ahead loads
 // issued before it.
 template <bool LDS = false>
-__device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, const CellPrep& P, int g,
+__device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                             const TraceRing& ring = TraceRing{nullptr}) {
   if constexpr (LDS) {
     if (ring_hit(P))
-      finish_cell_body<kTraceRing>(A, i, P, g, ring);
+      finish_cell_body<kTraceRing>(A, i, s, P, g, ring);
     else
-      finish_cell_body<kTraceGlobalRing>(A, i, P, g, ring);
+      finish_cell_body<kTraceGlobalRing>(A, i, s, P, g, ring);
   } else {
-    finish_cell_body<kTraceGlobal>(A, i, P, g, ring);
+    finish_cell_body<kTraceGlobal>(A, i, s, P, g, ring);
   }
 }
 
@@ -367,17 +370,20 @@ __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int
                                            const TraceRing& ring = TraceRing{nullptr}) {
   CellPrep P;
   prep_cell<NT, LDS>(A, i, s, g, P);
-  finish_cell<LDS>(A, i, P, g, ring);
+  finish_cell<LDS>(A, i, s, P, g, ring);
 }
 
 // The record of every (direction, sweep position); materials fit 8 bits
-// (LB_MAX_MATERIALS, checked at creation).
+// (LB_MAX_MATERIALS, checked at creation).  With inv (sweep position of each
+// element per direction) the neighbours are given by their positions, for
+// sweeps that keep the flux in sweep order.
 __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t NE, const uint32_t* __restrict__ tasks,
                                                                const double4* __restrict__ geo,
                                                                const int4* __restrict__ nbr,
                                                                const uint32_t* __restrict__ perm,
                                                                const int32_t* __restrict__ mat,
                                                                const double* __restrict__ kdir,
+                                                               const int32_t* __restrict__ inv,
                                                                TaskRec* __restrict__ rec) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= (int64_t)NI * NE) return;
@@ -388,10 +394,10 @@ __global__ __launch_bounds__(kBlock) void build_records_kernel(int NI, int64_t N
   r.w = w;
   r.pm = (perm[e] & 0xFFFFFFu) | ((uint32_t)mat[e] << 24);
   const int4 nb = nbr[e];
-  r.nb[0] = nb.x;
-  r.nb[1] = nb.y;
-  r.nb[2] = nb.z;
-  r.nb[3] = nb.w;
+  const int nbv[4] = {nb.x, nb.y, nb.z, nb.w};
+  const int32_t* ip = inv ? inv + (int64_t)i * NE : nullptr;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) r.nb[f] = (ip && nbv[f] >= 0) ? ip[nbv[f]] : nbv[f];
   const double kx = kdir[3 * i], ky = kdir[3 * i + 1], kz = kdir[3 * i + 2];
 #pragma unroll
   for (int f = 0; f < 3; ++f) r.ck[f] = face_ck(geo[(int64_t)e * 4 + f], kx, ky, kz);
@@ -427,7 +433,7 @@ __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = s0 + c + r * cpr;
-    if (s < s1) finish_cell<LDS>(A, i, P[r], g, ring);
+    if (s < s1) finish_cell<LDS>(A, i, s, P[r], g, ring);
   }
   for (int s = s0 + c + R * cpr; s < s1; s += cpr) solve_cell<NT, LDS>(A, i, s, g, ring);   // beyond R rounds
 }
@@ -558,7 +564,7 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
     if (lane_ok && s < cur.s1 && !(hit >> r & 1u)) load_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) decode_rec<NT, LDS>(A, i, g, raw[r], Pn[r]);
+  for (int r = 0; r < R; ++r) decode_rec<NT, LDS>(A, i, min(nx.s0 + c + r * cpr, nx.s1 - 1), g, raw[r], Pn[r]);
   bool more2 = false;
   const Chunk nn = has_nx ? next_chunk(off, nl, chunk, nx, more2) : nx;
 #pragma unroll
@@ -568,7 +574,7 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = cur.s0 + c + r * cpr;
-    if (lane_ok && s < cur.s1) solve_with_upstream<kSolve>(A, i, P[r], g, ring, u[r]);
+    if (lane_ok && s < cur.s1) solve_with_upstream<kSolve>(A, i, s, P[r], g, ring, u[r]);
   }
   stage_store<LDS>(ring, sc);
   if (last) __syncthreads();
@@ -698,6 +704,57 @@ __global__ void reset_counter_kernel(unsigned int* counter) {
   if (threadIdx.x == 0) counter[0] = 0u;
 }
 
+// inv[i][e] = sweep position of element e in direction i's device order.
+__global__ __launch_bounds__(kBlock) void inverse_order_kernel(int NI, int64_t NE, const uint32_t* __restrict__ tasks,
+                                                               int32_t* __restrict__ inv) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (int64_t)NI * NE) return;
+  const int64_t i = k / NE;
+  inv[i * NE + (tasks[k] & 0xFFFFFFu)] = (int32_t)(k - i * NE);
+}
+
+// Staging entries (LDS slot, element) of the LDS sweep's plan with the element
+// replaced by its sweep position: one workgroup per direction, whose entries
+// are so[0] .. so[stride - 1].
+__global__ __launch_bounds__(kBlock) void stage_positions_kernel(int64_t NE, int64_t stride,
+                                                                 const int32_t* __restrict__ soff,
+                                                                 const int2* __restrict__ stage,
+                                                                 const int32_t* __restrict__ inv,
+                                                                 int2* __restrict__ out) {
+  const int64_t i = blockIdx.x;
+  const int32_t* so = soff + i * stride;
+  for (int64_t j = so[0] + threadIdx.x; j < so[stride - 1]; j += kBlock) {
+    const int2 en = stage[j];
+    out[j] = make_int2(en.x, inv[i * NE + en.y]);
+  }
+}
+
+// One group's flux from sweep order to the natural layout: lane per (direction,
+// element, half cell), 16 bytes each.
+__global__ __launch_bounds__(kBlock) void flux_to_natural_kernel(int NI, int64_t NE, const int32_t* __restrict__ inv,
+                                                                 const double* __restrict__ psi_so,
+                                                                 double* __restrict__ psi) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (int64_t)NI * NE * 2) return;
+  const int64_t ie = k >> 1, i = ie / NE;
+  const int h = int(k & 1) * 2;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  *reinterpret_cast<d2*>(psi + ie * 4 + h) =
+      *reinterpret_cast<const d2*>(psi_so + (i * NE + inv[ie]) * 4 + h);
+}
+
+// Workgroup index with consecutive blocks on one XCD: the dispatcher deals
+// workgroups to the 8 XCDs in turn, so block b runs on XCD b mod 8; this maps
+// the workgroups of each XCD to one contiguous range of blocks (the last
+// gridDim.x mod 8 blocks keep their index).  Kernels that gather or scatter by
+// sweep position use it so that the cells of neighbouring blocks, whose
+// positions share cache lines, meet in the same L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned b = blockIdx.x, per = gridDim.x >> 3;
+  if (b >= per << 3) return b;
+  return (int64_t)(b & 7u) * per + (b >> 3);
+}
+
 // phi[lm][col] = sum_i D[lm][i] psi[i][col]; one lane per column (column =
 // spatial dof x group), D read through the scalar cache.
 template <int NM>
@@ -795,6 +852,66 @@ __global__ __launch_bounds__(kBlock) void moments_pair_kernel(const double* __re
   for (int m = 0; m < NM; ++m) *reinterpret_cast<dbl2*>(phi + (int64_t)m * ncols + c0) = dbl2{a0[m], a1[m]};
 }
 
+// D psi for one group whose flux is in sweep order: the lane of columns c, c + 1
+// (nodes k, k + 1 of cell e) reads them in direction i at e's sweep position
+// inv[i][e].  The positions of kGatherRows directions are loaded one batch
+// ahead of the flux they address.  Same sums in the same order as
+// moments_pair_kernel, so the moments are bit-identical.
+constexpr int kGatherRows = 8;
+
+template <int NM>
+__global__ __launch_bounds__(kBlock) void moments_gather_pair_kernel(const double* __restrict__ D, int NI, int64_t NE,
+                                                                     const int32_t* __restrict__ inv,
+                                                                     const double* __restrict__ psi_so,
+                                                                     double* __restrict__ phi) {
+  const int64_t ncols = NE * 4;
+  const int64_t c0 = (xcd_block() * kBlock + threadIdx.x) * 2;
+  if (c0 >= ncols) return;
+  const int64_t e = c0 >> 2;
+  const int h = int(c0 & 3);
+  double a0[NM], a1[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) a0[m] = a1[m] = 0.0;
+  const int32_t* ip = inv + e;
+  const double* ps = psi_so + h;
+  int pn[kGatherRows];
+  const int nfull = NI - NI % kGatherRows;
+  if (nfull > 0) {
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r) pn[r] = __builtin_nontemporal_load(ip + (int64_t)r * NE);
+  }
+  for (int i0 = 0; i0 < nfull; i0 += kGatherRows) {
+    dbl2 v[kGatherRows];
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r)
+      v[r] = *reinterpret_cast<const dbl2*>(ps + ((int64_t)(i0 + r) * NE + pn[r]) * 4);
+    if (i0 + kGatherRows < nfull) {
+#pragma unroll
+      for (int r = 0; r < kGatherRows; ++r)
+        pn[r] = __builtin_nontemporal_load(ip + (int64_t)(i0 + kGatherRows + r) * NE);
+    }
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r)
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const double d = D[m * NI + i0 + r];
+        a0[m] = fma(d, v[r].x, a0[m]);
+        a1[m] = fma(d, v[r].y, a1[m]);
+      }
+  }
+  for (int i0 = nfull; i0 < NI; ++i0) {
+    const dbl2 v = *reinterpret_cast<const dbl2*>(ps + ((int64_t)i0 * NE + ip[(int64_t)i0 * NE]) * 4);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const double d = D[m * NI + i0];
+      a0[m] = fma(d, v.x, a0[m]);
+      a1[m] = fma(d, v.y, a1[m]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) *reinterpret_cast<dbl2*>(phi + (int64_t)m * ncols + c0) = dbl2{a0[m], a1[m]};
+}
+
 template <int NM>
 __global__ __launch_bounds__(kBlock) void moments_to_discrete_pair_kernel(
     const double* __restrict__ M, const int32_t* __restrict__ deg, int NI, int64_t ncols, int G, int Gtot, int g0,
@@ -837,6 +954,50 @@ __global__ __launch_bounds__(kBlock) void moments_to_discrete_pair_kernel(
       x1 = old.y + x1;
     }
     __builtin_nontemporal_store(dbl2{x0, x1}, dst);
+  }
+}
+
+// M S phi for one group written in sweep order: psi_so[i][inv[i][e]] gets cell
+// e's values of direction i (the same sums as moments_to_discrete_pair_kernel,
+// so bit-identical values, at other addresses).  Neighbouring cells of one
+// level sit at consecutive positions; with consecutive blocks on one XCD their
+// 32-byte pieces meet in its L2 before they are written back.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void moments_to_order_pair_kernel(
+    const double* __restrict__ M, const int32_t* __restrict__ deg, int NI, int64_t NE, int Gtot, int g0, int L,
+    const int32_t* __restrict__ mat, const double* __restrict__ sigs, const int32_t* __restrict__ inv,
+    const double* __restrict__ phi, double* __restrict__ psi_so) {
+  const int64_t ncols = NE * 4;
+  const int64_t c0 = (xcd_block() * kBlock + threadIdx.x) * 2;
+  if (c0 >= ncols) return;
+  const int64_t e = c0 >> 2;
+  const int h = int(c0 & 3);
+  double s0[NM], s1[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    const dbl2 v = *reinterpret_cast<const dbl2*>(phi + (int64_t)m * ncols + c0);
+    s0[m] = v.x;
+    s1[m] = v.y;
+  }
+  if (sigs) {
+    const double* la = sigs + ((int64_t)mat[e] * Gtot + g0) * (L + 1);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      s0[m] *= la[deg[m]];
+      s1[m] *= la[deg[m]];
+    }
+  }
+  const int32_t* ip = inv + e;
+  for (int i = 0; i < NI; ++i) {
+    double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const double mm = M[i * NM + m];
+      x0 = fma(mm, s0[m], x0);
+      x1 = fma(mm, s1[m], x1);
+    }
+    const int p = __builtin_nontemporal_load(ip + (int64_t)i * NE);
+    *reinterpret_cast<dbl2*>(psi_so + ((int64_t)i * NE + p) * 4 + h) = dbl2{x0, x1};
   }
 }
 
@@ -1411,15 +1572,39 @@ static void launch_persistent(int block, bool nt, dim3 grid, hipStream_t st, con
     launch_persistent_nt<false>(block, grid, st, items, nitems, lvl, act, ctr, A);
 }
 
-// The records of every (direction, sweep position) for a task order.
-static int ensure_records(lb_op* op, DevBuf& rec, const DevBuf& tasks) {
+// The records of every (direction, sweep position) for a task order; with inv,
+// neighbours by sweep position.
+static int ensure_records(lb_op* op, DevBuf& rec, const DevBuf& tasks, const int32_t* inv = nullptr) {
   if (rec.p) return LB_OK;
   const int64_t nrec = op->NI * op->NE;
   LB_TRY(rec.reserve(nrec * sizeof(TaskRec)));
   hipLaunchKernelGGL(build_records_kernel, dim3(grid_for(nrec)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
                      tasks.as<const uint32_t>(), op->d_geo.as<const double4>(), op->d_nbr.as<const int4>(),
-                     op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(),
+                     op->d_perm.as<const uint32_t>(), op->d_mat.as<const int32_t>(), op->d_k.as<const double>(), inv,
                      rec.as<TaskRec>());
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+// The sweep position of every element per direction (built on first use).
+static int ensure_inverse(lb_op* op) {
+  if (op->d_inv.p) return LB_OK;
+  const int64_t n = op->NI * op->NE;
+  LB_TRY(op->d_inv.reserve(n * sizeof(int32_t)));
+  hipLaunchKernelGGL(inverse_order_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                     op->d_tasks.as<const uint32_t>(), op->d_inv.as<int32_t>());
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+// The staging entries of a slot plan by sweep position (built on first use).
+static int ensure_stage_positions(lb_op* op, TracePlan* plan) {
+  if (plan->d_stage_pos.p) return LB_OK;
+  LB_TRY(ensure_inverse(op));
+  LB_TRY(plan->d_stage_pos.reserve(plan->d_stage.bytes));
+  hipLaunchKernelGGL(stage_positions_kernel, dim3((unsigned)op->NI), dim3(kBlock), 0, op->stream, op->NE,
+                     (int64_t)op->maxlev + 2, plan->d_stage_off.as<const int32_t>(), plan->d_stage.as<const int2>(),
+                     op->d_inv.as<const int32_t>(), plan->d_stage_pos.as<int2>());
   LB_HIP(hipGetLastError());
   return LB_OK;
 }
@@ -1428,11 +1613,12 @@ static int ensure_records(lb_op* op, DevBuf& rec, const DevBuf& tasks) {
 // configuration), capturing enqueue() first if there is none.  The cache holds 8
 // graphs and is cleared whenever the physics table or the problem set moves.
 template <class F>
-static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg, F enqueue) {
+static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg, int order,
+                        F enqueue) {
   hipGraphExec_t exec = nullptr;
   for (auto& sg : op->graphs)
     if (sg.q == q && sg.psi == psi && sg.g0 == g0 && sg.G == G && sg.mode == cfg.mode && sg.block == cfg.block &&
-        sg.per_cu == cfg.per_cu)
+        sg.per_cu == cfg.per_cu && sg.order == order)
       exec = sg.exec;
   if (!exec) {
     LB_HIP(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
@@ -1456,7 +1642,7 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
       (void)hipGraphExecDestroy(op->graphs.front().exec);
       op->graphs.erase(op->graphs.begin());
     }
-    op->graphs.push_back({q, psi, g0, G, cfg.mode, cfg.block, cfg.per_cu, exec});
+    op->graphs.push_back({q, psi, g0, G, cfg.mode, cfg.block, cfg.per_cu, order, exec});
   }
   LB_HIP(hipEventRecord(op->ev0, op->stream));
   LB_HIP(hipGraphLaunch(exec, op->stream));
@@ -1464,9 +1650,19 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
   return LB_OK;
 }
 
-// One sweep issued in a fixed launch configuration.
-static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg) {
+// One sweep issued in a fixed launch configuration.  order: kOrderNatural, or
+// (one group of a single problem) kOrderFlux, the flux stored in sweep order
+// from a natural source (out of place), or kOrderBoth, source and flux in
+// sweep order (in place allowed).
+static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg,
+                           int order = kOrderNatural) {
   const int mode = cfg.mode, block = cfg.block;
+  const bool pos_out = order != kOrderNatural;
+  if (pos_out && (G != 1 || mode == LB_SWEEP_PERSISTENT || op->batch.P > 0 || (order == kOrderFlux && q == psi))) {
+    set_error("sweep: a sweep-order flux is one group of a single problem, out of place unless the source is in "
+              "sweep order too");
+    return LB_E_ARG;
+  }
   if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
       !((block == 256 || block == 512 || block == 1024 || (block == 768 && mode != LB_SWEEP_PERSISTENT)) &&
         block / G >= 1)) {
@@ -1495,16 +1691,23 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     const int32_t* act = B.d_active.as<const int32_t>();
     unsigned int* ctr = B.d_counter.as<unsigned int>();
     const dim3 grid((unsigned)(B.ncu * cfg.per_cu));
-    return replay_graph(op, q, psi, g0, G, cfg, [&]() {
+    return replay_graph(op, q, psi, g0, G, cfg, 0, [&]() {
       hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, op->stream, ctr);
       launch_persistent(block, nt, grid, op->stream, items, nitems, lvl, act, ctr, A);
     });
   }
 
-  LB_TRY(ensure_records(op, op->d_rec, op->d_tasks));
-  SweepArgs A{op->d_rec.as<const TaskRec>(), op->d_sigt.as<const double>(),
+  if (pos_out) {
+    LB_TRY(ensure_inverse(op));
+    LB_TRY(ensure_records(op, op->d_rec_pos, op->d_tasks, op->d_inv.as<const int32_t>()));
+  } else {
+    LB_TRY(ensure_records(op, op->d_rec, op->d_tasks));
+  }
+  SweepArgs A{(pos_out ? op->d_rec_pos : op->d_rec).as<const TaskRec>(), op->d_sigt.as<const double>(),
               op->G, g0, G, op->NE, q, psi};
   A.vec1 = one_group_vectors(G, psi);
+  A.pos_out = pos_out ? 1 : 0;
+  A.pos_in = order == kOrderBoth ? 1 : 0;
   const bool nt = G >= kStreamGroups;
   if (mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS) {
     if (!op->d_lvl_off.p) {
@@ -1518,13 +1721,14 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     const int32_t* nl = op->d_nlev.as<const int32_t>();
     if (mode == LB_SWEEP_DIRECTION_LDS) {
       int rc = LB_OK;
-      const TracePlan* plan = trace_plan(op, G, &rc);
+      TracePlan* plan = trace_plan(op, G, &rc);
       if (!plan) return rc;
+      if (pos_out) LB_TRY(ensure_stage_positions(op, plan));
       const size_t bytes = (size_t)(plan->ring + 2 * plan->stage) * 4 * G * sizeof(double);
       A.uslot = plan->d_uslot.as<const short4>();
       A.wslot = plan->d_wslot.as<const int16_t>();
       A.stage_off = plan->d_stage_off.as<const int32_t>();
-      A.stage = plan->d_stage.as<const int2>();
+      A.stage = (pos_out ? plan->d_stage_pos : plan->d_stage).as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
       if (nt)
         LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
@@ -1555,7 +1759,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
         hipLaunchKernelGGL(sweep_level_kernel<false>, dim3(nb), dim3(kBlock), 0, op->stream, work + t->level_off[l], A);
     }
   };
-  if (mode == LB_SWEEP_GRAPH) return replay_graph(op, q, psi, g0, G, cfg, enqueue);
+  if (mode == LB_SWEEP_GRAPH) return replay_graph(op, q, psi, g0, G, cfg, order, enqueue);
   LB_HIP(hipEventRecord(op->ev0, op->stream));
   enqueue();
   LB_HIP(hipGetLastError());
@@ -1578,18 +1782,17 @@ int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int 
   return launch_sweep_as(op, q, psi, g0, G, SweepConfig{LB_SWEEP_DIRECTION, block, 0});
 }
 
-int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
-  if (G < 1 || G > kBlock) {
-    set_error("sweep: the group batch must be between 1 and 256");
-    return LB_E_ARG;
-  }
+// A sweep in the operator's mode; in LB_SWEEP_AUTO the configuration is timed
+// on the first sweeps of each group-batch size (separately for sweep-order
+// output) and then fixed.
+static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, int G, int order) {
   if (op->sweep_mode != LB_SWEEP_AUTO)
-    return launch_sweep_as(op, q, psi, g0, G, SweepConfig{op->sweep_mode, op->dir_block, op->per_cu});
+    return launch_sweep_as(op, q, psi, g0, G, SweepConfig{op->sweep_mode, op->dir_block, op->per_cu}, order);
   const bool batched = op->batch.P > 0;
   const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
   const int nconf = batched ? kBatchTuneConfigs : kTuneConfigs;
-  SweepTune& tn = batched ? op->btune[G] : op->tune[G];
-  if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice]);
+  SweepTune& tn = batched ? op->btune[G] : (order != kOrderNatural ? op->tune_pos[order] : op->tune[G]);
+  if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice], order);
   // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
   // sweep timed alone between the operator's events; configurations whose block
   // cannot hold the group batch are skipped, and so is the LDS sweep with many
@@ -1604,7 +1807,7 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
     ++tn.trials;
     c = tn.trials % nconf;
   }
-  LB_TRY(launch_sweep_as(op, q, psi, g0, G, table[c]));
+  LB_TRY(launch_sweep_as(op, q, psi, g0, G, table[c], order));
   LB_HIP(hipEventSynchronize(op->ev1));
   float ms = 0.f;
   LB_HIP(hipEventElapsedTime(&ms, op->ev0, op->ev1));
@@ -1615,6 +1818,32 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
       if (tn.best_ms[i] < tn.best_ms[best]) best = i;
     tn.choice = best;
   }
+  return LB_OK;
+}
+
+int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
+  if (G < 1 || G > kBlock) {
+    set_error("sweep: the group batch must be between 1 and 256");
+    return LB_E_ARG;
+  }
+  return launch_sweep_tuned(op, q, psi, g0, G, kOrderNatural);
+}
+
+int launch_sweep_pos(lb_op* op, const double* q, double* psi_so, int g0, bool source_in_order) {
+  return launch_sweep_tuned(op, q, psi_so, g0, 1, source_in_order ? kOrderBoth : kOrderFlux);
+}
+
+bool sweep_order_flux(const lb_op* op) {
+  return op->flux_order != LB_FLUX_NATURAL && op->nmom <= 16 && op->moment_fmt == LB_FMT_FP64 &&
+         op->batch.P == 0;
+}
+
+int launch_flux_to_natural(lb_op* op, const double* psi_so, double* psi) {
+  LB_TRY(ensure_inverse(op));
+  const int64_t n = op->NI * op->NE * 2;
+  hipLaunchKernelGGL(flux_to_natural_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
+                     op->d_inv.as<const int32_t>(), psi_so, psi);
+  LB_HIP(hipGetLastError());
   return LB_OK;
 }
 
@@ -1738,6 +1967,38 @@ int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t nco
   hipLaunchKernelGGL(moments_kernel<NMV>, dim3(grid_for(ncols)), dim3(kBlock), 0, op->stream, op->d_D.as<double>(), \
                      (int)op->NI, ncols, psi, phi)
   LB_NM_DISPATCH(op->nmom, CALL)
+#undef CALL
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_moments_pos(lb_op* op, const double* psi_so, double* phi) {
+  if (op->nmom > 16 || !pair_ok(op->NENK, psi_so, phi)) {
+    set_error("moments: a sweep-order flux takes at most 16 moments and 16-byte aligned vectors");
+    return LB_E_ARG;
+  }
+  LB_TRY(ensure_inverse(op));
+#define CALL(NMV)                                                                                                   \
+  hipLaunchKernelGGL(moments_gather_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,     \
+                     op->d_D.as<const double>(), (int)op->NI, op->NE, op->d_inv.as<const int32_t>(), psi_so, phi)
+  LB_NM_DISPATCH16(op->nmom, CALL)
+#undef CALL
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_moments_to_order(lb_op* op, const double* phi, double* psi_so, int g0, bool scaled) {
+  if (op->nmom > 16 || !pair_ok(op->NENK, phi, psi_so)) {
+    set_error("moments: a sweep-order flux takes at most 16 moments and 16-byte aligned vectors");
+    return LB_E_ARG;
+  }
+  LB_TRY(ensure_inverse(op));
+  const double* sigs = scaled ? op->d_sigs.as<const double>() : nullptr;
+#define CALL(NMV)                                                                                                   \
+  hipLaunchKernelGGL(moments_to_order_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,   \
+                     op->d_M.as<const double>(), op->d_deg.as<const int32_t>(), (int)op->NI, op->NE, op->G, g0,      \
+                     op->L, op->d_mat.as<const int32_t>(), sigs, op->d_inv.as<const int32_t>(), phi, psi_so)
+  LB_NM_DISPATCH16(op->nmom, CALL)
 #undef CALL
   LB_HIP(hipGetLastError());
   return LB_OK;

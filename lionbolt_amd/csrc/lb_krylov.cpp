@@ -582,9 +582,24 @@ double group_thickness(const lb_op* op, int g) {
 }
 
 // out <- D T^-1 M S in: the scattering-sweep-moments operator K of group g0 on
-// flux moments, out of place.
+// flux moments, out of place.  With a sweep-order flux the sweep writes its
+// flux to w_so and D gathers it back by element; otherwise the sweep runs in
+// place on M S in.
 int moment_k_dev(lb_op* op, const double* in, double* out, int g0) {
   const int64_t n = op->NENK * op->NI;
+  if (sweep_order_flux(op)) {
+    LB_TRY(op->w_so.reserve(n * sizeof(double)));
+    double* so = op->w_so.as<double>();
+    if (op->flux_order == LB_FLUX_SWEEP_ORDER_SOURCE) {
+      LB_TRY(launch_moments_to_order(op, in, so, g0));
+      LB_TRY(launch_sweep_pos(op, so, so, g0, true));
+    } else {
+      LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+      LB_TRY(launch_moments_to_discrete(op, in, op->w_a2.as<double>(), g0, 1, 0));
+      LB_TRY(launch_sweep_pos(op, op->w_a2.as<const double>(), so, g0));
+    }
+    return launch_moments_pos(op, so, out);
+  }
   LB_TRY(op->w_a2.reserve(n * sizeof(double)));
   double* psi_t = op->w_a2.as<double>();
   LB_TRY(launch_moments_to_discrete(op, in, psi_t, g0, 1, 0));
@@ -605,6 +620,11 @@ int moment_matvec_dev(lb_op* op, double* v, int g0) {
 // b = D T^-1 q for a discrete source q of group g0.
 int moment_rhs_dev(lb_op* op, const double* q, double* b, int g0) {
   const int64_t n = op->NENK * op->NI;
+  if (sweep_order_flux(op)) {
+    LB_TRY(op->w_so.reserve(n * sizeof(double)));
+    LB_TRY(launch_sweep_pos(op, q, op->w_so.as<double>(), g0));
+    return launch_moments_pos(op, op->w_so.as<const double>(), b);
+  }
   LB_TRY(op->w_c.reserve(n * sizeof(double)));
   double* t = op->w_c.as<double>();
   LB_TRY(launch_copy(op->stream, n, q, t));
@@ -1004,6 +1024,8 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
   LB_TRY(op->w_a2.reserve(n * sizeof(double)));
   // V (m+1), rhs, and for the flexible solver Z (m) and the carried residual R
   LB_TRY(op->w_krylov.reserve((size_t)(flexible ? 2 * m + 3 : m + 2) * nphi * sizeof(double)));
+  const bool so = sweep_order_flux(op);
+  if (so) LB_TRY(op->w_so.reserve(n * sizeof(double)));
   double* q = op->w_mg.as<double>();
   double* acc = op->w_acc.as<double>();
   double* V = op->w_krylov.as<double>();
@@ -1016,12 +1038,23 @@ static int multigroup_moment_gmres(lb_op* op, const double* qmom, const double* 
   for (int g = 0; g < G; ++g) {
     auto t0 = std::chrono::steady_clock::now();
     LB_TRY(launch_downscatter(op, phi_out, S, g, G, qmom ? qmom + (int64_t)g * nphi : nullptr, acc));
-    LB_TRY(launch_moments_to_discrete(op, acc, q, g, 1, 0, false));
-    LB_TRY(sweep_inplace(op, q, g));          // T^-1 q_g
+    double* tq = q;                            // T^-1 q_g, in place or in sweep order
+    if (so && op->flux_order == LB_FLUX_SWEEP_ORDER_SOURCE) {
+      tq = op->w_so.as<double>();
+      LB_TRY(launch_moments_to_order(op, acc, tq, g, false));
+      LB_TRY(launch_sweep_pos(op, tq, tq, g, true));
+    } else if (so) {
+      tq = op->w_so.as<double>();
+      LB_TRY(launch_moments_to_discrete(op, acc, q, g, 1, 0, false));
+      LB_TRY(launch_sweep_pos(op, q, tq, g));
+    } else {
+      LB_TRY(launch_moments_to_discrete(op, acc, q, g, 1, 0, false));
+      LB_TRY(sweep_inplace(op, q, g));
+    }
     double tq2;
-    LB_TRY(dot(op, n, q, q, &tq2));
+    LB_TRY(dot(op, n, tq, tq, &tq2));
     tq_norm[g] = std::sqrt(tq2);
-    LB_TRY(launch_moments(op, q, rhs, 1, op->NENK));
+    LB_TRY(so ? launch_moments_pos(op, tq, rhs) : launch_moments(op, tq, rhs, 1, op->NENK));
     int sweeps = 1;
     Apply A = [&](double* v) -> int {
       ++sweeps;

@@ -139,6 +139,26 @@ class Operator:
         name = {v: k for k, v in self._SWEEP_MODES.items()}[mode.value]
         return name, threads.value, per_cu.value
 
+    def sweep_order_choice(self, source_in_order=False):
+        """(mode name, workgroup size) the autotuner settled on for one-group sweeps that
+        write a sweep-order flux (from a natural or a sweep-order source), or ("auto", 0)
+        while it is still timing."""
+        mode, threads = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().lb_op_sweep_order_choice(self.h, int(bool(source_in_order)), ctypes.byref(mode),
+                                                       ctypes.byref(threads)))
+        name = {v: k for k, v in self._SWEEP_MODES.items()}[mode.value]
+        return name, threads.value
+
+    def set_flux_order(self, order):
+        """Layout of one group's angular flux inside the within-group flux-moment solves:
+        "natural", "sweep" (each direction's flux stored in its sweep order) or
+        "sweep_source" (the sweep's source too).  All give bit-identical solutions."""
+        codes = {"natural": _lib.FLUX_NATURAL, "sweep": _lib.FLUX_SWEEP_ORDER,
+                 "sweep_source": _lib.FLUX_SWEEP_ORDER_SOURCE}
+        if order not in codes:
+            raise ValueError(f"flux order must be one of {sorted(codes)}")
+        _lib.check(_lib.lib().lb_op_set_flux_order(self.h, codes[order]))
+
     def set_group_batch(self, groups):
         """Energy groups solved together per coupled system in ``multigroup_solve_dev``."""
         _lib.check(_lib.lib().lb_op_set_group_batch(self.h, int(groups)))
@@ -228,6 +248,39 @@ class Operator:
         q_p, psi_p = _dev(q, need, "q"), _dev(psi, need, "psi")
         _torch_done()
         _lib.check(_lib.lib().lb_transport_matinv_dev(self.h, q_p, psi_p, g0, G))
+
+    def transport_matinv_order_dev(self, q, psi_so, g0=0, source_in_order=False):
+        """One group's psi = T^-1 q with psi_so in sweep order: psi_so[(i NE + pos) 4 + k]
+        is node k of the cell at position pos of direction i's device sweep order.
+        q is in the natural layout (then it must not overlap psi_so) or, with
+        ``source_in_order``, in sweep order too (then it may be psi_so itself).
+        Asynchronous, as ``transport_matinv_dev``."""
+        q_p, psi_p = _dev(q, self.n, "q"), _dev(psi_so, self.n, "psi_so")
+        _torch_done()
+        _lib.check(_lib.lib().lb_transport_matinv_order_dev(self.h, q_p, psi_p, int(g0), int(bool(source_in_order))))
+
+    def moments_to_discrete_order_dev(self, phi, psi_so, g0=0):
+        """One group's M S_g0 phi written in sweep order."""
+        phi_p = _dev(phi, self.nmom * self.NENK, "phi")
+        psi_p = _dev(psi_so, self.n, "psi_so")
+        _torch_done()
+        _lib.check(_lib.lib().lb_moments_to_discrete_order_dev(self.h, phi_p, psi_p, int(g0)))
+        self.synchronize()
+
+    def moments_order_dev(self, psi_so, phi):
+        """phi = D psi for one group's sweep-order flux."""
+        psi_p = _dev(psi_so, self.n, "psi_so")
+        phi_p = _dev(phi, self.nmom * self.NENK, "phi")
+        _torch_done()
+        _lib.check(_lib.lib().lb_moments_order_dev(self.h, psi_p, phi_p))
+        self.synchronize()
+
+    def flux_to_natural_dev(self, psi_so, psi):
+        """One group's sweep-order flux in the natural layout [i][e][k]."""
+        so_p, psi_p = _dev(psi_so, self.n, "psi_so"), _dev(psi, self.n, "psi")
+        _torch_done()
+        _lib.check(_lib.lib().lb_flux_to_natural_dev(self.h, so_p, psi_p))
+        self.synchronize()
 
     def moments_dev(self, psi, phi, G=1):
         psi_p = _dev(psi, self.n * int(G), "psi")

@@ -11,7 +11,8 @@ Through properties that hold at any size:
 * every launch mode gives the same flux bit for bit: level launches, the
   direction-resident kernel and the LDS kernel (its slot plan at this size
   reads 98.4 % of the upstream traces from ring slots, 0.7 % from staging
-  copies and 1 % from global memory, so every trace path runs);
+  copies and 1 % from global memory, so every trace path runs), in the natural
+  layout and with the flux stored in sweep order (and D psi gathered from it);
 * three- and 24-group sweeps equal single-group sweeps bit for bit (24 groups
   is the benchmark's sweep, streaming loads and graph-replayed level launches);
 * a two-group downscatter solve with the beam source reaches a true relative
@@ -128,6 +129,25 @@ def test_full_size_sweep_linearity_and_modes(full):
     assert torch.equal(out["direction_lds"], out["levels"])
     a = out["levels"]
     del out
+    # the sweep-order flux of the within-group solves: same values at each cell's sweep position,
+    # the LDS plan's staged and global trace reads addressing neighbours by position
+    so = torch.empty_like(q1)
+    nat = torch.empty_like(q1)
+    for mode, block in (("levels", 0), ("direction", 512), ("direction_lds", 768)):
+        op.set_sweep_mode(mode)
+        if block:
+            op.set_direction_block(block)
+        so.fill_(float("nan"))
+        op.transport_matinv_order_dev(q1, so, 0)
+        op.flux_to_natural_dev(so, nat)
+        assert torch.equal(nat, a), mode
+        print(f"full size: {mode} sweep-order sweep done", flush=True)
+    phi_a = torch.empty(op.nmom * op.NENK, dtype=torch.float64, device="cuda")
+    phi_b = torch.empty_like(phi_a)
+    op.moments_dev(a, phi_a)
+    op.moments_order_dev(so, phi_b)
+    assert torch.equal(phi_a, phi_b)
+    del so, nat, phi_a, phi_b
     op.set_sweep_mode("direction_lds")
     b = torch.empty_like(q1)
     op.transport_matinv_dev(q2, b, 0, 1)
