@@ -333,6 +333,21 @@ def batch_leg(reps=10, rtol=1e-6):
         op.set_sweep_mode(mode)
         ms = timed(reps)
         modes[name] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3)}
+    # the flux in the batch's sweep order, as the batched solve sweeps (autotuned persistent launch)
+    op.set_sweep_mode("auto")
+    for _ in range(32):
+        if op.sweep_order_choice()[0] != "auto":
+            break
+        op.transport_matinv_order_dev(q, psi, 0)
+    op.transport_matinv_order_dev(q, psi, 0)
+    op.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.transport_matinv_order_dev(q, psi, 0)
+    op.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    modes["auto-sweep-order"] = {"ms_per_sweep": ms, "cell_angle_sweeps_per_s": ca / (ms * 1e-3),
+                                 "choice": list(op.sweep_order_choice())}
     res["sweep"] = modes
     # the same problems swept one operator at a time (each with its own autotuned launch)
     singles = []
@@ -369,7 +384,17 @@ def batch_leg(reps=10, rtol=1e-6):
                     "iterates_max": max(s["iterates"] for s in st),
                     "iterates_mean": float(np.mean([s["iterates"] for s in st])),
                     "sweeps": st[0]["sweeps"], "max_true_residual": max(s["true_residual"] for s in st),
+                    "flux_layout": DEFAULT_FLUX_LAYOUT,
                     "method": "per-problem flux-moment GMRES(30) in lockstep, one sweep launch per step, FP64"}
+    # the same solve with the angular flux in the natural layout (bit-identical moments)
+    op.set_flux_order("natural")
+    phn = torch.zeros_like(qm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.batch_solve_dev(qm, phn, m=30, rtol=rtol)
+    torch.cuda.synchronize()
+    res["solve"]["natural_flux_layout"] = {"seconds": time.perf_counter() - t0,
+                                           "moments_bit_identical": bool(torch.equal(phn, phi))}
     op.close()
     return res
 

@@ -728,13 +728,16 @@ int lb_op_sweep_order_choice(lb_op* op, int source_in_order, int* mode, int* thr
       set_error("lb_op_sweep_order_choice: invalid argument");
       return LB_E_ARG;
     }
-    auto it = op->tune_pos.find(source_in_order ? kOrderBoth : kOrderFlux);
-    if (it == op->tune_pos.end() || it->second.choice < 0) {
+    const bool batched = op->batch.P > 0;
+    auto& tune = batched ? op->btune_pos : op->tune_pos;
+    const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
+    auto it = tune.find(source_in_order ? kOrderBoth : kOrderFlux);
+    if (it == tune.end() || it->second.choice < 0) {
       *mode = LB_SWEEP_AUTO;
       *threads = 0;
     } else {
-      *mode = kTuneConfig[it->second.choice].mode;
-      *threads = kTuneConfig[it->second.choice].block;
+      *mode = table[it->second.choice].mode;
+      *threads = table[it->second.choice].block;
     }
     return LB_OK;
   });
@@ -852,6 +855,7 @@ void lb_op_destroy(lb_op* op) {
   op->d_inv.release();
   op->w_so.release();
   for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec,
+                    &op->batch.d_inv, &op->batch.d_rec_pos,
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
                     &op->batch.d_out})
     b->release();

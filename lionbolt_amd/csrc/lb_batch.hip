@@ -122,6 +122,9 @@ int set_problems(lb_op* op, int P, const int64_t* eoff) {
   clear_graphs(op);
   BatchTable& B = op->batch;
   B.d_rec.release();
+  B.d_rec_pos.release();
+  B.d_inv.release();
+  op->btune_pos.clear();
   for (auto& kv : B.seg) {
     kv.second.d_tiles.release();
     kv.second.d_tile_off.release();
@@ -302,18 +305,35 @@ int batch_solve_dev(lb_op* op, const double* qmom, const lb_solver_opts* o, doub
   std::vector<char> all(P, 1);
   LB_TRY(set_active(op, all));
 
-  // rhs = D T^-1 q
+  // rhs = D T^-1 q; with a sweep-order flux the persistent sweep writes it to w_so
+  // in the batch's problem-major order and D gathers it back by element
+  const bool so = sweep_order_flux(op);
+  double* psi_so = nullptr;
+  if (so) {
+    LB_TRY(op->w_so.reserve(n * sizeof(double)));
+    psi_so = op->w_so.as<double>();
+  }
   LB_TRY(launch_moments_to_discrete(op, qmom, q, 0, 1, 0, false));
-  LB_TRY(launch_copy(str, n, q, psi_t));
-  LB_TRY(launch_sweep(op, psi_t, psi_t, 0, 1));
+  if (so) {
+    LB_TRY(launch_sweep_pos(op, q, psi_so, 0));
+    LB_TRY(launch_moments_pos(op, psi_so, rhs));
+  } else {
+    LB_TRY(launch_copy(str, n, q, psi_t));
+    LB_TRY(launch_sweep(op, psi_t, psi_t, 0, 1));
+    LB_TRY(launch_moments(op, psi_t, rhs, 1, op->NENK));
+  }
   ++sweeps;
-  LB_TRY(launch_moments(op, psi_t, rhs, 1, op->NENK));
   // A v = v - D T^-1 M S v for every problem; problems not in `act` skip the sweep
   auto apply = [&](double* v) -> int {
     ++sweeps;
     LB_TRY(launch_moments_to_discrete(op, v, psi_t, 0, 1, 0));
-    LB_TRY(launch_sweep(op, psi_t, psi_t, 0, 1));
-    LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+    if (so) {
+      LB_TRY(launch_sweep_pos(op, psi_t, psi_so, 0));
+      LB_TRY(launch_moments_pos(op, psi_so, tphi));
+    } else {
+      LB_TRY(launch_sweep(op, psi_t, psi_t, 0, 1));
+      LB_TRY(launch_moments(op, psi_t, tphi, 1, op->NENK));
+    }
     return launch_axpby(str, nphi, 1.0, v, -1.0, tphi, v);
   };
   // x0 = rhs / sigma_t of each problem

@@ -146,6 +146,7 @@ struct BatchTable {
   std::vector<int32_t> lvl;              // absolute level offsets (sweep positions in batch order)
   std::vector<uint32_t> tasks;           // [NI][NE] task words, problem-major inside each direction
   DevBuf d_items, d_lvl, d_tasks, d_rec, d_active, d_counter, d_coef, d_part, d_out;
+  DevBuf d_inv, d_rec_pos;               // sweep-order flux of the persistent sweep: positions, records
   double* h_io = nullptr;                // pinned host staging for per-problem scalars
   std::map<int, SegTable> seg;           // by rows
   int ncu = 0;
@@ -209,6 +210,7 @@ struct lb_op {
   int group_batch = 1;              // energy groups per coupled solve in the multigroup driver
   lb::BatchTable batch;             // independent problems (set by lb_op_set_problems)
   std::map<int, lb::SweepTune> btune;  // autotuner state of batched sweeps, per group-batch size
+  std::map<int, lb::SweepTune> btune_pos;  // the same for batched one-group sweeps in sweep order, by lb::kOrder*
   std::vector<int32_t> h_nbr;       // int32 [NE][4] neighbours (-1 boundary)
   std::vector<double> h_detJ;       // [NE] 6 x volume
   std::vector<double> h_vert;       // [NE][4][3] vertex coordinates in local node order
@@ -276,7 +278,9 @@ int launch_moments_to_order(lb_op* op, const double* phi, double* psi_so, int g0
 // Sweep-order flux of one group to the natural layout.
 int launch_flux_to_natural(lb_op* op, const double* psi_so, double* psi);
 // Whether the within-group solves keep their angular flux in sweep order (the
-// operator's flux order, FP64 moment products of at most 16 moments, no batch).
+// operator's flux order, FP64 moment products of at most 16 moments; with a
+// problem set, only when its sweeps are the persistent kernel's, whose
+// problem-major order the positions then follow).
 bool sweep_order_flux(const lb_op* op);
 // One sweep with the direction-resident kernel at a fixed workgroup size (no autotuning).
 int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int G, int block);

@@ -1586,15 +1586,23 @@ static int ensure_records(lb_op* op, DevBuf& rec, const DevBuf& tasks, const int
   return LB_OK;
 }
 
-// The sweep position of every element per direction (built on first use).
+// The sweep position of every element per direction (built on first use): in
+// the operator's device order, or with a problem set in the persistent
+// sweep's problem-major order.
 static int ensure_inverse(lb_op* op) {
-  if (op->d_inv.p) return LB_OK;
+  const bool batched = op->batch.P > 0;
+  DevBuf& inv = batched ? op->batch.d_inv : op->d_inv;
+  if (inv.p) return LB_OK;
   const int64_t n = op->NI * op->NE;
-  LB_TRY(op->d_inv.reserve(n * sizeof(int32_t)));
+  LB_TRY(inv.reserve(n * sizeof(int32_t)));
   hipLaunchKernelGGL(inverse_order_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_tasks.as<const uint32_t>(), op->d_inv.as<int32_t>());
+                     (batched ? op->batch.d_tasks : op->d_tasks).as<const uint32_t>(), inv.as<int32_t>());
   LB_HIP(hipGetLastError());
   return LB_OK;
+}
+
+static const int32_t* order_inverse(const lb_op* op) {
+  return (op->batch.P > 0 ? op->batch.d_inv : op->d_inv).as<const int32_t>();
 }
 
 // The staging entries of a slot plan by sweep position (built on first use).
@@ -1604,7 +1612,7 @@ static int ensure_stage_positions(lb_op* op, TracePlan* plan) {
   LB_TRY(plan->d_stage_pos.reserve(plan->d_stage.bytes));
   hipLaunchKernelGGL(stage_positions_kernel, dim3((unsigned)op->NI), dim3(kBlock), 0, op->stream, op->NE,
                      (int64_t)op->maxlev + 2, plan->d_stage_off.as<const int32_t>(), plan->d_stage.as<const int2>(),
-                     op->d_inv.as<const int32_t>(), plan->d_stage_pos.as<int2>());
+                     order_inverse(op), plan->d_stage_pos.as<int2>());
   LB_HIP(hipGetLastError());
   return LB_OK;
 }
@@ -1658,9 +1666,9 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
                            int order = kOrderNatural) {
   const int mode = cfg.mode, block = cfg.block;
   const bool pos_out = order != kOrderNatural;
-  if (pos_out && (G != 1 || mode == LB_SWEEP_PERSISTENT || op->batch.P > 0 || (order == kOrderFlux && q == psi))) {
-    set_error("sweep: a sweep-order flux is one group of a single problem, out of place unless the source is in "
-              "sweep order too");
+  if (pos_out && (G != 1 || (op->batch.P > 0) != (mode == LB_SWEEP_PERSISTENT) || (order == kOrderFlux && q == psi))) {
+    set_error("sweep: a sweep-order flux is one group, swept by the persistent kernel on a problem set and by the "
+              "other kernels otherwise, out of place unless the source is in sweep order too");
     return LB_E_ARG;
   }
   if ((mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS || mode == LB_SWEEP_PERSISTENT) &&
@@ -1680,10 +1688,17 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       return LB_E_ARG;
     }
     BatchTable& B = op->batch;
-    LB_TRY(ensure_records(op, B.d_rec, B.d_tasks));
-    SweepArgs A{B.d_rec.as<const TaskRec>(), op->d_sigt.as<const double>(),
+    if (pos_out) {
+      LB_TRY(ensure_inverse(op));
+      LB_TRY(ensure_records(op, B.d_rec_pos, B.d_tasks, B.d_inv.as<const int32_t>()));
+    } else {
+      LB_TRY(ensure_records(op, B.d_rec, B.d_tasks));
+    }
+    SweepArgs A{(pos_out ? B.d_rec_pos : B.d_rec).as<const TaskRec>(), op->d_sigt.as<const double>(),
                 op->G, g0, G, op->NE, q, psi};
     A.vec1 = one_group_vectors(G, psi);
+    A.pos_out = pos_out ? 1 : 0;
+    A.pos_in = order == kOrderBoth ? 1 : 0;
     const bool nt = G >= kStreamGroups;
     const int4* items = B.d_items.as<const int4>();
     const int nitems = (int)(B.items.size() / 4);
@@ -1691,7 +1706,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
     const int32_t* act = B.d_active.as<const int32_t>();
     unsigned int* ctr = B.d_counter.as<unsigned int>();
     const dim3 grid((unsigned)(B.ncu * cfg.per_cu));
-    return replay_graph(op, q, psi, g0, G, cfg, 0, [&]() {
+    return replay_graph(op, q, psi, g0, G, cfg, order, [&]() {
       hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, op->stream, ctr);
       launch_persistent(block, nt, grid, op->stream, items, nitems, lvl, act, ctr, A);
     });
@@ -1699,7 +1714,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
 
   if (pos_out) {
     LB_TRY(ensure_inverse(op));
-    LB_TRY(ensure_records(op, op->d_rec_pos, op->d_tasks, op->d_inv.as<const int32_t>()));
+    LB_TRY(ensure_records(op, op->d_rec_pos, op->d_tasks, order_inverse(op)));
   } else {
     LB_TRY(ensure_records(op, op->d_rec, op->d_tasks));
   }
@@ -1791,7 +1806,8 @@ static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, i
   const bool batched = op->batch.P > 0;
   const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
   const int nconf = batched ? kBatchTuneConfigs : kTuneConfigs;
-  SweepTune& tn = batched ? op->btune[G] : (order != kOrderNatural ? op->tune_pos[order] : op->tune[G]);
+  SweepTune& tn = order != kOrderNatural ? (batched ? op->btune_pos[order] : op->tune_pos[order])
+                                         : (batched ? op->btune[G] : op->tune[G]);
   if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice], order);
   // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
   // sweep timed alone between the operator's events; configurations whose block
@@ -1799,8 +1815,11 @@ static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, i
   // groups per cell, whose LDS then holds only a few hundred cells (24 groups:
   // 200 ms against 53 ms per sweep, plus a second slot plan).  Results are
   // identical in every mode.
+  // (a sweep-order flux of a problem set follows the persistent kernel's order, so
+  // only its configurations are timed there)
   auto skip = [&](const SweepConfig& k) {
-    return (k.block != 0 && k.block < G) || (k.mode == LB_SWEEP_DIRECTION_LDS && G >= kStreamGroups);
+    return (k.block != 0 && k.block < G) || (k.mode == LB_SWEEP_DIRECTION_LDS && G >= kStreamGroups) ||
+           (batched && order != kOrderNatural && k.mode != LB_SWEEP_PERSISTENT);
   };
   int c = tn.trials % nconf;
   while (skip(table[c])) {
@@ -1835,14 +1854,14 @@ int launch_sweep_pos(lb_op* op, const double* q, double* psi_so, int g0, bool so
 
 bool sweep_order_flux(const lb_op* op) {
   return op->flux_order != LB_FLUX_NATURAL && op->nmom <= 16 && op->moment_fmt == LB_FMT_FP64 &&
-         op->batch.P == 0;
+         (op->batch.P == 0 || op->sweep_mode == LB_SWEEP_AUTO || op->sweep_mode == LB_SWEEP_PERSISTENT);
 }
 
 int launch_flux_to_natural(lb_op* op, const double* psi_so, double* psi) {
   LB_TRY(ensure_inverse(op));
   const int64_t n = op->NI * op->NE * 2;
   hipLaunchKernelGGL(flux_to_natural_kernel, dim3(grid_for(n)), dim3(kBlock), 0, op->stream, (int)op->NI, op->NE,
-                     op->d_inv.as<const int32_t>(), psi_so, psi);
+                     order_inverse(op), psi_so, psi);
   LB_HIP(hipGetLastError());
   return LB_OK;
 }
@@ -1980,7 +1999,7 @@ int launch_moments_pos(lb_op* op, const double* psi_so, double* phi) {
   LB_TRY(ensure_inverse(op));
 #define CALL(NMV)                                                                                                   \
   hipLaunchKernelGGL(moments_gather_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,     \
-                     op->d_D.as<const double>(), (int)op->NI, op->NE, op->d_inv.as<const int32_t>(), psi_so, phi)
+                     op->d_D.as<const double>(), (int)op->NI, op->NE, order_inverse(op), psi_so, phi)
   LB_NM_DISPATCH16(op->nmom, CALL)
 #undef CALL
   LB_HIP(hipGetLastError());
@@ -1997,7 +2016,7 @@ int launch_moments_to_order(lb_op* op, const double* phi, double* psi_so, int g0
 #define CALL(NMV)                                                                                                   \
   hipLaunchKernelGGL(moments_to_order_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,   \
                      op->d_M.as<const double>(), op->d_deg.as<const int32_t>(), (int)op->NI, op->NE, op->G, g0,      \
-                     op->L, op->d_mat.as<const int32_t>(), sigs, op->d_inv.as<const int32_t>(), phi, psi_so)
+                     op->L, op->d_mat.as<const int32_t>(), sigs, order_inverse(op), phi, psi_so)
   LB_NM_DISPATCH16(op->nmom, CALL)
 #undef CALL
   LB_HIP(hipGetLastError());

@@ -12,7 +12,9 @@ as the natural layout, bit for bit.
   gathered from the sweep-order flux equals D psi of the natural flux.
 * Multigroup flux-moment GMRES and FGMRES, and the single-group flux-moment
   MatInv, give the same moments, iterates and angular flux in all three
-  layouts."""
+  layouts.
+* On a problem set the persistent sweep's sweep-order flux (the batch's
+  problem-major order) and the batched GMRES agree with the natural layout."""
 import numpy as np
 import pytest
 
@@ -139,4 +141,45 @@ def test_single_group_moment_matinv_same_in_both_layouts():
     for order in ("sweep", "sweep_source"):
         assert np.array_equal(out[order][0], out["natural"][0]), order
         assert out[order][1] == out["natural"][1], order
+    op.close()
+
+
+def test_problem_set_sweep_order_flux():
+    """A problem set (five uneven cubes, S8): the persistent sweep writing the flux in
+    the batch's problem-major sweep order equals the natural persistent sweep mapped
+    back, at each persistent configuration; the batched GMRES gives the same moments
+    and iterates with the flux in sweep order as in the natural layout."""
+    import torch
+    from test_gpu_batch import batch_op, small_batch
+    um, eoff, sigt, sigs, probs = small_batch()
+    ang = angular.level_symmetric_s8()
+    op = batch_op(um, eoff, sigt, sigs, ang)
+    q = torch.rand(op.n, dtype=torch.float64, device="cuda")
+    op.set_sweep_mode("persistent")
+    for block, per_cu in ((256, 4), (512, 2), (1024, 2)):
+        op.set_direction_block(block)
+        op.set_workgroups_per_cu(per_cu)
+        ref = torch.full_like(q, float("nan"))
+        op.transport_matinv_dev(q, ref, 0, 1)
+        for _ in range(2):                       # graph replays with eager work between
+            so = torch.full_like(q, float("nan"))
+            op.transport_matinv_order_dev(q, so, 0)
+            op.synchronize()
+            nat = torch.full_like(q, float("nan"))
+            op.flux_to_natural_dev(so, nat)
+            assert torch.equal(nat, ref), (block, per_cu)
+    op.set_sweep_mode("levels")
+    with pytest.raises(Exception):               # positions follow the persistent kernel's order
+        op.transport_matinv_order_dev(q, so, 0)
+    op.set_sweep_mode("auto")
+    qmom = problems.uneven_beams_sources(op, eoff, probs, sigt, sigs).reshape(-1).contiguous()
+    out = {}
+    for order in ("natural", "sweep"):
+        op.set_flux_order(order)
+        phi = torch.empty(op.nmom * op.NENK, dtype=torch.float64, device="cuda")
+        stats = op.batch_solve_dev(qmom, phi, m=30, rtol=1e-10)
+        out[order] = (phi.cpu().numpy(), [s["iterates"] for s in stats])
+    assert np.array_equal(out["sweep"][0], out["natural"][0])
+    assert out["sweep"][1] == out["natural"][1]
+    assert op.sweep_order_choice()[0] == "persistent"
     op.close()
