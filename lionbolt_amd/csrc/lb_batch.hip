@@ -314,8 +314,12 @@ int batch_solve_dev(lb_op* op, const double* qmom, const lb_solver_opts* o, doub
     psi_so = op->w_so.as<double>();
   }
   LB_TRY(launch_moments_to_discrete(op, qmom, q, 0, 1, 0, false));
+  // ||T^-1 q|| per problem, the true residual's denominator: a problem's positions
+  // in the batch's sweep order are its element range, so the segments are the same
+  std::vector<double> bb(P), rr(P);
   if (so) {
     LB_TRY(launch_sweep_pos(op, q, psi_so, 0));
+    LB_TRY(seg_dot(op, tp, psi_so, psi_so, bb.data()));
     LB_TRY(launch_moments_pos(op, psi_so, rhs));
   } else {
     LB_TRY(launch_copy(str, n, q, psi_t));
@@ -496,20 +500,34 @@ int batch_solve_dev(lb_op* op, const double* qmom, const lb_solver_opts* o, doub
   }
   // psi = T^-1 (M S phi + q); true transport residual per problem
   LB_TRY(set_active(op, all));
-  double* psi = op->w_mg.as<double>();
-  LB_TRY(launch_copy(str, n, q, psi));
-  LB_TRY(launch_moments_to_discrete(op, sol, psi, 0, 1, 1));
-  LB_TRY(launch_sweep(op, psi, psi, 0, 1));
-  double* b = psi_t;
-  LB_TRY(launch_copy(str, n, q, b));
-  LB_TRY(launch_sweep(op, b, b, 0, 1));
-  LB_TRY(launch_copy(str, n, psi, q));
-  LB_TRY(boltzmann_matvec_dev(op, q, 0));
-  sweeps += 3;
-  std::vector<double> bb(P), rr(P);
-  LB_TRY(seg_dot(op, tp, b, b, bb.data()));
-  LB_TRY(launch_axpby(str, n, 1.0, b, -1.0, q, q));
-  LB_TRY(seg_dot(op, tp, q, q, rr.data()));
+  if (so) {
+    // T^-1 q - (I - T^-1 K) psi = T^-1 M S (D psi - phi), exact for this psi: two
+    // sweep-order sweeps, with ||T^-1 q|| from the first sweep of the solve
+    double* qs = op->w_mg.as<double>();
+    LB_TRY(launch_copy(str, n, q, qs));
+    LB_TRY(launch_moments_to_discrete(op, sol, qs, 0, 1, 1));
+    LB_TRY(launch_sweep_pos(op, qs, psi_so, 0));
+    LB_TRY(launch_moments_pos(op, psi_so, tphi));
+    LB_TRY(launch_axpby(str, nphi, 1.0, tphi, -1.0, sol, tphi));
+    LB_TRY(launch_moments_to_discrete(op, tphi, qs, 0, 1, 0));
+    LB_TRY(launch_sweep_pos(op, qs, psi_so, 0));
+    sweeps += 2;
+    LB_TRY(seg_dot(op, tp, psi_so, psi_so, rr.data()));
+  } else {
+    double* psi = op->w_mg.as<double>();
+    LB_TRY(launch_copy(str, n, q, psi));
+    LB_TRY(launch_moments_to_discrete(op, sol, psi, 0, 1, 1));
+    LB_TRY(launch_sweep(op, psi, psi, 0, 1));
+    double* b = psi_t;
+    LB_TRY(launch_copy(str, n, q, b));
+    LB_TRY(launch_sweep(op, b, b, 0, 1));
+    LB_TRY(launch_copy(str, n, psi, q));
+    LB_TRY(boltzmann_matvec_dev(op, q, 0));
+    sweeps += 3;
+    LB_TRY(seg_dot(op, tp, b, b, bb.data()));
+    LB_TRY(launch_axpby(str, n, 1.0, b, -1.0, q, q));
+    LB_TRY(seg_dot(op, tp, q, q, rr.data()));
+  }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   int rc_all = LB_OK;
   for (int p = 0; p < P; ++p) {

@@ -14,7 +14,8 @@ as the natural layout, bit for bit.
   MatInv, give the same moments, iterates and angular flux in all three
   layouts.
 * On a problem set the persistent sweep's sweep-order flux (the batch's
-  problem-major order) and the batched GMRES agree with the natural layout."""
+  problem-major order) and the batched GMRES agree with the natural layout
+  (the true residuals, evaluated by two different exact formulas, to 1e-3)."""
 import numpy as np
 import pytest
 
@@ -178,8 +179,11 @@ def test_problem_set_sweep_order_flux():
         op.set_flux_order(order)
         phi = torch.empty(op.nmom * op.NENK, dtype=torch.float64, device="cuda")
         stats = op.batch_solve_dev(qmom, phi, m=30, rtol=1e-10)
-        out[order] = (phi.cpu().numpy(), [s["iterates"] for s in stats])
+        out[order] = (phi.cpu().numpy(), [s["iterates"] for s in stats], [s["true_residual"] for s in stats])
     assert np.array_equal(out["sweep"][0], out["natural"][0])
     assert out["sweep"][1] == out["natural"][1]
+    # the sweep-order check evaluates T^-1 M S (D psi - phi), the natural one T^-1 q - (I - T^-1 K) psi
+    np.testing.assert_allclose(out["sweep"][2], out["natural"][2], rtol=1e-3)
+    assert max(out["sweep"][2]) <= 1e-9
     assert op.sweep_order_choice()[0] == "persistent"
     op.close()
