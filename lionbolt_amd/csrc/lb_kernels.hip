@@ -56,7 +56,6 @@ struct SweepArgs {
   // LDS sweep only: the slot plan of this group-batch size (lb_trace_plan.cpp)
   const short4* __restrict__ uslot = nullptr;    // [dir][pos]: LDS slot of each upstream trace, -1 global
   const int16_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
-  const int32_t* __restrict__ stage_off = nullptr;   // [dir][maxlev + 2]: staging entries per consumer level
   const int2* __restrict__ stage = nullptr;      // (LDS slot, element) copied from global memory
 };
 
@@ -659,10 +658,16 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
 template <int BLOCK, bool NT>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
                                                                     const int32_t* __restrict__ nlev, int maxlev,
+                                                                    const int32_t* __restrict__ stage_off,
                                                                     SweepArgs A) {
   extern __shared__ double ring_buf[];
   const int i = blockIdx.x;
-  const int32_t* so = A.stage_off + (int64_t)i * (maxlev + 2);
+  // the staging offsets as a kernel parameter of their own: a restrict pointer the
+  // kernel never stores through, so their wave-uniform reads are scalar loads and
+  // do not wait on the vector-memory counter, which at each level's end would also
+  // wait for the look-ahead loads just issued (a restrict member of SweepArgs does
+  // not carry that guarantee to the compiler)
+  const int32_t* so = stage_off + (int64_t)i * (maxlev + 2);
   if constexpr (NT)
     walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
                                                             TraceRing{ring_buf}, so);
@@ -1469,20 +1474,24 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
 
 template <bool NT>
 static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
-                                   int maxlev, const SweepArgs& A, size_t bytes) {
+                                   int maxlev, const int32_t* so, const SweepArgs& A, size_t bytes) {
   const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
                    : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
                    : block == 768 ? (const void*)&sweep_direction_lds_kernel<768, NT>
                                   : (const void*)&sweep_direction_lds_kernel<1024, NT>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, so,
+                       A);
   else if (block == 512)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, so,
+                       A);
   else if (block == 768)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<768, NT>), dim3(ndir), dim3(768), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<768, NT>), dim3(ndir), dim3(768), bytes, st, lo, nl, maxlev, so,
+                       A);
   else
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, so,
+                       A);
   return LB_OK;
 }
 
@@ -1742,13 +1751,13 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       const size_t bytes = (size_t)(plan->ring + 2 * plan->stage) * 4 * G * sizeof(double);
       A.uslot = plan->d_uslot.as<const short4>();
       A.wslot = plan->d_wslot.as<const int16_t>();
-      A.stage_off = plan->d_stage_off.as<const int32_t>();
+      const int32_t* so = plan->d_stage_off.as<const int32_t>();
       A.stage = (pos_out ? plan->d_stage_pos : plan->d_stage).as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
       if (nt)
-        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
+        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, so, A, bytes));
       else
-        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, A, bytes));
+        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, so, A, bytes));
       LB_HIP(hipGetLastError());
       LB_HIP(hipEventRecord(op->ev1, op->stream));
       return LB_OK;
