@@ -426,7 +426,8 @@ int lb_moments_to_discrete_order_dev(lb_op* op, const double* d_phi, double* d_p
  * on the same flux in the natural layout). */
 int lb_moments_order_dev(lb_op* op, const double* d_psi_so, double* d_phi);
 
-/* One group's sweep-order flux in the natural layout. */
+/* One group's sweep-order flux in the natural layout (the two vectors must not
+ * overlap). */
 int lb_flux_to_natural_dev(lb_op* op, const double* d_psi_so, double* d_psi);
 
 /* Multigroup downscatter solve (the group loop of Main.f90:326-476 with the

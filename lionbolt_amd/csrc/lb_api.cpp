@@ -783,8 +783,13 @@ int lb_moments_order_dev(lb_op* op, const double* d_psi_so, double* d_phi) {
 
 int lb_flux_to_natural_dev(lb_op* op, const double* d_psi_so, double* d_psi) {
   return guarded([&]() -> int {
-    if (!op || !d_psi_so || !d_psi || d_psi_so == d_psi) {
+    if (!op || !d_psi_so || !d_psi) {
       set_error("lb_flux_to_natural_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    const int64_t n = op->NI * op->NENK;
+    if (d_psi_so < d_psi + n && d_psi < d_psi_so + n) {
+      set_error("lb_flux_to_natural_dev: the two layouts' vectors overlap");
       return LB_E_ARG;
     }
     return launch_flux_to_natural(op, d_psi_so, d_psi);
