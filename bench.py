@@ -142,6 +142,11 @@ def settle_autotuner(op, m, ang, G=1):
             if op.sweep_order_choice(True)[0] != "auto":
                 break
             op.transport_matinv_order_dev(psi, psi, 0, source_in_order=True)
+        # the FP32 flux mode's sweeps (the LDS kernel at its three workgroup sizes, two timings each)
+        f32 = torch.empty(q.numel(), dtype=torch.float32, device="cuda")
+        for _ in range(8):
+            op.transport_matinv_order32_dev(q, psi, f32, 0)
+        del f32
     op.synchronize()
     dt = time.perf_counter() - t0
     del q, psi
@@ -150,7 +155,7 @@ def settle_autotuner(op, m, ang, G=1):
 
 
 # Layouts of the within-group angular flux (Operator.set_flux_order); the library's default first.
-FLUX_LAYOUTS = ("sweep", "sweep_source", "natural")
+FLUX_LAYOUTS = ("sweep", "sweep_source", "natural", "sweep_fp32")
 DEFAULT_FLUX_LAYOUT = FLUX_LAYOUTS[0]
 
 
@@ -185,9 +190,16 @@ def solve_leg(m, ang, rtol, group_batch=1):
             phn = torch.zeros_like(qmom)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            op.multigroup_solve_dev(S, phn, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
+            sl = op.multigroup_solve_dev(S, phn, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
             torch.cuda.synchronize()
             layouts[order] = {"seconds": time.perf_counter() - t1, "moments_bit_identical": bool(torch.equal(phn, phi))}
+            if order == "sweep_fp32":
+                # the Arnoldi steps' flux for D in FP32: its error against the FP64 solve
+                layouts[order].update({
+                    "sweeps": sum(x["sweeps"] for x in sl), "iterates": [x["iterates"] for x in sl],
+                    "max_true_residual": max(x["true_residual"] for x in sl),
+                    "scalar_flux_max_rel_diff_vs_fp64":
+                        float(((phn[:, 0] - phi[:, 0]).abs().amax(dim=1) / phi[:, 0].abs().amax(dim=1)).max())})
             del phn
         op.set_flux_order(DEFAULT_FLUX_LAYOUT)
     # scalar flux = sqrt(4 pi) x the l = 0 moment; group 1 adds its uncollided flux (Main.f90:443)
@@ -524,9 +536,11 @@ def high_order_leg(n=24, orders=(3, 7, 11, 15), reps=5):
     return out
 
 
-def single_group_leg(m, ang, reps=10):
+def single_group_leg(m, ang, reps=10, only=None):
     """One-group sweeps of the same mesh and directions (what a within-group solve
-    issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s."""
+    issues), per launch mode: milliseconds per sweep and cell-angle sweeps/s.
+    ``only``: the names of the variants to run (all when None), so that a counter
+    run sees one variant of a kernel name."""
     import torch
     from lionbolt_amd import problems
     from lionbolt_amd.ops import Operator
@@ -538,6 +552,8 @@ def single_group_leg(m, ang, reps=10):
     res = {}
     for mode in ("levels", "graph", "direction", "direction-256", "direction-512", "direction_lds-256",
                  "direction_lds-512", "auto"):
+        if only is not None and mode not in only:
+            continue
         if mode == "auto":
             op.set_sweep_mode("auto")
             for _ in range(32):
@@ -569,6 +585,8 @@ def single_group_leg(m, ang, reps=10):
                        lambda: op.transport_matinv_order_dev(psi, psi, 0, source_in_order=True)),
                       ("direction_lds-512-sweep-order", None), ("direction-512-sweep-order", None),
                       ("levels-sweep-order", None)):
+        if only is not None and name not in only:
+            continue
         if run is None:
             mode = name.rsplit("-sweep-order", 1)[0]
             if "-" in mode:
@@ -703,7 +721,7 @@ def run(args):
     if other_modes:
         out["other_sweep_modes"] = other_modes
     if args.single:
-        out["single_group_sweep"] = single_group_leg(m, ang)
+        out["single_group_sweep"] = single_group_leg(m, ang, only=args.single_modes)
         _note("single-group sweep modes done")
     if args.high_order:
         out["high_order_operator"] = high_order_leg()
@@ -757,6 +775,8 @@ def main():
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--no-solve", dest="solve", action="store_false")
     p.add_argument("--no-single", dest="single", action="store_false")
+    p.add_argument("--single-modes", type=lambda t: t.split(","), default=None,
+                   help="comma-separated one-group sweep variants to run (default: all)")
     p.add_argument("--no-high-order", dest="high_order", action="store_false",
                    help="skip the scattering orders 7, 11 and 15 of the reference's decks")
     p.add_argument("--solve-rtol", type=float, default=1e-6,

@@ -359,10 +359,15 @@ int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t
  *   stage     2 per entry: (LDS slot, element copied from global memory), filled
  *             only if *nstage holds at least the entry count; *nstage returns it
  *   sizes     {ring slots, staging slots per level parity}
- *   stats     upstream traces from the ring, staged, one level back, overflow */
+ *   stats     upstream traces from the ring, staged, one level back, overflow
+ *   gprod     [NI][NE]     1 = some consumer reads the cell's flux from global
+ *                          memory (a staging copy, or a consumer with any trace
+ *                          outside LDS, which reads all its traces there): the
+ *                          cells whose FP64 flux the FP32 flux mode stores
+ * Any output may be NULL. */
 int lb_trace_plan_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int G, int lds_bytes, int32_t* uslot,
                        int32_t* wslot, int32_t* stage_off, int32_t* stage, int64_t* nstage, int32_t* sizes,
-                       int64_t* stats);
+                       int64_t* stats, uint8_t* gprod);
 
 /* Outward unit normals n[e][f][3] as PostProcess.f90:48-102 computes them. */
 int lb_op_get_normals(lb_op* op, double* n);
@@ -401,7 +406,7 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
  * sweep mode is LB_SWEEP_AUTO or LB_SWEEP_PERSISTENT (the batched sweep of the
  * 64 uneven beams: 11.0 -> 8.0 ms).  Other operators keep the natural layout
  * whatever is set here. */
-enum { LB_FLUX_NATURAL = 0, LB_FLUX_SWEEP_ORDER = 1, LB_FLUX_SWEEP_ORDER_SOURCE = 2 };
+enum { LB_FLUX_NATURAL = 0, LB_FLUX_SWEEP_ORDER = 1, LB_FLUX_SWEEP_ORDER_SOURCE = 2, LB_FLUX_SWEEP_ORDER_FP32 = 3 };
 int lb_op_set_flux_order(lb_op* op, int order);
 
 /* The launch configuration LB_SWEEP_AUTO settled on for one-group sweeps with a
@@ -425,6 +430,20 @@ int lb_moments_to_discrete_order_dev(lb_op* op, const double* d_phi, double* d_p
 /* phi = D psi for one group's sweep-order flux (bit-identical to lb_moments_dev
  * on the same flux in the natural layout). */
 int lb_moments_order_dev(lb_op* op, const double* d_psi_so, double* d_phi);
+
+/* One group's T^-1 (the LDS sweep) storing the flux in sweep order as FP32 in
+ * d_psi32_so: each value is the FP64 flux rounded to nearest FP32, the sweep's
+ * arithmetic and its LDS traces staying FP64.  d_psi_so (FP64, sweep order)
+ * receives the flux only of the cells that some later cell reads from global
+ * memory; its other entries are left as they were.  d_q is natural and must
+ * not overlap the outputs.  The solves' LB_FLUX_SWEEP_ORDER_FP32 mode runs the
+ * Arnoldi steps' K on it; right-hand side, psi and the true residual stay FP64.
+ * No reference counterpart. */
+int lb_transport_matinv_order32_dev(lb_op* op, const double* d_q, double* d_psi_so, float* d_psi32_so, int g0);
+
+/* phi = D psi for one group's FP32 sweep-order flux (FP64 sums in the order of
+ * lb_moments_order_dev). */
+int lb_moments_order32_dev(lb_op* op, const float* d_psi32_so, double* d_phi);
 
 /* One group's sweep-order flux in the natural layout (the two vectors must not
  * overlap). */

@@ -16,7 +16,7 @@ SOLVER_GMRES, SOLVER_SI, SOLVER_FGMRES = 0, 1, 2
 SPACE_PSI, SPACE_MOMENT = 0, 1
 SCATTER_MOMENT, SCATTER_DENSE = 0, 1
 SWEEP_LEVELS, SWEEP_GRAPH, SWEEP_DIRECTION, SWEEP_AUTO, SWEEP_PERSISTENT, SWEEP_DIRECTION_LDS = 0, 1, 2, 3, 4, 5
-FLUX_NATURAL, FLUX_SWEEP_ORDER, FLUX_SWEEP_ORDER_SOURCE = 0, 1, 2
+FLUX_NATURAL, FLUX_SWEEP_ORDER, FLUX_SWEEP_ORDER_SOURCE, FLUX_SWEEP_ORDER_FP32 = 0, 1, 2, 3
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i64 = ctypes.POINTER(ctypes.c_int64)
@@ -104,7 +104,7 @@ SIGNATURES = {
     "lb_op_get_schedule": (ctypes.c_int, [_vp, _i32, _i32]),
     "lb_schedule_host": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), _i32, _i32, _d]),
     "lb_trace_plan_host": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), ctypes.c_int,
-                                          ctypes.c_int, _i32, _i32, _i32, _i32, _i64, _i32, _i64]),
+                                          ctypes.c_int, _i32, _i32, _i32, _i32, _i64, _i32, _i64, _vp]),
     "lb_op_get_normals": (ctypes.c_int, [_vp, _d]),
     "lb_op_info": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64]),
     "lb_transport_matinv_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
@@ -116,6 +116,8 @@ SIGNATURES = {
     "lb_transport_matinv_order_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "lb_moments_to_discrete_order_dev": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "lb_moments_order_dev": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lb_transport_matinv_order32_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "lb_moments_order32_dev": (ctypes.c_int, [_vp, _vp, _vp]),
     "lb_flux_to_natural_dev": (ctypes.c_int, [_vp, _vp, _vp]),
     "lb_multigroup_solve_dev": (ctypes.c_int, [_vp, _vp, _d, ctypes.POINTER(SolverOpts), _vp,
                                                ctypes.POINTER(SolveStats)]),

@@ -203,7 +203,7 @@ int lb_schedule_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int32_t
 
 int lb_trace_plan_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int G, int lds_bytes, int32_t* uslot,
                        int32_t* wslot, int32_t* stage_off, int32_t* stage, int64_t* nstage, int32_t* sizes,
-                       int64_t* stats) {
+                       int64_t* stats, uint8_t* gprod) {
   return guarded([&]() -> int {
     if (!mesh || !quad || !mesh->rg || !mesh->node || !mesh->nbr || !mesh->mat || !quad->k || !quad->w ||
         mesh->NE <= 0 || mesh->NE > LB_MAX_ELEMENTS || quad->NI <= 0 || G < 1 || lds_bytes <= 0 ||
@@ -227,7 +227,7 @@ int lb_trace_plan_host(const lb_mesh_desc* mesh, const lb_quad_desc* quad, int G
     std::vector<int32_t> us(NI * NE * 4), ws(NI * NE), so(NI * stride), ent;
     int64_t sts[4];
     LB_TRY(build_trace_plan(NI, NE, op.maxlev, op.h_nlev.data(), op.h_lvl_off.data(), op.h_tasks.data(),
-                            op.h_nbr.data(), ring, st, us.data(), ws.data(), so.data(), ent, sts));
+                            op.h_nbr.data(), ring, st, us.data(), ws.data(), so.data(), ent, sts, gprod));
     if (uslot) std::memcpy(uslot, us.data(), us.size() * sizeof(int32_t));
     if (wslot) std::memcpy(wslot, ws.data(), ws.size() * sizeof(int32_t));
     if (stage_off) std::memcpy(stage_off, so.data(), so.size() * sizeof(int32_t));
@@ -712,9 +712,9 @@ int lb_moments_dev(lb_op* op, const double* d_psi, double* d_phi, int G) {
 
 int lb_op_set_flux_order(lb_op* op, int order) {
   return guarded([&]() -> int {
-    if (!op || order < LB_FLUX_NATURAL || order > LB_FLUX_SWEEP_ORDER_SOURCE) {
-      set_error("lb_op_set_flux_order: order must be LB_FLUX_NATURAL, LB_FLUX_SWEEP_ORDER or "
-                "LB_FLUX_SWEEP_ORDER_SOURCE");
+    if (!op || order < LB_FLUX_NATURAL || order > LB_FLUX_SWEEP_ORDER_FP32) {
+      set_error("lb_op_set_flux_order: order must be LB_FLUX_NATURAL, LB_FLUX_SWEEP_ORDER, "
+                "LB_FLUX_SWEEP_ORDER_SOURCE or LB_FLUX_SWEEP_ORDER_FP32");
       return LB_E_ARG;
     }
     op->flux_order = order;
@@ -768,6 +768,36 @@ int lb_moments_to_discrete_order_dev(lb_op* op, const double* d_phi, double* d_p
       return LB_E_ARG;
     }
     return launch_moments_to_order(op, d_phi, d_psi_so, g0);
+  });
+}
+
+int lb_transport_matinv_order32_dev(lb_op* op, const double* d_q, double* d_psi_so, float* d_psi32_so, int g0) {
+  return guarded([&]() -> int {
+    LB_TRY(check_ready(op, "lb_transport_matinv_order32_dev"));
+    if (g0 < 0 || g0 >= op->G || !d_q || !d_psi_so || !d_psi32_so) {
+      set_error("lb_transport_matinv_order32_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    const int64_t n = op->NI * op->NENK;
+    const char* qa = reinterpret_cast<const char*>(d_q);
+    const char* pa = reinterpret_cast<const char*>(d_psi_so);
+    const char* fa = reinterpret_cast<const char*>(d_psi32_so);
+    auto overlap = [](const char* a, size_t na, const char* b, size_t nb) { return a < b + nb && b < a + na; };
+    if (overlap(qa, n * 8, pa, n * 8) || overlap(qa, n * 8, fa, n * 4) || overlap(pa, n * 8, fa, n * 4)) {
+      set_error("lb_transport_matinv_order32_dev: the source and the outputs overlap");
+      return LB_E_ARG;
+    }
+    return launch_sweep_f32(op, d_q, d_psi_so, d_psi32_so, g0);
+  });
+}
+
+int lb_moments_order32_dev(lb_op* op, const float* d_psi32_so, double* d_phi) {
+  return guarded([&]() -> int {
+    if (!op || !d_psi32_so || !d_phi) {
+      set_error("lb_moments_order32_dev: invalid argument");
+      return LB_E_ARG;
+    }
+    return launch_moments_pos32(op, d_psi32_so, d_phi);
   });
 }
 
@@ -859,6 +889,7 @@ void lb_op_destroy(lb_op* op) {
   op->d_rec_pos.release();
   op->d_inv.release();
   op->w_so.release();
+  op->w_so32.release();
   for (DevBuf* b : {&op->batch.d_items, &op->batch.d_lvl, &op->batch.d_tasks, &op->batch.d_rec,
                     &op->batch.d_inv, &op->batch.d_rec_pos,
                     &op->batch.d_active, &op->batch.d_counter, &op->batch.d_coef, &op->batch.d_part,
@@ -877,7 +908,7 @@ void lb_op_destroy(lb_op* op) {
   op->d_nlev.release();
   for (auto& kv : op->plans)
     for (DevBuf* b : {&kv.second.d_uslot, &kv.second.d_wslot, &kv.second.d_stage_off, &kv.second.d_stage,
-                      &kv.second.d_stage_pos})
+                      &kv.second.d_stage_pos, &kv.second.d_gprod})
       b->release();
   op->d_xsdd.release();
   op->w_phis.release();

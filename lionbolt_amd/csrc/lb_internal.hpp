@@ -95,6 +95,9 @@ struct SweepConfig {
 // Layout of a one-group sweep's vectors: both natural; the flux in sweep order
 // from a natural source; source and flux in sweep order.
 constexpr int kOrderNatural = 0, kOrderFlux = 1, kOrderBoth = 2;
+// The flux in sweep order as FP32 (one group, LDS sweep), from a natural source;
+// the FP64 flux only where a later cell reads it from global memory.
+constexpr int kOrderFlux32 = 3;
 // (The LDS sweep at 1024 threads spills registers and is not timed; at 768 it
 // runs three waves per SIMD on one round of look-ahead.)
 constexpr int kTuneConfigs = 7;
@@ -124,6 +127,8 @@ struct TracePlan {
   DevBuf d_stage_off;        // int32 [NI][maxlev + 2]: first staging entry of each consumer level
   DevBuf d_stage;            // int2 entries: (LDS slot, element whose flux is copied)
   DevBuf d_stage_pos;        // the same entries with the element's sweep position (sweep-order flux)
+  DevBuf d_gprod;            // uint8 [NI][NE] (one group): 1 where a consumer reads the cell's flux from
+                             // global memory (staged or directly), 0 where only ring slots do
 };
 
 // Most energy groups solved together by the group-batched multigroup solve.
@@ -238,6 +243,7 @@ struct lb_op {
   lb::DevBuf w_tq;                  // T^-1 q of the current solve, for its true residual
   lb::DevBuf w_tail;                // batched residual checks of the multigroup solve (freed after it)
   lb::DevBuf w_so;                  // one group's angular flux in sweep order (within-group solves)
+  lb::DevBuf w_so32;                // the same in FP32 (LB_FLUX_SWEEP_ORDER_FP32)
 };
 
 namespace lb {
@@ -257,10 +263,13 @@ int trace_plan_sizes(int lds_bytes, int G, int* ring, int* stage);
 // Plans every direction of a level-ordered task array (tasks [NI][NE], level
 // offsets [NI][maxlev + 1]); uslot [NI][NE][4], wslot [NI][NE], stage_off
 // [NI][maxlev + 2] (entry indices into entries, 2 ints per entry); stats: upstream
-// traces from the ring, staged, one level back, staging overflow.
+// traces from the ring, staged, one level back, staging overflow; gprod
+// (optional) [NI][NE]: 1 where a consumer reads the cell's flux from global
+// memory (a staging copy or a direct read).
 int build_trace_plan(int64_t NI, int64_t NE, int maxlev, const int32_t* nlev, const int32_t* lvl_off,
                      const uint32_t* tasks, const int32_t* nbr, int ring, int stage, int32_t* uslot, int32_t* wslot,
-                     int32_t* stage_off, std::vector<int32_t>& entries, int64_t stats[4]);
+                     int32_t* stage_off, std::vector<int32_t>& entries, int64_t stats[4],
+                     uint8_t* gprod = nullptr);
 LevelTable* level_table(lb_op* op, int G, int* rc);
 
 // lb_kernels.hip (launchers; all enqueue on op->stream)
@@ -270,6 +279,12 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G);
 // source q is in the natural layout (then it must not alias psi_so) or, with
 // source_in_order, in sweep order too (in place allowed).
 int launch_sweep_pos(lb_op* op, const double* q, double* psi_so, int g0, bool source_in_order = false);
+// One group's sweep (the LDS kernel) writing the flux in sweep order as FP32 to
+// psi32_so; psi_so (FP64, sweep order) receives only the cells whose flux a later
+// cell reads from global memory, and holds nothing else of use.
+int launch_sweep_f32(lb_op* op, const double* q, double* psi_so, float* psi32_so, int g0);
+// phi = D psi for one group's FP32 sweep-order flux (FP64 sums).
+int launch_moments_pos32(lb_op* op, const float* psi32_so, double* phi);
 // phi = D psi_so for one group's sweep-order flux (bit-identical to launch_moments
 // on the same flux in natural layout).
 int launch_moments_pos(lb_op* op, const double* psi_so, double* phi);
@@ -282,6 +297,9 @@ int launch_flux_to_natural(lb_op* op, const double* psi_so, double* psi);
 // problem set, only when its sweeps are the persistent kernel's, whose
 // problem-major order the positions then follow).
 bool sweep_order_flux(const lb_op* op);
+// Whether they keep it in sweep order as FP32 (LB_FLUX_SWEEP_ORDER_FP32; single
+// problems, FP64 moment products of at most 16 moments).
+bool sweep_order_fp32(const lb_op* op);
 // One sweep with the direction-resident kernel at a fixed workgroup size (no autotuning).
 int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int G, int block);
 int launch_moments(lb_op* op, const double* psi, double* phi, int G, int64_t ncols);

@@ -41,6 +41,12 @@ struct TaskRec {
 };
 static_assert(sizeof(TaskRec) == 48, "three 16-byte loads per record");
 
+// Kinds of LDS trace handling of the walks: none; LDS slots by the slot plan;
+// the same with the one-group FP32 flux output (kLdsF32): the flux goes to an
+// FP32 vector in sweep order, and the FP64 flux is written only for the cells
+// whose flux some consumer reads from global memory (the plan's gprod marks).
+constexpr int kLdsNone = 0, kLdsPlan = 1, kLdsF32 = 2;
+
 // Arguments shared by the sweep kernels.
 struct SweepArgs {
   const TaskRec* __restrict__ rec;
@@ -53,6 +59,8 @@ struct SweepArgs {
   int pos_out = 0;   // flux stored at the cell's sweep position instead of its element (one group); the
                      // records then carry the neighbours' positions and the staging entries positions
   int pos_in = 0;    // source read at the cell's sweep position (with pos_out; q may then alias psi)
+  float* psi32 = nullptr;                        // kLdsF32: the flux in FP32, sweep order
+  const uint8_t* __restrict__ gprod = nullptr;   // kLdsF32: [dir][pos] 1 = the FP64 flux is read globally
   // LDS sweep only: the slot plan of this group-batch size (lb_trace_plan.cpp)
   const short4* __restrict__ uslot = nullptr;    // [dir][pos]: LDS slot of each upstream trace, -1 global
   const int16_t* __restrict__ wslot = nullptr;   // [dir][pos]: slot the cell writes, -1 none
@@ -98,6 +106,7 @@ struct CellPrep {
   uint32_t pm;  // node permutation (24 bits)
   int ws;     // LDS sweep only: slot this cell writes and slots of its upstream traces
   int4 us;
+  int gp;     // kLdsF32: the FP64 flux is also written to global memory
 };
 
 // A cell's record as loaded (and, in the LDS sweep, its slot-plan entries);
@@ -106,9 +115,10 @@ struct RecRaw {
   u32x4 a, b, c;
   int ws;
   short4 us;
+  int gp;
 };
 
-template <bool NT, bool LDS>
+template <bool NT, int LDS>
 __device__ __forceinline__ void load_rec(const SweepArgs& A, int i, int s, RecRaw& R) {
   const int64_t k = (int64_t)i * A.NE + s;
   const u32x4* q = reinterpret_cast<const u32x4*>(A.rec + k);
@@ -125,14 +135,16 @@ __device__ __forceinline__ void load_rec(const SweepArgs& A, int i, int s, RecRa
     R.ws = A.wslot[k];
     R.us = A.uslot[k];
   }
+  if constexpr (LDS == kLdsF32) R.gp = A.gprod[k];
 }
 
-template <bool NT, bool LDS>
+template <bool NT, int LDS>
 __device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int s, int g, const RecRaw& R, CellPrep& P) {
   if constexpr (LDS) {
     P.ws = R.ws;
     P.us = make_int4(R.us.x, R.us.y, R.us.z, R.us.w);
   }
+  if constexpr (LDS == kLdsF32) P.gp = R.gp;
   P.w = R.a.x;
   P.pm = R.a.y & 0xFFFFFFu;
   P.nb = make_int4((int)R.a.z, (int)R.a.w, (int)R.b.x, (int)R.b.y);
@@ -147,7 +159,7 @@ __device__ __forceinline__ void decode_rec(const SweepArgs& A, int i, int s, int
   for (int a = 0; a < 4; ++a) P.b[a] = NT ? __builtin_nontemporal_load(A.q + base + a * A.G) : A.q[base + a * A.G];
 }
 
-template <bool NT, bool LDS = false>
+template <bool NT, int LDS = 0>
 __device__ __forceinline__ void prep_cell(const SweepArgs& A, int i, int s, int g, CellPrep& P) {
   RecRaw R;
   load_rec<NT, LDS>(A, i, s, R);
@@ -205,7 +217,7 @@ __device__ __forceinline__ double pivot_recip(double x) {
   return fma(r, e, r);
 }
 
-template <int SRC>
+template <int SRC, int LDS = kLdsNone>
 __device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                                     const TraceRing& ring, const double (&u)[4][3]) {
   const uint32_t w = P.w;
@@ -315,7 +327,16 @@ __device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, i
   // order, so fewer store instructions also shorten the look-ahead's waits).
   // The same for the source loads and the LDS writes measured no gain (loads:
   // 6.48 ms).
-  if (A.vec1) {
+  if constexpr (LDS == kLdsF32) {
+    // one group, sweep order: the FP32 flux for the moments, the FP64 flux only
+    // where a later cell reads it from global memory (its LDS traces are FP64)
+    *reinterpret_cast<float4*>(A.psi32 + base) = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+    if (P.gp) {
+      double2* o = reinterpret_cast<double2*>(A.psi + base);
+      o[0] = make_double2(x[0], x[1]);
+      o[1] = make_double2(x[2], x[3]);
+    }
+  } else if (A.vec1) {
     double2* o = reinterpret_cast<double2*>(A.psi + base);
     o[0] = make_double2(x[0], x[1]);
     o[1] = make_double2(x[2], x[3]);
@@ -332,12 +353,12 @@ __device__ __forceinline__ void solve_with_upstream(const SweepArgs& A, int i, i
   }
 }
 
-template <int SRC>
+template <int SRC, int LDS = kLdsNone>
 __device__ __forceinline__ void finish_cell_body(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                                  const TraceRing& ring) {
   double u[4][3];
   load_upstream<SRC>(A, i, P, g, ring, u);
-  solve_with_upstream<SRC>(A, i, s, P, g, ring, u);
+  solve_with_upstream<SRC, LDS>(A, i, s, P, g, ring, u);
 }
 
 __device__ __forceinline__ bool ring_hit(const CellPrep& P) {
@@ -351,20 +372,20 @@ __device__ __forceinline__ bool ring_hit(const CellPrep& P) {
 // separate to the end of the cell, so a wave whose cells all hit never waits on
 // the vector-memory counter, which would also wait for the look-ahead loads
 // issued before it.
-template <bool LDS = false>
+template <int LDS = 0>
 __device__ __forceinline__ void finish_cell(const SweepArgs& A, int i, int s, const CellPrep& P, int g,
                                             const TraceRing& ring = TraceRing{nullptr}) {
   if constexpr (LDS) {
     if (ring_hit(P))
-      finish_cell_body<kTraceRing>(A, i, s, P, g, ring);
+      finish_cell_body<kTraceRing, LDS>(A, i, s, P, g, ring);
     else
-      finish_cell_body<kTraceGlobalRing>(A, i, s, P, g, ring);
+      finish_cell_body<kTraceGlobalRing, LDS>(A, i, s, P, g, ring);
   } else {
     finish_cell_body<kTraceGlobal>(A, i, s, P, g, ring);
   }
 }
 
-template <bool NT, bool LDS = false>
+template <bool NT, int LDS = 0>
 __device__ __forceinline__ void solve_cell(const SweepArgs& A, int i, int s, int g,
                                            const TraceRing& ring = TraceRing{nullptr}) {
   CellPrep P;
@@ -414,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void sweep_level_kernel(const int4* __restr
 
 // Static part of the cells this lane sweeps in one level: up to R rounds of
 // BLOCK / G cells each.
-template <int R, bool NT, bool LDS = false>
+template <int R, bool NT, int LDS = 0>
 __device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                            bool lane_ok, CellPrep (&P)[R]) {
 #pragma unroll
@@ -424,7 +445,7 @@ __device__ __forceinline__ void prep_level(const SweepArgs& A, int i, int s0, in
   }
 }
 
-template <int R, bool NT, bool LDS = false>
+template <int R, bool NT, int LDS = 0>
 __device__ __forceinline__ void finish_level(const SweepArgs& A, int i, int s0, int s1, int cpr, int c, int g,
                                              bool lane_ok, const CellPrep (&P)[R],
                                              const TraceRing& ring = TraceRing{nullptr}) {
@@ -448,7 +469,7 @@ struct StageCopy {
   int dst;   // LDS double index, -1 when the lane copies nothing
 };
 
-template <bool LDS>
+template <int LDS>
 __device__ __forceinline__ StageCopy stage_load(const SweepArgs& A, int i, const int32_t* __restrict__ so, int l) {
   StageCopy c{0.0, -1};
   if constexpr (LDS) {
@@ -465,7 +486,7 @@ __device__ __forceinline__ StageCopy stage_load(const SweepArgs& A, int i, const
   return c;
 }
 
-template <bool LDS>
+template <int LDS>
 __device__ __forceinline__ void stage_store(const TraceRing& ring, const StageCopy& c) {
   if constexpr (LDS) {
     if (c.dst >= 0) ring.buf[c.dst] = c.v;
@@ -481,7 +502,7 @@ __device__ __forceinline__ void stage_store(const TraceRing& ring, const StageCo
 // (measured: 24 groups at 1024 threads 56.4 ms per sweep, against 80.7 ms in
 // chunks of R rounds, which spill more).  so: the direction's staging offsets
 // (LDS sweep only).  Few groups per cell take walk_levels_two_stage.
-template <int BLOCK, int R, bool NT, bool LDS = false>
+template <int BLOCK, int R, bool NT, int LDS = 0>
 __device__ __forceinline__ void walk_levels(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
                                             TraceRing ring = TraceRing{nullptr},
                                             const int32_t* __restrict__ so = nullptr) {
@@ -536,7 +557,7 @@ __device__ __forceinline__ Chunk next_chunk(const int32_t* __restrict__ off, int
 // waits for was issued a chunk earlier, and no load address waits on a load of
 // the same chunk.  The look-ahead issues the same loads on every path (lanes past
 // a chunk's end re-read its last cell).
-template <int R, bool LDS, bool NT = false>
+template <int R, int LDS, bool NT = false>
 __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off, int nl,
                                                 int chunk, int cpr, int c, int g, bool lane_ok, Chunk& cur, Chunk& nx,
                                                 bool& has_nx, const CellPrep (&P)[R], CellPrep (&Pn)[R],
@@ -573,7 +594,7 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int s = cur.s0 + c + r * cpr;
-    if (lane_ok && s < cur.s1) solve_with_upstream<kSolve>(A, i, s, P[r], g, ring, u[r]);
+    if (lane_ok && s < cur.s1) solve_with_upstream<kSolve, LDS>(A, i, s, P[r], g, ring, u[r]);
   }
   stage_store<LDS>(ring, sc);
   if (last) __syncthreads();
@@ -591,7 +612,7 @@ __device__ __forceinline__ bool chunk_two_stage(const SweepArgs& A, int i, const
 // kernel at 512 threads 5.07 -> 4.70 ms per sweep inside the solve, at 256
 // threads 6.23 -> 5.62 ms; plain direction kernel at 256 threads 6.79 -> 6.43
 // ms; two groups at 512 threads 8.58 -> 8.13 ms.
-template <int BLOCK, int R, bool LDS = false, bool NT = false>
+template <int BLOCK, int R, int LDS = 0, bool NT = false>
 __device__ __forceinline__ void walk_levels_two_stage(const SweepArgs& A, int i, const int32_t* __restrict__ off,
                                                       int nl, TraceRing ring = TraceRing{nullptr},
                                                       const int32_t* __restrict__ so = nullptr) {
@@ -646,7 +667,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
   // the two-stage walk at any group count (8 groups, the multigroup solve's tail, at 512 threads: 19.6 -> 17.3 ms
   // per sweep of 384k tets x 256 directions against one step per level; 24 groups 57.4 -> 54.8 ms at 512
   // threads, 53.7 -> 55.7 ms at 1024, where level launches stay faster)
-  walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, false, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1),
+  walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, kLdsNone, NT>(A, i, lvl_off + (int64_t)i * (maxlev + 1),
                                                                     nlev[i]);
 }
 
@@ -655,7 +676,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_kernel(const int32_t* _
 // upstream traces come from LDS where the plan holds them, otherwise from global
 // memory.  The flux is written to both; the workgroup barrier between levels
 // orders the LDS writes and reads.
-template <int BLOCK, bool NT>
+template <int BLOCK, bool NT, bool F32 = false>
 __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_t* __restrict__ lvl_off,
                                                                     const int32_t* __restrict__ nlev, int maxlev,
                                                                     const int32_t* __restrict__ stage_off,
@@ -669,11 +690,11 @@ __global__ __launch_bounds__(BLOCK) void sweep_direction_lds_kernel(const int32_
   // not carry that guarantee to the compiler)
   const int32_t* so = stage_off + (int64_t)i * (maxlev + 2);
   if constexpr (NT)
-    walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
-                                                            TraceRing{ring_buf}, so);
+    walk_levels<BLOCK, LevelRounds<BLOCK>::value, NT, kLdsPlan>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
+                                                                TraceRing{ring_buf}, so);
   else
-    walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, true>(A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i],
-                                                                  TraceRing{ring_buf}, so);
+    walk_levels_two_stage<BLOCK, LevelRounds<BLOCK>::value, F32 ? kLdsF32 : kLdsPlan>(
+        A, i, lvl_off + (int64_t)i * (maxlev + 1), nlev[i], TraceRing{ring_buf}, so);
 }
 
 // Persistent sweep of a batch of independent problems: a fixed grid (a few
@@ -960,6 +981,66 @@ __global__ __launch_bounds__(kBlock) void moments_to_discrete_pair_kernel(
     }
     __builtin_nontemporal_store(dbl2{x0, x1}, dst);
   }
+}
+
+// D psi for one group whose flux is FP32 in sweep order (the FP32 flux mode):
+// as moments_gather_pair_kernel, 8-byte pieces converted to FP64 before the
+// FP64 sums.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void moments_gather32_pair_kernel(const double* __restrict__ D, int NI,
+                                                                       int64_t NE, const int32_t* __restrict__ inv,
+                                                                       const float* __restrict__ psi32,
+                                                                       double* __restrict__ phi) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const int64_t ncols = NE * 4;
+  const int64_t c0 = (xcd_block() * kBlock + threadIdx.x) * 2;
+  if (c0 >= ncols) return;
+  const int64_t e = c0 >> 2;
+  const int h = int(c0 & 3);
+  double a0[NM], a1[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) a0[m] = a1[m] = 0.0;
+  const int32_t* ip = inv + e;
+  const float* ps = psi32 + h;
+  int pn[kGatherRows];
+  const int nfull = NI - NI % kGatherRows;
+  if (nfull > 0) {
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r) pn[r] = __builtin_nontemporal_load(ip + (int64_t)r * NE);
+  }
+  for (int i0 = 0; i0 < nfull; i0 += kGatherRows) {
+    f32x2 v[kGatherRows];
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r)
+      v[r] = *reinterpret_cast<const f32x2*>(ps + ((int64_t)(i0 + r) * NE + pn[r]) * 4);
+    if (i0 + kGatherRows < nfull) {
+#pragma unroll
+      for (int r = 0; r < kGatherRows; ++r)
+        pn[r] = __builtin_nontemporal_load(ip + (int64_t)(i0 + kGatherRows + r) * NE);
+    }
+#pragma unroll
+    for (int r = 0; r < kGatherRows; ++r) {
+      const double x0 = (double)v[r].x, x1 = (double)v[r].y;
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const double d = D[m * NI + i0 + r];
+        a0[m] = fma(d, x0, a0[m]);
+        a1[m] = fma(d, x1, a1[m]);
+      }
+    }
+  }
+  for (int i0 = nfull; i0 < NI; ++i0) {
+    const f32x2 v = *reinterpret_cast<const f32x2*>(ps + ((int64_t)i0 * NE + ip[(int64_t)i0 * NE]) * 4);
+    const double x0 = (double)v.x, x1 = (double)v.y;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      const double d = D[m * NI + i0];
+      a0[m] = fma(d, x0, a0[m]);
+      a1[m] = fma(d, x1, a1[m]);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NM; ++m) *reinterpret_cast<dbl2*>(phi + (int64_t)m * ncols + c0) = dbl2{a0[m], a1[m]};
 }
 
 // M S phi for one group written in sweep order: psi_so[i][inv[i][e]] gets cell
@@ -1472,26 +1553,26 @@ static void launch_direction(int block, bool nt, unsigned ndir, hipStream_t st, 
     launch_direction_nt<false>(block, ndir, st, lo, nl, maxlev, A);
 }
 
-template <bool NT>
+template <bool NT, bool F32 = false>
 static int launch_direction_lds_nt(int block, unsigned ndir, hipStream_t st, const int32_t* lo, const int32_t* nl,
                                    int maxlev, const int32_t* so, const SweepArgs& A, size_t bytes) {
-  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT>
-                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT>
-                   : block == 768 ? (const void*)&sweep_direction_lds_kernel<768, NT>
-                                  : (const void*)&sweep_direction_lds_kernel<1024, NT>;
+  const void* fn = block == 256   ? (const void*)&sweep_direction_lds_kernel<256, NT, F32>
+                   : block == 512 ? (const void*)&sweep_direction_lds_kernel<512, NT, F32>
+                   : block == 768 ? (const void*)&sweep_direction_lds_kernel<768, NT, F32>
+                                  : (const void*)&sweep_direction_lds_kernel<1024, NT, F32>;
   LB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   if (block == 256)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev, so,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<256, NT, F32>), dim3(ndir), dim3(256), bytes, st, lo, nl, maxlev,
+                       so, A);
   else if (block == 512)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev, so,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<512, NT, F32>), dim3(ndir), dim3(512), bytes, st, lo, nl, maxlev,
+                       so, A);
   else if (block == 768)
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<768, NT>), dim3(ndir), dim3(768), bytes, st, lo, nl, maxlev, so,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<768, NT, F32>), dim3(ndir), dim3(768), bytes, st, lo, nl, maxlev,
+                       so, A);
   else
-    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev, so,
-                       A);
+    hipLaunchKernelGGL((sweep_direction_lds_kernel<1024, NT, F32>), dim3(ndir), dim3(1024), bytes, st, lo, nl, maxlev,
+                       so, A);
   return LB_OK;
 }
 
@@ -1526,9 +1607,10 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
     return nullptr;
   }
   std::vector<int32_t> uslot(NI * NE * 4), wslot(NI * NE), soff(NI * stride), ent;
+  std::vector<uint8_t> gprod(G == 1 ? NI * NE : 0);   // one group: the FP32 flux mode's global producers
   if ((*rc = build_trace_plan(NI, NE, op->maxlev, op->h_nlev.data(), op->h_lvl_off.data(), tasks.data(),
                               op->h_nbr.data(), p.ring, p.stage, uslot.data(), wslot.data(), soff.data(), ent,
-                              p.stats)) != LB_OK)
+                              p.stats, G == 1 ? gprod.data() : nullptr)) != LB_OK)
     return nullptr;
   const int slots = p.ring + 2 * p.stage;
   bool ok = slots <= 32767;   // slots travel as 16-bit indices
@@ -1550,7 +1632,7 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
     TracePlan* t;
     ~Guard() {
       if (t)
-        for (DevBuf* b : {&t->d_uslot, &t->d_wslot, &t->d_stage_off, &t->d_stage}) b->release();
+        for (DevBuf* b : {&t->d_uslot, &t->d_wslot, &t->d_stage_off, &t->d_stage, &t->d_gprod}) b->release();
     }
   } guard{&p};
   const std::vector<int16_t> us16(uslot.begin(), uslot.end()), ws16(wslot.begin(), wslot.end());
@@ -1558,6 +1640,7 @@ static TracePlan* trace_plan(lb_op* op, int G, int* rc) {
   if ((*rc = upload(op, p.d_wslot, ws16.data(), ws16.size() * sizeof(int16_t))) != LB_OK) return nullptr;
   if ((*rc = upload(op, p.d_stage_off, soff.data(), soff.size() * sizeof(int32_t))) != LB_OK) return nullptr;
   if ((*rc = upload(op, p.d_stage, ent.data(), ent.size() * sizeof(int32_t))) != LB_OK) return nullptr;
+  if (G == 1 && (*rc = upload(op, p.d_gprod, gprod.data(), gprod.size())) != LB_OK) return nullptr;
   guard.t = nullptr;
   return &op->plans.emplace(G, p).first->second;
 }
@@ -1672,9 +1755,13 @@ static int replay_graph(lb_op* op, const double* q, double* psi, int g0, int G, 
 // from a natural source (out of place), or kOrderBoth, source and flux in
 // sweep order (in place allowed).
 static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int G, const SweepConfig& cfg,
-                           int order = kOrderNatural) {
+                           int order = kOrderNatural, float* psi32 = nullptr) {
   const int mode = cfg.mode, block = cfg.block;
   const bool pos_out = order != kOrderNatural;
+  if (order == kOrderFlux32 && (mode != LB_SWEEP_DIRECTION_LDS || !psi32)) {
+    set_error("sweep: the FP32 flux output needs the LDS sweep and an FP32 vector");
+    return LB_E_ARG;
+  }
   if (pos_out && (G != 1 || (op->batch.P > 0) != (mode == LB_SWEEP_PERSISTENT) || (order == kOrderFlux && q == psi))) {
     set_error("sweep: a sweep-order flux is one group, swept by the persistent kernel on a problem set and by the "
               "other kernels otherwise, out of place unless the source is in sweep order too");
@@ -1732,6 +1819,7 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
   A.vec1 = one_group_vectors(G, psi);
   A.pos_out = pos_out ? 1 : 0;
   A.pos_in = order == kOrderBoth ? 1 : 0;
+  A.psi32 = psi32;
   const bool nt = G >= kStreamGroups;
   if (mode == LB_SWEEP_DIRECTION || mode == LB_SWEEP_DIRECTION_LDS) {
     if (!op->d_lvl_off.p) {
@@ -1752,12 +1840,16 @@ static int launch_sweep_as(lb_op* op, const double* q, double* psi, int g0, int 
       A.uslot = plan->d_uslot.as<const short4>();
       A.wslot = plan->d_wslot.as<const int16_t>();
       const int32_t* so = plan->d_stage_off.as<const int32_t>();
+      A.gprod = plan->d_gprod.as<const uint8_t>();
       A.stage = (pos_out ? plan->d_stage_pos : plan->d_stage).as<const int2>();
       LB_HIP(hipEventRecord(op->ev0, op->stream));
+      const unsigned nd = (unsigned)op->NI;
       if (nt)
-        LB_TRY(launch_direction_lds_nt<true>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, so, A, bytes));
+        LB_TRY((launch_direction_lds_nt<true>(block, nd, op->stream, lo, nl, op->maxlev, so, A, bytes)));
+      else if (order == kOrderFlux32)
+        LB_TRY((launch_direction_lds_nt<false, true>(block, nd, op->stream, lo, nl, op->maxlev, so, A, bytes)));
       else
-        LB_TRY(launch_direction_lds_nt<false>(block, (unsigned)op->NI, op->stream, lo, nl, op->maxlev, so, A, bytes));
+        LB_TRY((launch_direction_lds_nt<false>(block, nd, op->stream, lo, nl, op->maxlev, so, A, bytes)));
       LB_HIP(hipGetLastError());
       LB_HIP(hipEventRecord(op->ev1, op->stream));
       return LB_OK;
@@ -1809,15 +1901,18 @@ int launch_sweep_direction(lb_op* op, const double* q, double* psi, int g0, int 
 // A sweep in the operator's mode; in LB_SWEEP_AUTO the configuration is timed
 // on the first sweeps of each group-batch size (separately for sweep-order
 // output) and then fixed.
-static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, int G, int order) {
-  if (op->sweep_mode != LB_SWEEP_AUTO)
+static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, int G, int order,
+                              float* psi32 = nullptr) {
+  if (op->sweep_mode != LB_SWEEP_AUTO && order != kOrderFlux32)
     return launch_sweep_as(op, q, psi, g0, G, SweepConfig{op->sweep_mode, op->dir_block, op->per_cu}, order);
+  if (op->sweep_mode == LB_SWEEP_DIRECTION_LDS)
+    return launch_sweep_as(op, q, psi, g0, G, SweepConfig{op->sweep_mode, op->dir_block, op->per_cu}, order, psi32);
   const bool batched = op->batch.P > 0;
   const SweepConfig* table = batched ? kBatchTuneConfig : kTuneConfig;
   const int nconf = batched ? kBatchTuneConfigs : kTuneConfigs;
   SweepTune& tn = order != kOrderNatural ? (batched ? op->btune_pos[order] : op->tune_pos[order])
                                          : (batched ? op->btune[G] : op->tune[G]);
-  if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice], order);
+  if (tn.choice >= 0) return launch_sweep_as(op, q, psi, g0, G, table[tn.choice], order, psi32);
   // Timing phase: each configuration sweeps kTuneTrials times (round robin), each
   // sweep timed alone between the operator's events; configurations whose block
   // cannot hold the group batch are skipped, and so is the LDS sweep with many
@@ -1828,14 +1923,15 @@ static int launch_sweep_tuned(lb_op* op, const double* q, double* psi, int g0, i
   // only its configurations are timed there)
   auto skip = [&](const SweepConfig& k) {
     return (k.block != 0 && k.block < G) || (k.mode == LB_SWEEP_DIRECTION_LDS && G >= kStreamGroups) ||
-           (batched && order != kOrderNatural && k.mode != LB_SWEEP_PERSISTENT);
+           (batched && order != kOrderNatural && k.mode != LB_SWEEP_PERSISTENT) ||
+           (order == kOrderFlux32 && k.mode != LB_SWEEP_DIRECTION_LDS);
   };
   int c = tn.trials % nconf;
   while (skip(table[c])) {
     ++tn.trials;
     c = tn.trials % nconf;
   }
-  LB_TRY(launch_sweep_as(op, q, psi, g0, G, table[c], order));
+  LB_TRY(launch_sweep_as(op, q, psi, g0, G, table[c], order, psi32));
   LB_HIP(hipEventSynchronize(op->ev1));
   float ms = 0.f;
   LB_HIP(hipEventElapsedTime(&ms, op->ev0, op->ev1));
@@ -1859,6 +1955,23 @@ int launch_sweep(lb_op* op, const double* q, double* psi, int g0, int G) {
 
 int launch_sweep_pos(lb_op* op, const double* q, double* psi_so, int g0, bool source_in_order) {
   return launch_sweep_tuned(op, q, psi_so, g0, 1, source_in_order ? kOrderBoth : kOrderFlux);
+}
+
+int launch_sweep_f32(lb_op* op, const double* q, double* psi_so, float* psi32_so, int g0) {
+  if (op->batch.P > 0) {
+    set_error("sweep: the FP32 flux output is for single problems");
+    return LB_E_ARG;
+  }
+  if ((reinterpret_cast<uintptr_t>(psi_so) & 15) || (reinterpret_cast<uintptr_t>(psi32_so) & 15)) {
+    set_error("sweep: the FP32 flux output needs 16-byte aligned flux vectors");   // each cell: 16-byte stores
+    return LB_E_ARG;
+  }
+  return launch_sweep_tuned(op, q, psi_so, g0, 1, kOrderFlux32, psi32_so);
+}
+
+bool sweep_order_fp32(const lb_op* op) {
+  return op->flux_order == LB_FLUX_SWEEP_ORDER_FP32 && op->nmom <= 16 && op->moment_fmt == LB_FMT_FP64 &&
+         op->batch.P == 0;
 }
 
 bool sweep_order_flux(const lb_op* op) {
@@ -2009,6 +2122,21 @@ int launch_moments_pos(lb_op* op, const double* psi_so, double* phi) {
 #define CALL(NMV)                                                                                                   \
   hipLaunchKernelGGL(moments_gather_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,     \
                      op->d_D.as<const double>(), (int)op->NI, op->NE, order_inverse(op), psi_so, phi)
+  LB_NM_DISPATCH16(op->nmom, CALL)
+#undef CALL
+  LB_HIP(hipGetLastError());
+  return LB_OK;
+}
+
+int launch_moments_pos32(lb_op* op, const float* psi32_so, double* phi) {
+  if (op->nmom > 16 || !pair_ok(op->NENK, psi32_so, phi)) {
+    set_error("moments: an FP32 sweep-order flux takes at most 16 moments and 16-byte aligned vectors");
+    return LB_E_ARG;
+  }
+  LB_TRY(ensure_inverse(op));
+#define CALL(NMV)                                                                                                   \
+  hipLaunchKernelGGL(moments_gather32_pair_kernel<NMV>, dim3(grid_for(op->NENK / 2)), dim3(kBlock), 0, op->stream,   \
+                     op->d_D.as<const double>(), (int)op->NI, op->NE, order_inverse(op), psi32_so, phi)
   LB_NM_DISPATCH16(op->nmom, CALL)
 #undef CALL
   LB_HIP(hipGetLastError());

@@ -587,6 +587,15 @@ double group_thickness(const lb_op* op, int g) {
 // place on M S in.
 int moment_k_dev(lb_op* op, const double* in, double* out, int g0) {
   const int64_t n = op->NENK * op->NI;
+  if (sweep_order_fp32(op)) {
+    // the flux for D in FP32 (the sweep's arithmetic and its traces stay FP64)
+    LB_TRY(op->w_a2.reserve(n * sizeof(double)));
+    LB_TRY(op->w_so.reserve(n * sizeof(double)));
+    LB_TRY(op->w_so32.reserve(n * sizeof(float)));
+    LB_TRY(launch_moments_to_discrete(op, in, op->w_a2.as<double>(), g0, 1, 0));
+    LB_TRY(launch_sweep_f32(op, op->w_a2.as<const double>(), op->w_so.as<double>(), op->w_so32.as<float>(), g0));
+    return launch_moments_pos32(op, op->w_so32.as<const float>(), out);
+  }
   if (sweep_order_flux(op)) {
     LB_TRY(op->w_so.reserve(n * sizeof(double)));
     double* so = op->w_so.as<double>();

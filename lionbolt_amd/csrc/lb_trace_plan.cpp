@@ -46,10 +46,14 @@ struct Scratch {
 // One direction.  tk: task words of the direction in level order, off: its level
 // offsets (nl + 1).  Writes uslot (4 per position), wslot (1 per position) and
 // appends the staging entries (LDS slot, element) level by level, with
-// soff[l] = first entry of consumer level l (nl + 1 values).
+// soff[l] = first entry of consumer level l (nl + 1 values).  gprod (optional,
+// 1 per position): 1 for a cell whose flux some consumer reads from global
+// memory: through a staging copy, or directly by a consumer with any upstream
+// trace outside LDS (such a consumer reads all its traces from global memory);
+// 0 for one read from LDS only.
 void plan_direction(int64_t NE, int nl, const int32_t* off, const uint32_t* tk, const int32_t* nb, int ring,
                     int stage, int32_t* uslot, int32_t* wslot, int32_t* soff, std::vector<int32_t>& ent,
-                    int64_t* stats, Scratch& s) {
+                    int64_t* stats, Scratch& s, uint8_t* gprod) {
   s.pos.resize(NE);
   s.lev.resize(NE);
   s.start.assign(NE + 1, 0);
@@ -183,6 +187,7 @@ void plan_direction(int64_t NE, int nl, const int32_t* off, const uint32_t* tk, 
     int used = 0;
     for (int32_t p = off[l]; p < off[l + 1]; ++p) {
       const unsigned up = (tk[p] >> 24) & 0xFu;
+      bool miss = false;
       for (int f = 0; f < 4; ++f) {
         int32_t v = -1;
         if (up >> f & 1u) {
@@ -210,7 +215,15 @@ void plan_direction(int64_t NE, int nl, const int32_t* off, const uint32_t* tk, 
           }
         }
         uslot[(int64_t)p * 4 + f] = v;
+        if (up >> f & 1u) {
+          miss |= v < 0;
+          if (gprod && v >= ring) gprod[producer(p, f)] = 1;   // its staging copy is read from global memory
+        }
       }
+      // a cell with any trace outside LDS reads all its upstream traces from global memory
+      if (gprod && miss)
+        for (int f = 0; f < 4; ++f)
+          if (up >> f & 1u) gprod[producer(p, f)] = 1;
       wslot[p] = s.slot[p];
     }
   }
@@ -234,7 +247,8 @@ int trace_plan_sizes(int lds_bytes, int G, int* ring, int* stage) {
 
 int build_trace_plan(int64_t NI, int64_t NE, int maxlev, const int32_t* nlev, const int32_t* lvl_off,
                      const uint32_t* tasks, const int32_t* nbr, int ring, int stage, int32_t* uslot, int32_t* wslot,
-                     int32_t* stage_off, std::vector<int32_t>& entries, int64_t stats[4]) {
+                     int32_t* stage_off, std::vector<int32_t>& entries, int64_t stats[4], uint8_t* gprod) {
+  if (gprod) std::memset(gprod, 0, (size_t)(NI * NE));
   const int64_t stride = maxlev + 2;
   std::vector<std::vector<int32_t>> ent(NI);
   std::vector<int64_t> st(NI * 4, 0);
@@ -245,7 +259,7 @@ int build_trace_plan(int64_t NI, int64_t NE, int maxlev, const int32_t* nlev, co
     for (int64_t i = 0; i < NI; ++i) {
       int32_t* so = stage_off + i * stride;
       plan_direction(NE, nlev[i], lvl_off + i * (maxlev + 1), tasks + i * NE, nbr, ring, stage, uslot + i * NE * 4,
-                     wslot + i * NE, so, ent[i], &st[i * 4], s);
+                     wslot + i * NE, so, ent[i], &st[i * 4], s, gprod ? gprod + i * NE : nullptr);
       for (int l = nlev[i] + 1; l < stride; ++l) so[l] = so[nlev[i]];
     }
   }

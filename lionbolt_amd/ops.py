@@ -32,16 +32,17 @@ def _torch_done():
     torch.cuda.current_stream().synchronize()
 
 
-def _dev(t, need, name):
+def _dev(t, need, name, dtype=None):
     """Device pointer of a tensor handed to a ``lb_*_dev`` entry point, after
-    checking what the library assumes of it: float64, on the GPU, contiguous and
-    holding at least ``need`` entries.  A tensor that fails any check would let
-    the kernels read or write past its end."""
+    checking what the library assumes of it: float64 (or ``dtype``), on the GPU,
+    contiguous and holding at least ``need`` entries.  A tensor that fails any
+    check would let the kernels read or write past its end."""
     import torch
+    dtype = torch.float64 if dtype is None else dtype
     if not isinstance(t, torch.Tensor):
         raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
-    if t.dtype != torch.float64:
-        raise ValueError(f"{name}: expected float64, got {t.dtype}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: expected {dtype}, got {t.dtype}")
     if not t.is_cuda:
         raise ValueError(f"{name}: expected a tensor on the GPU")
     if not t.is_contiguous():
@@ -151,10 +152,11 @@ class Operator:
 
     def set_flux_order(self, order):
         """Layout of one group's angular flux inside the within-group flux-moment solves:
-        "natural", "sweep" (each direction's flux stored in its sweep order) or
-        "sweep_source" (the sweep's source too).  All give bit-identical solutions."""
+        "natural", "sweep" (each direction's flux stored in its sweep order),
+        "sweep_source" (the sweep's source too), which all give bit-identical
+        solutions, or "sweep_fp32" (the Arnoldi steps' flux for D stored in FP32)."""
         codes = {"natural": _lib.FLUX_NATURAL, "sweep": _lib.FLUX_SWEEP_ORDER,
-                 "sweep_source": _lib.FLUX_SWEEP_ORDER_SOURCE}
+                 "sweep_source": _lib.FLUX_SWEEP_ORDER_SOURCE, "sweep_fp32": _lib.FLUX_SWEEP_ORDER_FP32}
         if order not in codes:
             raise ValueError(f"flux order must be one of {sorted(codes)}")
         _lib.check(_lib.lib().lb_op_set_flux_order(self.h, codes[order]))
@@ -265,6 +267,24 @@ class Operator:
         psi_p = _dev(psi_so, self.n, "psi_so")
         _torch_done()
         _lib.check(_lib.lib().lb_moments_to_discrete_order_dev(self.h, phi_p, psi_p, int(g0)))
+        self.synchronize()
+
+    def transport_matinv_order32_dev(self, q, psi_so, psi32_so, g0=0):
+        """One group's T^-1 q with the flux in sweep order as FP32 in psi32_so (float32);
+        psi_so (float64) receives only the cells whose flux is read from global memory."""
+        import torch
+        q_p, so_p = _dev(q, self.n, "q"), _dev(psi_so, self.n, "psi_so")
+        f_p = _dev(psi32_so, self.n, "psi32_so", torch.float32)
+        _torch_done()
+        _lib.check(_lib.lib().lb_transport_matinv_order32_dev(self.h, q_p, so_p, f_p, int(g0)))
+
+    def moments_order32_dev(self, psi32_so, phi):
+        """phi = D psi for one group's FP32 sweep-order flux."""
+        import torch
+        f_p = _dev(psi32_so, self.n, "psi32_so", torch.float32)
+        phi_p = _dev(phi, self.nmom * self.NENK, "phi")
+        _torch_done()
+        _lib.check(_lib.lib().lb_moments_order32_dev(self.h, f_p, phi_p))
         self.synchronize()
 
     def moments_order_dev(self, psi_so, phi):
@@ -417,8 +437,9 @@ def schedule_host(mesh, ang):
 def trace_plan_host(mesh, ang, G, lds_bytes):
     """The LDS slot plan of the direction-resident LDS sweep on the schedule of
     ``schedule_host`` (no GPU needed): a dict with uslot [NI, NE, 4], wslot [NI, NE],
-    stage_off [NI, maxlev + 2], stage [n, 2] (LDS slot, element), ring, stage_slots
-    and stats (ring, staged, one level back, overflow)."""
+    stage_off [NI, maxlev + 2], stage [n, 2] (LDS slot, element), ring, stage_slots,
+    stats (ring, staged, one level back, overflow) and gprod [NI, NE] (1 = the
+    cell's flux is read from global memory by some consumer)."""
     keep = [np.ascontiguousarray(a) for a in (mesh.rg, mesh.node, mesh.nbr, mesh.mat - 1, ang.k, ang.w)]
     rg, node, nbr, mat, k, w = keep
     md = _lib.MeshDesc(mesh.NE, mesh.NK, _lib.dptr(rg), _lib.i64ptr(node), _lib.i64ptr(nbr), _lib.i64ptr(mat), 0)
@@ -430,16 +451,17 @@ def trace_plan_host(mesh, ang, G, lds_bytes):
     soff = np.empty((ang.NI, maxlev + 2), dtype=np.int32)
     sizes = np.empty(2, dtype=np.int32)
     stats = np.empty(4, dtype=np.int64)
+    gprod = np.empty((ang.NI, mesh.NE), dtype=np.uint8)
     n = ctypes.c_int64(0)
     L = _lib.lib()
     _lib.check(L.lb_trace_plan_host(ctypes.byref(md), ctypes.byref(qd), G, lds_bytes, None, None, None, None,
-                                    ctypes.byref(n), None, None))
+                                    ctypes.byref(n), None, None, None))
     stage = np.empty((max(n.value, 1), 2), dtype=np.int32)
     _lib.check(L.lb_trace_plan_host(ctypes.byref(md), ctypes.byref(qd), G, lds_bytes, _lib.i32ptr(uslot),
                                     _lib.i32ptr(wslot), _lib.i32ptr(soff), _lib.i32ptr(stage), ctypes.byref(n),
-                                    _lib.i32ptr(sizes), stats.ctypes.data_as(_lib._i64)))
+                                    _lib.i32ptr(sizes), stats.ctypes.data_as(_lib._i64), gprod.ctypes.data))
     return dict(uslot=uslot, wslot=wslot, stage_off=soff, stage=stage[:n.value], ring=int(sizes[0]),
-                stage_slots=int(sizes[1]), stats=stats, nlev=nlev)
+                stage_slots=int(sizes[1]), stats=stats, nlev=nlev, gprod=gprod)
 
 
 class SpaceAngleVector:

@@ -187,3 +187,64 @@ def test_problem_set_sweep_order_flux():
     assert max(out["sweep"][2]) <= 1e-9
     assert op.sweep_order_choice()[0] == "persistent"
     op.close()
+
+
+def test_fp32_flux_output_is_the_rounded_fp64_sweep():
+    """The FP32 flux mode's sweep: its FP32 output is the FP64 sweep-order flux
+    rounded to nearest, bit for bit, at each LDS workgroup size (the arithmetic
+    and the LDS traces stay FP64; the FP64 flux goes to global memory only for
+    cells read from there, and any such read of a value not written would change
+    the fluxes downstream).  D psi of the FP32 flux equals the FP64 gather on the
+    same values widened to FP64, bit for bit."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(6, a=5.0, seed=1007, slab=(2.0, 3.0))
+    ang = angular.product(4, 8, 3)
+    op = Operator(m, ang)
+    op.set_physics_groups(np.array([[0.4, 1.7]]), np.zeros((1, 2, ang.L + 1)))
+    q = torch.rand(ang.NI * m.NE * 4, dtype=torch.float64, device="cuda")
+    for block in (256, 512, 768):
+        op.set_direction_block(block)
+        op.set_sweep_mode("direction_lds")
+        ref = torch.empty_like(q)
+        op.transport_matinv_order_dev(q, ref, 0)
+        so = torch.full_like(q, float("nan"))
+        f32 = torch.full((q.numel(),), float("nan"), dtype=torch.float32, device="cuda")
+        op.transport_matinv_order32_dev(q, so, f32, 0)
+        op.synchronize()
+        assert torch.equal(f32, ref.float()), block
+    phi32 = torch.empty(op.nmom * op.NENK, dtype=torch.float64, device="cuda")
+    phi64 = torch.empty_like(phi32)
+    op.moments_order32_dev(f32, phi32)
+    op.moments_order_dev(f32.double(), phi64)
+    assert torch.equal(phi32, phi64)
+    op.close()
+
+
+def test_fp32_flux_mode_multigroup_solve():
+    """The multigroup flux-moment GMRES with the Arnoldi steps' flux in FP32: the
+    true residual (FP64 sweeps) meets the tolerance and the moments agree with the
+    FP64 solve to well within it."""
+    import torch
+    from lionbolt_amd.ops import Operator
+    m = M.jittered_cube(5, a=4.0, seed=1021, slab=(1.5, 2.5))
+    ang = angular.product(4, 8, 3)
+    G = 3
+    sigt, S = problems.downscatter_xs(G=G, L=ang.L)
+    op = Operator(m, ang)
+    op.set_physics_groups(sigt, problems.within_group(S))
+    nphi = op.nmom * op.NENK
+    g = torch.Generator(device="cuda").manual_seed(11)
+    qmom = torch.rand(G * nphi, dtype=torch.float64, device="cuda", generator=g)
+    out = {}
+    for order in ("sweep", "sweep_fp32"):
+        op.set_flux_order(order)
+        phi = torch.empty_like(qmom)
+        stats = op.multigroup_solve_dev(S, phi, qmom=qmom, kind="gmres", space="moment", m=30, rtol=1e-6)
+        out[order] = (phi.view(G, -1).cpu().numpy(), stats)
+    for g_ in range(G):
+        a, b = out["sweep_fp32"][0][g_], out["sweep"][0][g_]
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), g_
+        assert out["sweep_fp32"][1][g_]["true_residual"] <= 1e-6, g_
+        assert abs(out["sweep_fp32"][1][g_]["iterates"] - out["sweep"][1][g_]["iterates"]) <= 1, g_
+    op.close()

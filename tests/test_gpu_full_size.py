@@ -12,7 +12,8 @@ Through properties that hold at any size:
   direction-resident kernel and the LDS kernel (its slot plan at this size
   reads 98.4 % of the upstream traces from ring slots, 0.7 % from staging
   copies and 1 % from global memory, so every trace path runs), in the natural
-  layout and with the flux stored in sweep order (and D psi gathered from it);
+  layout and with the flux stored in sweep order (and D psi gathered from it),
+  and the FP32 flux mode's output is that flux rounded to FP32;
 * three- and 24-group sweeps equal single-group sweeps bit for bit (24 groups
   is the benchmark's sweep, streaming loads and graph-replayed level launches);
 * a two-group downscatter solve with the beam source reaches a true relative
@@ -147,7 +148,15 @@ def test_full_size_sweep_linearity_and_modes(full):
     op.moments_dev(a, phi_a)
     op.moments_order_dev(so, phi_b)
     assert torch.equal(phi_a, phi_b)
-    del so, nat, phi_a, phi_b
+    # the FP32 flux mode's sweep: the rounded FP64 flux, its staged and global trace reads
+    # served by the FP64 flux of exactly the cells the slot plan marks
+    f32 = torch.full((n,), float("nan"), dtype=torch.float32, device="cuda")
+    nat.fill_(float("nan"))
+    op.transport_matinv_order32_dev(q1, nat, f32, 0)
+    op.synchronize()
+    assert torch.equal(f32, so.float())
+    print("full size: FP32 flux sweep done", flush=True)
+    del so, nat, phi_a, phi_b, f32
     op.set_sweep_mode("direction_lds")
     b = torch.empty_like(q1)
     op.transport_matinv_dev(q2, b, 0, 1)

@@ -8,7 +8,10 @@ stands for the workgroup's LDS and records which element's flux each slot holds:
   and staging copies write staging slots of the consumer level's parity only;
 * a staged producer lies two or more levels before its consumer (its global
   store is ordered by an earlier barrier);
-* the traces are counted once each in the plan's statistics.
+* the traces are counted once each in the plan's statistics;
+* with the FP64 flux stored only for the cells the plan marks (the FP32 flux
+  mode), every read from global memory finds its producer stored: the staging
+  copies, and all traces of a cell with any trace outside LDS.
 
 The upstream faces and levels come from the normals and schedule of
 lb_schedule_host, recomputed here with the reference's rule (k.n < -EPSILON).
@@ -43,11 +46,15 @@ def replay(m, ang, G, lds_bytes):
         off = np.searchsorted(lv, np.arange(nlev[i] + 1))
         us, ws, so = p["uslot"][i], p["wslot"][i], p["stage_off"][i]
         lds = np.full(ring + 2 * S, -1, np.int64)
+        stored = np.zeros(m.NE, bool)   # FP64 flux in global memory, FP32 flux mode
+        gp = p["gprod"][i]
         for l in range(nlev[i]):
             pos = np.arange(off[l], off[l + 1])
             read = set()
             for s in pos:
                 e = order[s]
+                if np.any(us[s][up[e]] < 0):   # read from global memory, every upstream trace
+                    assert np.all(stored[nbr[e][up[e]]]), (i, l, e)
                 for f in range(4):
                     v = us[s, f]
                     if not up[e, f]:
@@ -65,6 +72,7 @@ def replay(m, ang, G, lds_bytes):
             w = w[w >= 0]
             assert np.all(w < ring) and len(np.unique(w)) == len(w)
             lds[w] = order[pos][ws[pos] >= 0]
+            stored[order[pos][gp[pos] != 0]] = True
             ent = p["stage"][so[l + 1]:so[l + 2]]          # copies made during level l for level l + 1
             # waves of one level run unordered: no slot is both read and written in it
             assert not read & (set(w.tolist()) | set(ent[:, 0].tolist()))
@@ -73,6 +81,7 @@ def replay(m, ang, G, lds_bytes):
                 assert np.all((ent[:, 0] >= ring + ((l + 1) & 1) * S) & (ent[:, 0] < ring + ((l + 1) & 1) * S + S))
                 assert len(np.unique(ent[:, 0])) == len(ent)
                 assert np.all(lev[ent[:, 1]] <= l - 1)
+                assert np.all(stored[ent[:, 1]])
                 lds[ent[:, 0]] = ent[:, 1]
         total += int(up.sum())
     assert p["stats"].sum() == total
@@ -88,6 +97,7 @@ def test_ample_lds_reads_every_trace_from_the_ring(small):
     m, ang = small
     p = replay(m, ang, 1, 163840)
     assert p["stats"][0] == p["stats"].sum()
+    assert not p["gprod"].any()
 
 
 @pytest.mark.parametrize("G,slots", [(1, 64), (1, 96), (1, 128), (2, 100), (4, 48)])
@@ -96,6 +106,7 @@ def test_tight_lds_plan_is_consistent(small, G, slots):
     p = replay(m, ang, G, slots * 32 * G)
     ringr, staged, late, over = p["stats"]
     assert staged > 0 and ringr > 0
+    assert 0 < p["gprod"].mean() < 1
     assert ringr + staged > 0.5 * p["stats"].sum()
 
 
