@@ -159,9 +159,11 @@ FLUX_LAYOUTS = ("sweep", "sweep_source", "natural", "sweep_fp32")
 DEFAULT_FLUX_LAYOUT = FLUX_LAYOUTS[0]
 
 
-def solve_leg(m, ang, rtol, group_batch=1):
+def solve_leg(m, ang, rtol, group_batch=1, layout=DEFAULT_FLUX_LAYOUT, others=True):
     """The 384,000-tet multigroup downscatter solve with the +z beam first-collision source;
-    ``group_batch`` consecutive groups per coupled solve (1 = group by group)."""
+    ``group_batch`` consecutive groups per coupled solve (1 = group by group), the within-group
+    flux in ``layout`` (one group at a time only); ``others`` adds the same solve in the other
+    layouts and with reduced-precision D psi."""
     import torch
     from lionbolt_amd import angular, problems
     from lionbolt_amd.ops import Operator
@@ -171,6 +173,8 @@ def solve_leg(m, ang, rtol, group_batch=1):
     op.set_physics_groups(sigt, problems.within_group(S))
     op.set_group_batch(group_batch)
     tune_s = settle_autotuner(op, m, ang, group_batch)
+    if group_batch == 1:
+        op.set_flux_order(layout)
     qmom, umom = _beam_qmom(op, m, sigt, S)
     phi = torch.zeros_like(qmom)
     torch.cuda.synchronize()
@@ -181,10 +185,11 @@ def solve_leg(m, ang, rtol, group_batch=1):
     choice = list(op.sweep_choice(group_batch))
     order_choice = {"flux": list(op.sweep_order_choice()), "source_and_flux": list(op.sweep_order_choice(True))}
     layouts = {}
-    if group_batch == 1:
-        # the same solve with the other layouts of the within-group angular flux (bit-identical moments)
+    if group_batch == 1 and others:
+        # the same solve with the other layouts of the within-group angular flux (bit-identical
+        # moments between the FP64 ones)
         for order in FLUX_LAYOUTS:
-            if order == DEFAULT_FLUX_LAYOUT:
+            if order == layout:
                 continue
             op.set_flux_order(order)
             phn = torch.zeros_like(qmom)
@@ -193,7 +198,7 @@ def solve_leg(m, ang, rtol, group_batch=1):
             sl = op.multigroup_solve_dev(S, phn, qmom=qmom, kind="gmres", space="moment", m=30, rtol=rtol)
             torch.cuda.synchronize()
             layouts[order] = {"seconds": time.perf_counter() - t1, "moments_bit_identical": bool(torch.equal(phn, phi))}
-            if order == "sweep_fp32":
+            if "sweep_fp32" in (order, layout):
                 # the Arnoldi steps' flux for D in FP32: its error against the FP64 solve
                 layouts[order].update({
                     "sweeps": sum(x["sweeps"] for x in sl), "iterates": [x["iterates"] for x in sl],
@@ -201,13 +206,13 @@ def solve_leg(m, ang, rtol, group_batch=1):
                     "scalar_flux_max_rel_diff_vs_fp64":
                         float(((phn[:, 0] - phi[:, 0]).abs().amax(dim=1) / phi[:, 0].abs().amax(dim=1)).max())})
             del phn
-        op.set_flux_order(DEFAULT_FLUX_LAYOUT)
+        op.set_flux_order(layout)
     # scalar flux = sqrt(4 pi) x the l = 0 moment; group 1 adds its uncollided flux (Main.f90:443)
     s4pi = math.sqrt(4.0 * math.pi)
     flux = {"collided_sum_all_groups": float(phi[:, 0].sum()) * s4pi,
             "uncollided_sum_group1": float(umom[0].sum()) * s4pi}
     reduced = {}
-    if group_batch == 1 and ang.NI == 256 and ang.L == 3:
+    if group_batch == 1 and others and ang.NI == 256 and ang.L == 3:
         # D psi of the Arnoldi steps on MFMA with reduced inputs (FP32 accumulation);
         # right-hand side, psi and the true residual in FP64
         ref = phi[:, 0].clone()
@@ -230,7 +235,7 @@ def solve_leg(m, ang, rtol, group_batch=1):
     max_res = max(s["true_residual"] for s in stats)
     return {"seconds": wall, "groups": G, "rtol": rtol, "autotune_s": tune_s, "sweep_choice": choice,
             "sweep_order_choice": order_choice,
-            "flux_layout": DEFAULT_FLUX_LAYOUT if group_batch == 1 else "natural", "other_flux_layouts": layouts,
+            "flux_layout": layout if group_batch == 1 else "natural", "other_flux_layouts": layouts,
             "max_true_residual": max_res, "target_residual": 1e-6, "target_met": bool(max_res <= 1e-6),
             "iterates": [s["iterates"] for s in stats], "sweeps": sum(s["sweeps"] for s in stats),
             "group_batch": group_batch, "scalar_flux": flux, "reduced_precision_D_psi": reduced,
@@ -727,7 +732,8 @@ def run(args):
         out["high_order_operator"] = high_order_leg()
         _note("scattering orders 3-15 done")
     if args.solve:
-        runs = [solve_leg(m, ang, args.solve_rtol, b) for b in args.group_batches]
+        runs = [solve_leg(m, ang, args.solve_rtol, b, args.flux_layout, args.solve_variants)
+                for b in args.group_batches]
         _note("24-group solve done")
         out["solve"] = min(runs, key=lambda r: r["seconds"])
         out["solve_by_group_batch"] = {str(r["group_batch"]): {k: r[k] for k in ("seconds", "sweeps",
@@ -774,6 +780,10 @@ def main():
     p.add_argument("--groups", type=int, default=24)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--no-solve", dest="solve", action="store_false")
+    p.add_argument("--flux-layout", choices=FLUX_LAYOUTS, default=DEFAULT_FLUX_LAYOUT,
+                   help="layout of the within-group flux in the timed solve (default: the library's)")
+    p.add_argument("--no-solve-variants", dest="solve_variants", action="store_false",
+                   help="time only the solve in --flux-layout (no other layouts, no reduced-precision D psi)")
     p.add_argument("--no-single", dest="single", action="store_false")
     p.add_argument("--single-modes", type=lambda t: t.split(","), default=None,
                    help="comma-separated one-group sweep variants to run (default: all)")
