@@ -105,11 +105,11 @@ def test_autotuned_sweeps_equal_level_launches(G):
     ref = ref.cpu().numpy()
     op.set_sweep_mode("auto")
     assert op.sweep_choice(G) == ("auto", 0, 0)
-    for _ in range(16):     # two timed sweeps for each of the six configurations, then settled ones
+    for _ in range(16):     # two timed sweeps for each of the seven configurations, then settled ones
         psi = torch.full_like(q, float("nan"))
         op.transport_matinv_dev(q, psi, 0, G)
         op.synchronize()
         assert np.array_equal(psi.cpu().numpy(), ref)
     mode, threads, _ = op.sweep_choice(G)
     assert (mode, threads) in {("graph", 0), ("direction", 256), ("direction", 512), ("direction", 1024),
-                               ("direction_lds", 256), ("direction_lds", 512)}
+                               ("direction_lds", 256), ("direction_lds", 512), ("direction_lds", 768)}
