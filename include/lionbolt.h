@@ -405,7 +405,15 @@ int lb_moments_to_discrete_dev(lb_op* op, const double* d_phi, double* d_psi, in
  * follows the persistent sweep's problem-major order and applies when the
  * sweep mode is LB_SWEEP_AUTO or LB_SWEEP_PERSISTENT (the batched sweep of the
  * 64 uneven beams: 11.0 -> 8.0 ms).  Other operators keep the natural layout
- * whatever is set here. */
+ * whatever is set here.  LB_FLUX_SWEEP_ORDER_FP32 (single problems only; an
+ * operator with a problem set runs LB_FLUX_SWEEP_ORDER instead) stores the
+ * Arnoldi steps' flux for D psi in FP32: the sweep is always the LDS kernel
+ * (at the operator's workgroup size in LB_SWEEP_DIRECTION_LDS, otherwise the
+ * fastest of its sizes, timed on the first sweeps), its arithmetic and LDS
+ * traces FP64, its FP32 flux the FP64 flux rounded to nearest; the right-hand
+ * side T^-1 q, the true residual and the multigroup tail stay FP64.  Not
+ * bit-identical to the FP64 layouts (the 24-group solve: 1.07-1.09 s, scalar
+ * flux within 6e-9 of the FP64 solve, same iterate counts). */
 enum { LB_FLUX_NATURAL = 0, LB_FLUX_SWEEP_ORDER = 1, LB_FLUX_SWEEP_ORDER_SOURCE = 2, LB_FLUX_SWEEP_ORDER_FP32 = 3 };
 int lb_op_set_flux_order(lb_op* op, int order);
 
